@@ -1,0 +1,618 @@
+// capi.hip -- the C-ABI of include/aladyn_gpu.h: context, device memory,
+// host<->device transfers in the reference's Fortran layouts and the step
+// orchestration of lpf2_evolve (pic_evolve_in_time.f90:37-109).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/aladyn_gpu.h"
+#include "lp_kernels.hpp"
+#include "tile_store.hpp"
+
+using namespace aladyn;
+
+struct SpeciesF64 {
+  double* part = nullptr;  // part(cap, 2D+1) on the device
+  double* tmp = nullptr;   // compaction target, same shape
+  int64_t cap = 0, np = 0;
+};
+
+struct aladyn_ctx {
+  aladyn_config cfg{};
+  DevGrid g{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  int dev = 0;
+  int ncol = 0;
+  bool have_fields = false;
+  // FP64 reference-layout state
+  double* ebf64 = nullptr;
+  double* jc64 = nullptr;
+  SpeciesF64 sp64[ALADYN_MAX_SPECIES];
+  int* d_blk = nullptr;
+  int64_t blk_cap = 0;
+  // FP32 tile-path state
+  float* ebf32 = nullptr;
+  float* jc32 = nullptr;
+  TileStore ts[ALADYN_MAX_SPECIES];
+  TileGeom tg{};
+  float* patch = nullptr;  // per-tile J partial sums
+  bool j_is_zero = true;
+  // small device scalars: [0] ndead (int), [8..] int64 count
+  int* d_ndead = nullptr;
+  int64_t* d_count = nullptr;
+  double* d_red = nullptr;      // reduction outputs (16 doubles)
+  double* h_red = nullptr;      // pinned mirror
+  // diagnostics (continuity)
+  bool diag = false;
+  double* rho0 = nullptr;
+  double* rho1 = nullptr;
+  void* jraw = nullptr;
+  bool have_resid = false;
+  // timing
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double part_ms = 0, field_ms = 0;
+  int64_t n_timed = 0;
+};
+
+static int fail(aladyn_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCK(ctx, call)                                                                  \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(ctx, ALADYN_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                    \
+  } while (0)
+
+static bool f64(const aladyn_ctx* c) { return c->cfg.precision == ALADYN_PREC_F64; }
+
+// ----------------------------------------------------------------- set-up
+static int setup_grid(aladyn_ctx* c) {
+  const aladyn_config& a = c->cfg;
+  DevGrid& g = c->g;
+  std::memset(&g, 0, sizeof g);
+  g.ndim = a.ndim;
+  g.nfield = a.ndim == 3 ? 6 : 3;
+  g.curr_ndim = a.ndim == 3 ? 3 : 2;
+  g.nx = a.nx;
+  g.ny = a.ny;
+  g.nz = a.ndim == 3 ? a.nz : 1;
+  // array_alloc.f90:62-69: n1p = nxp + ihx with nxp = nx + 2, ihx = 3
+  g.n1p = g.nx + 5;
+  g.n2p = g.ny + 5;
+  g.n3p = a.ndim == 3 ? g.nz + 5 : 1;
+  g.ix1 = 3;
+  g.ix2 = g.nx + 2;
+  g.jy1 = 3;
+  g.jy2 = g.ny + 2;
+  g.kz1 = a.ndim == 3 ? 3 : 1;
+  g.kz2 = a.ndim == 3 ? g.nz + 2 : 1;
+  g.ibx = a.ibx;
+  g.iby = a.iby;
+  g.ibz = a.ibz;
+  g.ncell = (int64_t)g.n1p * g.n2p * g.n3p;
+  // set_grid (set_grid_param.f90:68-71, 107-113, 158-163)
+  g.dx = a.dx;
+  g.dx_inv = 1.0 / g.dx;
+  g.dy = a.dy;
+  g.dy_inv = 1.0 / g.dy;
+  g.dz = a.ndim == 3 ? a.dz : 1.0;
+  g.dz_inv = a.ndim == 3 ? 1.0 / g.dz : 1.0;
+  g.xmn = 0.0;
+  g.xmx = g.dx * (double)g.nx;
+  g.ymn = g.dy * (double)(-(g.ny / 2));
+  g.ymx = g.dy * (double)(g.ny - g.ny / 2);
+  g.zmn = a.ndim == 3 ? g.dz * (double)(-(g.nz / 2)) : 0.0;
+  g.zmx = a.ndim == 3 ? g.dz * (double)(g.nz - g.nz / 2) : 0.0;
+  g.dt = a.dt;
+  g.ompe = a.ompe;
+  // djc (set_init_param.f90:92-102)
+  if (a.ndim == 2) {
+    g.djc[0] = 0.5 * g.dx;
+    g.djc[1] = 0.5 * g.dy;
+    g.djc[2] = 0.0;
+  } else {
+    g.djc[0] = g.dx / 3.;
+    g.djc[1] = g.dy / 3.;
+    g.djc[2] = g.dz / 3.;
+  }
+  // se_coeff (grid_field_param.f90:36-99)
+  const double yx_rat = g.dy / g.dx;
+  const double nu = a.cfl * yx_rat / std::sqrt(yx_rat * yx_rat + (double)a.ndim - 1.);
+  if (a.der_ord == 3) {
+    g.se[0] = 1. + 0.125 * (1. - nu * nu);
+    g.se[1] = (1. - g.se[0]) / 3.;
+  } else if (a.der_ord == 4) {
+    g.se[0] = 1.125;
+    g.se[1] = (1. - g.se[0]) / 3.;
+  } else {
+    g.se[0] = 1.;
+    g.se[1] = 0.;
+  }
+  g.dgvol = a.ndim == 2 ? g.dx * g.dy * g.dy : g.dx * g.dy * g.dz;
+  return 0;
+}
+
+extern "C" int aladyn_create(const aladyn_config* cfg, aladyn_ctx** out) {
+  if (!cfg || !out) return ALADYN_EINVAL;
+  *out = nullptr;
+  aladyn_ctx* c = new aladyn_ctx();
+  c->cfg = *cfg;
+  const aladyn_config& a = c->cfg;
+  auto bad = [&](const char* m) {
+    *out = c;  // keep the context so the caller can read the message
+    return fail(c, ALADYN_EINVAL, "%s", m);
+  };
+  if (a.ndim != 2 && a.ndim != 3) return bad("ndim must be 2 or 3");
+  if (a.nx < 4 || a.ny < 4 || (a.ndim == 3 && a.nz < 4)) return bad("grid too small (>= 4 cells per axis)");
+  if (a.ibx != 0 || a.iby != 0 || (a.ndim == 3 && a.ibz != 0))
+    return bad("only open boundaries (ibx = iby = ibz = 0) are supported");
+  if (a.der_ord != 2 && a.der_ord != 3 && a.der_ord != 4) return bad("der_ord must be 2, 3 or 4");
+  if (a.precision != ALADYN_PREC_F64 && a.precision != ALADYN_PREC_F32)
+    return bad("precision must be 64 or 32");
+  if (a.nsp < 1 || a.nsp > ALADYN_MAX_SPECIES) return bad("nsp must be 1..4");
+  if (!(a.dx > 0) || !(a.dy > 0) || (a.ndim == 3 && !(a.dz > 0)) || !(a.dt > 0))
+    return bad("dx, dy, dz, dt must be positive");
+  // |v| < c must keep every particle within one cell per step (the
+  // Esirkepov window of grid_part_connect.f90:1008 holds 5 slots)
+  if (a.dt >= a.dx || a.dt >= a.dy || (a.ndim == 3 && a.dt >= a.dz))
+    return bad("dt must be smaller than every cell size (one-cell-per-step limit)");
+  setup_grid(c);
+  c->ncol = 2 * c->g.curr_ndim + 1;
+  *out = c;
+  HIPCK(c, hipGetDevice(&c->dev));
+  HIPCK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& e : c->ev) HIPCK(c, hipEventCreate(&e));
+  const DevGrid& g = c->g;
+  const size_t nf = (size_t)g.ncell * g.nfield, nj = (size_t)g.ncell * g.curr_ndim;
+  if (f64(c)) {
+    HIPCK(c, hipMalloc(&c->ebf64, nf * sizeof(double)));
+    HIPCK(c, hipMalloc(&c->jc64, nj * sizeof(double)));
+    HIPCK(c, hipMemsetAsync(c->ebf64, 0, nf * sizeof(double), c->stream));
+    HIPCK(c, hipMemsetAsync(c->jc64, 0, nj * sizeof(double), c->stream));
+  } else {
+    HIPCK(c, hipMalloc(&c->ebf32, nf * sizeof(float)));
+    HIPCK(c, hipMalloc(&c->jc32, nj * sizeof(float)));
+    HIPCK(c, hipMemsetAsync(c->ebf32, 0, nf * sizeof(float), c->stream));
+    HIPCK(c, hipMemsetAsync(c->jc32, 0, nj * sizeof(float), c->stream));
+    int r = tile_geom_init(g, a, &c->tg);
+    if (r) return fail(c, ALADYN_EINVAL, "%s", tile_geom_error(r));
+    const size_t npatch = (size_t)c->tg.ntiles * c->tg.pvol * g.curr_ndim;
+    HIPCK(c, hipMalloc(&c->patch, npatch * sizeof(float)));
+  }
+  HIPCK(c, hipMalloc(&c->d_ndead, 64));
+  HIPCK(c, hipMalloc(&c->d_count, 64));
+  HIPCK(c, hipMalloc(&c->d_red, 32 * sizeof(double)));
+  HIPCK(c, hipHostMalloc(&c->h_red, 32 * sizeof(double), hipHostMallocDefault));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_destroy(aladyn_ctx* c) {
+  if (!c) return ALADYN_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->ebf64);
+  (void)hipFree(c->jc64);
+  (void)hipFree(c->ebf32);
+  (void)hipFree(c->jc32);
+  (void)hipFree(c->patch);
+  for (auto& s : c->sp64) {
+    (void)hipFree(s.part);
+    (void)hipFree(s.tmp);
+  }
+  for (auto& t : c->ts) tile_store_free(&t);
+  (void)hipFree(c->d_blk);
+  (void)hipFree(c->d_ndead);
+  (void)hipFree(c->d_count);
+  (void)hipFree(c->d_red);
+  if (c->h_red) (void)hipHostFree(c->h_red);
+  (void)hipFree(c->rho0);
+  (void)hipFree(c->rho1);
+  (void)hipFree(c->jraw);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return ALADYN_OK;
+}
+
+extern "C" const char* aladyn_last_error(const aladyn_ctx* c) {
+  return c ? c->err.c_str() : "null context";
+}
+
+// ---------------------------------------------------------------- species
+static int check_species(aladyn_ctx* c, int32_t ic) {
+  if (!c) return ALADYN_EINVAL;
+  if (ic < 0 || ic >= c->cfg.nsp) return fail(c, ALADYN_EINVAL, "species %d out of range", ic);
+  return 0;
+}
+
+extern "C" int aladyn_put_species(aladyn_ctx* c, int32_t ic, const double* part, int64_t ld,
+                                  int64_t np) {
+  if (int r = check_species(c, ic)) return r;
+  if (np < 0 || (np > 0 && (!part || ld < np)))
+    return fail(c, ALADYN_EINVAL, "put_species: need part != NULL and ld >= np");
+  const DevGrid& g = c->g;
+  const int D = g.curr_ndim, ncol = c->ncol;
+  // positions must be in the particle box (x > gmin, x <= gmax): every
+  // kernel index is derived from them, so this is the bounds guarantee
+  const double mn[3] = {g.xmn, g.ymn, g.zmn}, mx[3] = {g.xmx, g.ymx, g.zmx};
+  for (int d = 0; d < D; d++)
+    for (int64_t n = 0; n < np; n++) {
+      const double x = part[d * ld + n];
+      if (!(x > mn[d] && x <= mx[d]))
+        return fail(c, ALADYN_ERANGE, "particle %lld coordinate %d = %g outside (%g, %g]",
+                    (long long)n, d + 1, x, mn[d], mx[d]);
+    }
+  for (int d = D; d < 2 * D; d++)
+    for (int64_t n = 0; n < np; n++)
+      if (!std::isfinite(part[d * ld + n]))
+        return fail(c, ALADYN_EINVAL, "particle %lld momentum is not finite", (long long)n);
+  if (f64(c)) {
+    SpeciesF64& s = c->sp64[ic];
+    if (np > s.cap) {
+      (void)hipFree(s.part);
+      (void)hipFree(s.tmp);
+      s.part = s.tmp = nullptr;
+      s.cap = np > 0 ? np : 1;
+      HIPCK(c, hipMalloc(&s.part, (size_t)s.cap * ncol * sizeof(double)));
+      HIPCK(c, hipMalloc(&s.tmp, (size_t)s.cap * ncol * sizeof(double)));
+    }
+    for (int col = 0; col < ncol && np > 0; col++)
+      HIPCK(c, hipMemcpyAsync(s.part + col * s.cap, part + col * ld, (size_t)np * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
+    s.np = np;
+    const int64_t nb = nblk_compact(s.cap);
+    if (nb > c->blk_cap) {
+      (void)hipFree(c->d_blk);
+      HIPCK(c, hipMalloc(&c->d_blk, (size_t)(nb + 1) * sizeof(int)));
+      c->blk_cap = nb;
+    }
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return ALADYN_OK;
+  }
+  std::string msg;
+  int r = tile_store_put(&c->ts[ic], c->tg, g, part, ld, np, c->stream, c->cfg.reserved[0] != 0,
+                         &msg);
+  if (r) return fail(c, r, "%s", msg.c_str());
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_species_count(aladyn_ctx* c, int32_t ic, int64_t* np) {
+  if (int r = check_species(c, ic)) return r;
+  if (!np) return fail(c, ALADYN_EINVAL, "np is NULL");
+  if (f64(c)) {
+    *np = c->sp64[ic].np;
+    return ALADYN_OK;
+  }
+  std::string msg;
+  int r = tile_store_count(&c->ts[ic], c->tg, c->stream, np, &msg);
+  if (r) return fail(c, r, "%s", msg.c_str());
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_get_species(aladyn_ctx* c, int32_t ic, double* part, int64_t ld,
+                                  int64_t* np) {
+  if (int r = check_species(c, ic)) return r;
+  int64_t n = 0;
+  if (int r = aladyn_species_count(c, ic, &n)) return r;
+  if (np) *np = n;
+  if (n > 0 && (!part || ld < n))
+    return fail(c, ALADYN_EINVAL, "get_species: ld (%lld) < particle count (%lld)",
+                (long long)ld, (long long)n);
+  if (f64(c)) {
+    SpeciesF64& s = c->sp64[ic];
+    for (int col = 0; col < c->ncol && n > 0; col++)
+      HIPCK(c, hipMemcpyAsync(part + col * ld, s.part + col * s.cap, (size_t)n * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return ALADYN_OK;
+  }
+  std::string msg;
+  int r = tile_store_get(&c->ts[ic], c->tg, c->g, part, ld, c->stream, &msg);
+  if (r) return fail(c, r, "%s", msg.c_str());
+  return ALADYN_OK;
+}
+
+// ----------------------------------------------------------------- fields
+template <typename T>
+static int put_field_array(aladyn_ctx* c, T* dst, const double* src, int64_t n) {
+  if (sizeof(T) == sizeof(double)) {
+    HIPCK(c, hipMemcpyAsync(dst, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));  // the host array is only borrowed
+  } else {
+    std::vector<float> h((size_t)n);
+    for (int64_t i = 0; i < n; i++) h[i] = (float)src[i];
+    HIPCK(c, hipMemcpyAsync(dst, h.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice,
+                            c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+template <typename T>
+static int get_field_array(aladyn_ctx* c, double* dst, const T* src, int64_t n) {
+  if (sizeof(T) == sizeof(double)) {
+    HIPCK(c, hipMemcpyAsync(dst, src, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> h((size_t)n);
+    HIPCK(c, hipMemcpyAsync(h.data(), src, (size_t)n * sizeof(float), hipMemcpyDeviceToHost,
+                            c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < n; i++) dst[i] = (double)h[i];
+  }
+  return 0;
+}
+
+extern "C" int aladyn_put_fields(aladyn_ctx* c, const double* ebf) {
+  if (!c || !ebf) return fail(c, ALADYN_EINVAL, "put_fields: NULL argument");
+  const int64_t n = c->g.ncell * c->g.nfield;
+  for (int64_t i = 0; i < n; i++)
+    if (!std::isfinite(ebf[i])) return fail(c, ALADYN_EINVAL, "put_fields: non-finite value");
+  int r = f64(c) ? put_field_array(c, c->ebf64, ebf, n) : put_field_array(c, c->ebf32, ebf, n);
+  if (r) return r;
+  c->have_fields = true;
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_get_fields(aladyn_ctx* c, double* ebf) {
+  if (!c || !ebf) return fail(c, ALADYN_EINVAL, "get_fields: NULL argument");
+  const int64_t n = c->g.ncell * c->g.nfield;
+  return f64(c) ? get_field_array(c, ebf, c->ebf64, n) : get_field_array(c, ebf, c->ebf32, n);
+}
+
+extern "C" int aladyn_get_current(aladyn_ctx* c, double* jc) {
+  if (!c || !jc) return fail(c, ALADYN_EINVAL, "get_current: NULL argument");
+  const int64_t n = c->g.ncell * c->g.curr_ndim;
+  return f64(c) ? get_field_array(c, jc, c->jc64, n) : get_field_array(c, jc, c->jc32, n);
+}
+
+// ----------------------------------------------------------------- stages
+extern "C" int aladyn_pfields_prepare(aladyn_ctx* c) {
+  if (!c) return ALADYN_EINVAL;
+  if (f64(c))
+    field_xyzbd<double>(c->g, c->ebf64, c->g.nfield, c->stream);
+  else
+    field_xyzbd<float>(c->g, c->ebf32, c->g.nfield, c->stream);
+  HIPCK(c, hipGetLastError());
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_curr_clean(aladyn_ctx* c) {
+  if (!c) return ALADYN_EINVAL;
+  const int64_t n = c->g.ncell * c->g.curr_ndim;
+  if (f64(c))
+    fill<double>(c->jc64, n, 0.0, c->stream);
+  else
+    fill<float>(c->jc32, n, 0.0f, c->stream);
+  c->j_is_zero = true;
+  HIPCK(c, hipGetLastError());
+  return ALADYN_OK;
+}
+
+static int ensure_diag_buffers(aladyn_ctx* c) {
+  const size_t nc = (size_t)c->g.ncell;
+  const size_t esz = f64(c) ? sizeof(double) : sizeof(float);
+  if (!c->rho0) HIPCK(c, hipMalloc(&c->rho0, nc * sizeof(double)));
+  if (!c->rho1) HIPCK(c, hipMalloc(&c->rho1, nc * sizeof(double)));
+  if (!c->jraw) HIPCK(c, hipMalloc(&c->jraw, nc * c->g.curr_ndim * esz));
+  return 0;
+}
+
+// rho(ncell) += grid charge of species ic (device FP64)
+static int grid_charge_dev(aladyn_ctx* c, int ic, double* rho) {
+  if (f64(c)) {
+    const SpeciesF64& s = c->sp64[ic];
+    grid_charge_f64(c->g, s.part, s.cap, s.np, rho, c->stream);
+  } else {
+    tile_store_grid_charge(&c->ts[ic], c->tg, c->g, rho, c->stream);
+  }
+  HIPCK(c, hipGetLastError());
+  return 0;
+}
+
+extern "C" int aladyn_particle_stage(aladyn_ctx* c, int32_t ic, int32_t initial_time) {
+  if (int r = check_species(c, ic)) return r;
+  if (!c->have_fields) return fail(c, ALADYN_ESTATE, "particle_stage before put_fields");
+  const DevGrid& g = c->g;
+  const double alp = 0.5 * g.dt * c->cfg.lorentz_fact[ic];
+  if (c->diag) {
+    if (int r = ensure_diag_buffers(c)) return r;
+    fill<double>(c->rho0, g.ncell, 0.0, c->stream);
+    if (int r = grid_charge_dev(c, ic, c->rho0)) return r;
+    // jraw = jc before this species' deposit; the species' raw current is
+    // recovered as jc(after) - jraw below
+    const size_t esz = f64(c) ? sizeof(double) : sizeof(float);
+    HIPCK(c, hipMemcpyAsync(c->jraw, f64(c) ? (void*)c->jc64 : (void*)c->jc32,
+                            (size_t)g.ncell * g.curr_ndim * esz, hipMemcpyDeviceToDevice,
+                            c->stream));
+  }
+  HIPCK(c, hipEventRecord(c->ev[0], c->stream));
+  if (f64(c)) {
+    SpeciesF64& s = c->sp64[ic];
+    if (s.np > 0) {
+      HIPCK(c, hipMemsetAsync(c->d_ndead, 0, sizeof(int), c->stream));
+      particles_f64(g, s.part, s.cap, s.np, c->ebf64, c->jc64, alp, initial_time ? 1 : 0,
+                    c->d_ndead, c->stream);
+      HIPCK(c, hipGetLastError());
+      int ndead = 0;
+      HIPCK(c, hipMemcpyAsync(&ndead, c->d_ndead, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCK(c, hipEventRecord(c->ev[1], c->stream));
+      HIPCK(c, hipStreamSynchronize(c->stream));
+      if (ndead > 0) {
+        compact_f64(g, s.part, s.tmp, s.cap, s.np, c->d_blk, c->d_count, c->stream);
+        HIPCK(c, hipGetLastError());
+        std::swap(s.part, s.tmp);
+        s.np -= ndead;
+      }
+    } else {
+      HIPCK(c, hipEventRecord(c->ev[1], c->stream));
+    }
+  } else {
+    std::string msg;
+    int r = tile_store_step(&c->ts[ic], c->tg, g, c->ebf32, c->jc32, c->patch, (float)alp,
+                            initial_time ? 1 : 0, c->j_is_zero, c->stream, &msg);
+    if (r) return fail(c, r, "%s", msg.c_str());
+    HIPCK(c, hipEventRecord(c->ev[1], c->stream));
+  }
+  c->j_is_zero = false;
+  if (c->diag) {
+    fill<double>(c->rho1, g.ncell, 0.0, c->stream);
+    if (int r = grid_charge_dev(c, ic, c->rho1)) return r;
+    c->have_resid = true;
+  }
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_curr_mpi_collect(aladyn_ctx* c) {
+  if (!c) return ALADYN_EINVAL;
+  if (f64(c))
+    curr_collect<double>(c->g, c->jc64, c->stream);
+  else
+    curr_collect<float>(c->g, c->jc32, c->stream);
+  HIPCK(c, hipGetLastError());
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_advance_lpf_fields(aladyn_ctx* c) {
+  if (!c) return ALADYN_EINVAL;
+  if (f64(c))
+    advance_fields<double>(c->g, c->ebf64, c->jc64, c->stream);
+  else
+    advance_fields<float>(c->g, c->ebf32, c->jc32, c->stream);
+  HIPCK(c, hipGetLastError());
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_lpf2_evolve(aladyn_ctx* c, int32_t iter, int32_t initial_time) {
+  (void)iter;
+  if (!c) return ALADYN_EINVAL;
+  if (!c->have_fields) return fail(c, ALADYN_ESTATE, "lpf2_evolve before put_fields");
+  if (int r = aladyn_pfields_prepare(c)) return r;
+  if (int r = aladyn_curr_clean(c)) return r;
+  for (int ic = 0; ic < c->cfg.nsp; ic++)
+    if (int r = aladyn_particle_stage(c, ic, initial_time)) return r;
+  HIPCK(c, hipEventRecord(c->ev[2], c->stream));
+  if (int r = aladyn_curr_mpi_collect(c)) return r;
+  if (int r = aladyn_advance_lpf_fields(c)) return r;
+  HIPCK(c, hipEventRecord(c->ev[3], c->stream));
+  return ALADYN_OK;
+}
+
+// ------------------------------------------------------------ diagnostics
+extern "C" int aladyn_energy_momenta(aladyn_ctx* c, int32_t ic, double ek[7], double* ekmax) {
+  if (int r = check_species(c, ic)) return r;
+  HIPCK(c, hipMemsetAsync(c->d_red, 0, 8 * sizeof(double), c->stream));
+  if (f64(c)) {
+    const SpeciesF64& s = c->sp64[ic];
+    energy_momenta_f64(c->g, s.part, s.cap, s.np, c->d_red, c->stream);
+  } else {
+    tile_store_energy(&c->ts[ic], c->tg, c->g, c->d_red, c->stream);
+  }
+  HIPCK(c, hipGetLastError());
+  HIPCK(c, hipMemcpyAsync(c->h_red, c->d_red, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 7; i++) ek[i] = c->h_red[i];
+  if (ekmax) *ekmax = c->h_red[7];
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_field_energy(aladyn_ctx* c, double out[6]) {
+  if (!c || !out) return fail(c, ALADYN_EINVAL, "field_energy: NULL argument");
+  HIPCK(c, hipMemsetAsync(c->d_red, 0, 8 * sizeof(double), c->stream));
+  if (f64(c))
+    field_energy<double>(c->g, c->ebf64, c->d_red, c->stream);
+  else
+    field_energy<float>(c->g, c->ebf32, c->d_red, c->stream);
+  HIPCK(c, hipGetLastError());
+  HIPCK(c, hipMemcpyAsync(c->h_red, c->d_red, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 6; i++) out[i] = i < c->g.nfield ? c->g.dgvol * c->h_red[i] : 0.0;
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_grid_charge(aladyn_ctx* c, int32_t ic, double* rho) {
+  if (int r = check_species(c, ic)) return r;
+  if (!rho) return fail(c, ALADYN_EINVAL, "grid_charge: NULL rho");
+  if (int r = ensure_diag_buffers(c)) return r;
+  const int64_t n = c->g.ncell;
+  HIPCK(c, hipMemcpyAsync(c->rho1, rho, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (int r = grid_charge_dev(c, ic, c->rho1)) return r;
+  HIPCK(c, hipMemcpyAsync(rho, c->rho1, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  c->have_resid = false;  // rho1 was reused
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_set_diagnostics(aladyn_ctx* c, int32_t enable) {
+  if (!c) return ALADYN_EINVAL;
+  c->diag = enable != 0;
+  c->have_resid = false;
+  return ALADYN_OK;
+}
+
+// The raw current of the last stage is jc - jraw (jraw holds jc before it).
+template <typename T>
+__global__ void k_sub(T* a, const T* b, int64_t n) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    a[t] = b[t] - a[t];
+}
+
+extern "C" int aladyn_continuity_residual(aladyn_ctx* c, double out[2]) {
+  if (!c || !out) return fail(c, ALADYN_EINVAL, "continuity_residual: NULL argument");
+  if (!c->have_resid)
+    return fail(c, ALADYN_ESTATE, "continuity_residual needs set_diagnostics(1) then a particle_stage");
+  const int64_t n = c->g.ncell * c->g.curr_ndim;
+  const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+  HIPCK(c, hipMemsetAsync(c->d_red, 0, 2 * sizeof(double), c->stream));
+  if (f64(c)) {
+    k_sub<double><<<nb, 256, 0, c->stream>>>((double*)c->jraw, c->jc64, n);
+    continuity<double>(c->g, c->rho0, c->rho1, (const double*)c->jraw, c->d_red, c->stream);
+  } else {
+    k_sub<float><<<nb, 256, 0, c->stream>>>((float*)c->jraw, c->jc32, n);
+    continuity<float>(c->g, c->rho0, c->rho1, (const float*)c->jraw, c->d_red, c->stream);
+  }
+  HIPCK(c, hipGetLastError());
+  HIPCK(c, hipMemcpyAsync(c->h_red, c->d_red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  out[0] = c->h_red[0];
+  out[1] = c->h_red[1];
+  c->have_resid = false;
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_synchronize(aladyn_ctx* c) {
+  if (!c) return ALADYN_EINVAL;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_kernel_timing(aladyn_ctx* c, int32_t reset, double* particle_ms,
+                                    double* field_ms, int64_t* launches) {
+  if (!c) return ALADYN_EINVAL;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  float pm = 0, fm = 0;
+  if (hipEventElapsedTime(&pm, c->ev[0], c->ev[1]) != hipSuccess) pm = 0;
+  if (hipEventElapsedTime(&fm, c->ev[2], c->ev[3]) != hipSuccess) fm = 0;
+  if (particle_ms) *particle_ms = pm;
+  if (field_ms) *field_ms = fm;
+  if (launches) *launches = c->n_timed;
+  if (reset) c->n_timed = 0;
+  return ALADYN_OK;
+}

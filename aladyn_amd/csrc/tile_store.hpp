@@ -1,0 +1,96 @@
+// tile_store.hpp -- the FP32 particle store of the tile path.
+//
+// Particles of one species live in SoA arrays grouped by tile: tile t owns
+// the segment [seg_start[t], seg_start[t] + seg_cap[t]) of which the first
+// count[t] slots are live.  A particle is stored as
+//   cell  int32  packed nearest-node indices (i | j << bx | k << (bx+by)),
+//                Fortran node numbering of ebf (interior starts at 3)
+//   s[d]  FP32   offset from that node, in cells, in [-1/2, 1/2)
+//   p[d]  FP32   momentum p/(mc)
+//   w     FP32   weight (the FP32 `wgh` of the packed slot)
+// so x = xmn + (i - 3 + s_x)*dx exactly as the reference's local coordinate
+// xx = 3 + (x - xmn)/dx = i + s_x (grid_part_lib.f90:114-116); a moving-window
+// shift is an integer subtraction.  Two such buffers are used in turn: the
+// particle kernel reads the tile's segment from one and writes every
+// surviving particle into the segment of its new tile in the other (slots
+// reserved with per-workgroup aggregated atomics), so the store stays
+// tile-sorted every step at the cost of one read and one write per particle.
+// Particles that find their destination segment full go to an overflow list
+// at the end of the buffer, which a global-memory kernel processes the next
+// step; a re-layout (counting sort with fresh capacities) runs every
+// sort_every steps and whenever the overflow grows.
+#pragma once
+
+#include <string>
+
+#include "../../include/aladyn_gpu.h"
+#include "lp_common.hpp"
+
+namespace aladyn {
+
+struct TileGeom {
+  int T[3];    // tile extent in nodes (T[2] = 1 in 2-D)
+  int nt[3];   // tiles per axis
+  int ntiles;
+  int G[3];    // field patch extent: T + 2 (nodes T0-1 .. T0+T)
+  int J[3];    // current patch extent: T + 4 (nodes T0-2 .. T0+T+1)
+  int gvol, pvol;
+  int bx, by;  // bit widths of i and j in the packed cell
+  int block;   // threads per workgroup
+  int sort_every;
+  int lds_bytes;
+};
+
+struct RecPtr {
+  int* cell;
+  float* s[3];
+  float* p[3];
+  float* w;
+  int* id;  // optional original index (tests), nullptr when not tracked
+};
+
+struct TileStore {
+  int64_t alloc = 0;     // records per buffer
+  int64_t ovf_base = 0;  // first record of the overflow region
+  int64_t ovf_cap = 0;
+  void* mem[2] = {nullptr, nullptr};
+  RecPtr buf[2] = {};
+  int cur = 0;
+  int* seg_start = nullptr;  // ntiles + 1
+  int* seg_cap = nullptr;    // ntiles
+  int* count[2] = {nullptr, nullptr};
+  int* ovf_count = nullptr;  // [2]
+  int* scratch = nullptr;    // ntiles + 1 ints for scans / fills
+  int64_t* d_total = nullptr;
+  int charge = 0, part_ind = 0;
+  bool track_ids = false;
+  bool loaded = false;
+  int64_t np = 0;  // particles at the last put / re-layout
+  int steps_since_sort = 0;
+  int* h_ovf = nullptr;  // pinned: overflow count read back lazily
+  hipEvent_t ovf_ev = nullptr;
+};
+
+int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg);
+const char* tile_geom_error(int code);
+
+int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const double* part,
+                   int64_t ld, int64_t np, hipStream_t s, bool track_ids, std::string* msg);
+int tile_store_count(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t* np,
+                     std::string* msg);
+int tile_store_get(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* part, int64_t ld,
+                   hipStream_t s, std::string* msg);
+// One species-loop body: gather + push + box test + deposit + re-binning.
+// jc(raw) = (j_is_zero ? 0 : jc) + this species' Esirkepov current.
+int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const float* ebf,
+                    float* jc, float* patch, float alp, int initial, bool j_is_zero,
+                    hipStream_t s, std::string* msg);
+int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
+                        std::string* msg);
+void tile_store_grid_charge(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* rho,
+                            hipStream_t s);
+void tile_store_energy(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* out,
+                       hipStream_t s);
+void tile_store_free(TileStore* ts);
+
+}  // namespace aladyn

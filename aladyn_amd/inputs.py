@@ -1,0 +1,188 @@
+"""Seeded synthetic inputs for the LP step: plasma particles and laser fields.
+
+The reference builds its initial state in part_distribute / lp_pulse
+(src/IO/init_part_distrib.f90:2475-2553, src/IO/init_laser_field.f90:34-103),
+placing particles at sub-cell centres and drawing thermal momenta from an
+unseeded generator.  Parity runs need identical seeded inputs on every path,
+so the state is generated here instead (numpy PCG64 from a given seed) in the
+reference's layouts:
+
+    part(np, 2*curr_ndim+1) float64, column-major: x, y[, z], px, py[, pz], W
+    ebf(n1p, n2p, n3p, nfield) float64, column-major
+
+W packs int16 charge, int16 part_ind and the FP32 weight as in
+src/work/precision_def.F90:33-40.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .config import PI, LPConfig
+
+
+def pack_weight(charge: int, part_ind: int, wgh) -> np.ndarray:
+    """Packs (int16 charge, int16 part_ind, float32 wgh) into float64 slots."""
+    wgh = np.asarray(wgh, dtype=np.float32)
+    rec = np.zeros(wgh.shape, dtype=[("c", "<i2"), ("p", "<i2"), ("w", "<f4")])
+    rec["c"] = charge
+    rec["p"] = part_ind
+    rec["w"] = wgh
+    return rec.view("<f8").reshape(wgh.shape)
+
+
+def unpack_weight(w: np.ndarray):
+    rec = np.ascontiguousarray(w, dtype="<f8").view([("c", "<i2"), ("p", "<i2"), ("w", "<f4")])
+    return rec["c"].astype(np.int32), rec["p"].astype(np.int32), rec["w"].astype(np.float32)
+
+
+def uniform_plasma(cfg: LPConfig, ppc, *, x_start_cell: float = 0.0, x_end_cell: float | None = None,
+                   y_cells=None, z_cells=None, seed: int = 0, jitter: bool = True,
+                   charge: int = -1, n_rel: float = 1.0, thermal: float = 0.0,
+                   ramp_cells: float = 0.0) -> np.ndarray:
+    """Electrons at n0_ref*n_rel on a ppc sub-grid per cell.
+
+    Each particle sits at its sub-cell centre (init_part_distrib.f90:513-516)
+    plus, if jitter, a uniform U(-1/2, 1/2) displacement in sub-cell units.
+    The weight is n_rel*n0_ref/ppc (init_part_distrib.f90:501-503); with
+    ramp_cells > 0 the weight rises linearly over the first ramp_cells cells.
+    thermal > 0 draws momenta from N(0, thermal) per component.
+    Returns part(np, 2D+1), column-major.
+    """
+    rng = np.random.default_rng(seed)
+    D = cfg.curr_ndim
+    ppc = tuple(int(v) for v in ppc)[:D]
+    x1 = cfg.nx if x_end_cell is None else x_end_cell
+    ylo, yhi = (0, cfg.ny) if y_cells is None else y_cells
+    cells_x = np.arange(int(math.ceil(x_start_cell)), int(x1))
+    cells_y = np.arange(ylo, yhi)
+    axes_cells = [cells_x, cells_y]
+    if D == 3:
+        zlo, zhi = (0, cfg.nz) if z_cells is None else z_cells
+        axes_cells.append(np.arange(zlo, zhi))
+    # sub-cell centres in cell units along each axis, local coordinate
+    axes = []
+    for d in range(D):
+        sub = (np.arange(ppc[d]) + 0.5) / ppc[d]
+        axes.append((axes_cells[d][:, None] + sub[None, :]).ravel())
+    grids = np.meshgrid(*axes, indexing="ij")
+    loc = [g.ravel(order="F") for g in grids]  # x fastest: cell-ordered
+    n = loc[0].size
+    mins = (cfg.xmn, cfg.ymn, cfg.zmn)
+    sizes = (cfg.dx, cfg.dy, cfg.dz)
+    part = np.zeros((n, cfg.ncol), dtype=np.float64, order="F")
+    for d in range(D):
+        u = loc[d]
+        if jitter:
+            u = u + (rng.random(n) - 0.5) / ppc[d]
+        # keep strictly inside (0, n] in local units
+        top = (cfg.nx, cfg.ny, cfg.nz)[d]
+        u = np.clip(u, 1e-6, top - 1e-6)
+        part[:, d] = mins[d] + u * sizes[d]
+    if thermal > 0:
+        for d in range(D):
+            part[:, D + d] = rng.normal(0.0, thermal, n)
+    wgh = np.full(n, n_rel * cfg.n0_ref / float(np.prod(ppc)), dtype=np.float64)
+    if ramp_cells > 0:
+        xl = (part[:, 0] - cfg.xmn) / cfg.dx - x_start_cell
+        wgh *= np.clip(xl / ramp_cells, 0.0, 1.0)
+    part[:, 2 * D] = pack_weight(charge, 0, wgh.astype(np.float32))
+    return part
+
+
+def gaussian_laser(cfg: LPConfig, *, a0: float, w0: float, tau_fwhm: float, xc: float,
+                   lam0: float | None = None, polarization: str = "y") -> np.ndarray:
+    """Linearly polarised Gaussian pulse, paraxial leading order, at its focus.
+
+    E_y = B_z = a0*omega0 * exp(-r^2/w0^2) * exp(-(x-xc)^2/L^2) * cos(k0 (x-xc))
+    with the envelope length L from the intensity FWHM tau_fwhm (laser
+    periods), plus the longitudinal E_x that keeps div E = 0 to leading order.
+    Staggering follows the Yee layout of init_grid_fields.f90:1025-1029:
+    Ex(i+1/2,j,k), Ey(i,j+1/2,k), Ez(i,j,k+1/2), Bx(i,j+1/2,k+1/2),
+    By(i+1/2,j,k+1/2), Bz(i+1/2,j+1/2,k).  xc, w0 in micron.
+    """
+    lam = cfg.lam0 if lam0 is None else lam0
+    om = 2.0 * PI / lam
+    amp = a0 * om
+    tau = tau_fwhm * lam  # FWHM of the intensity in micron (c = 1)
+    sig = tau / math.sqrt(2.0 * math.log(2.0))  # exp(-(x/sig)^2) envelope of E
+    ebf = np.zeros(cfg.field_shape, dtype=np.float64, order="F")
+
+    def coord(n, mn, d, half):
+        idx = np.arange(1, n + 1, dtype=np.float64)
+        return mn + (idx - 3.0 + (0.5 if half else 0.0)) * d
+
+    x_i = coord(cfg.n1p, cfg.xmn, cfg.dx, False)
+    x_h = coord(cfg.n1p, cfg.xmn, cfg.dx, True)
+    y_i = coord(cfg.n2p, cfg.ymn, cfg.dy, False)
+    y_h = coord(cfg.n2p, cfg.ymn, cfg.dy, True)
+    if cfg.ndim == 3:
+        z_i = coord(cfg.n3p, cfg.zmn, cfg.dz, False)
+        z_h = coord(cfg.n3p, cfg.zmn, cfg.dz, True)
+    else:
+        z_i = z_h = np.zeros(1)
+
+    def envelope(x, y, z):
+        X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+        r2 = Y * Y + Z * Z
+        xi = X - xc
+        return X, Y, Z, np.exp(-r2 / (w0 * w0)) * np.exp(-(xi / sig) ** 2), xi
+
+    if cfg.ndim == 2:
+        _, _, _, env, xi = envelope(x_i, y_h, z_i)   # Ey(i, j+1/2)
+        ebf[:, :, :, 1] = amp * env * np.cos(om * xi)
+        _, _, _, env, xi = envelope(x_h, y_h, z_i)   # Bz(i+1/2, j+1/2)
+        ebf[:, :, :, 2] = amp * env * np.cos(om * xi)
+        _, Y, _, env, xi = envelope(x_h, y_i, z_i)   # Ex(i+1/2, j)
+        ebf[:, :, :, 0] = amp * env * (2.0 * Y / (w0 * w0)) * np.sin(om * xi) / om
+    else:
+        _, _, _, env, xi = envelope(x_i, y_h, z_i)   # Ey(i, j+1/2, k)
+        ebf[:, :, :, 1] = amp * env * np.cos(om * xi)
+        _, _, _, env, xi = envelope(x_h, y_h, z_i)   # Bz(i+1/2, j+1/2, k)
+        ebf[:, :, :, 5] = amp * env * np.cos(om * xi)
+        _, Y, _, env, xi = envelope(x_h, y_i, z_i)   # Ex(i+1/2, j, k)
+        ebf[:, :, :, 0] = amp * env * (2.0 * Y / (w0 * w0)) * np.sin(om * xi) / om
+        _, _, Z, env, xi = envelope(x_i, y_h, z_h)   # Bx(i, j+1/2, k+1/2)
+        ebf[:, :, :, 3] = amp * env * (2.0 * Z / (w0 * w0)) * np.sin(om * xi) / om
+    if polarization != "y":
+        raise ValueError("only y polarisation is generated")
+    return ebf
+
+
+def lwfa_case(cfg: LPConfig, *, ppc, a0: float, w0: float, tau_fwhm: float, xc: float,
+              x_start_cell: float = 0.0, seed: int = 0, thermal: float = 0.0,
+              ramp_cells: float = 0.0):
+    """Plasma (x > x_start_cell) plus a Gaussian pulse: (part, ebf)."""
+    part = uniform_plasma(cfg, ppc, x_start_cell=x_start_cell, seed=seed, thermal=thermal,
+                          ramp_cells=ramp_cells)
+    ebf = gaussian_laser(cfg, a0=a0, w0=w0, tau_fwhm=tau_fwhm, xc=xc)
+    return part, ebf
+
+
+# ----------------------------------------------------------- named workloads
+def lwfa_2d_config(nx=512, ny=256) -> LPConfig:
+    """2-D Cartesian LWFA: dx = dy = 0.05, n/nc = 0.01, cfl 0.9."""
+    return LPConfig(ndim=2, nx=nx, ny=ny, k0=20.0, yx_rat=1.0, cfl=0.9, n_over_nc=0.01)
+
+
+def lwfa_3d_config(nx=256, ny=128, nz=128, n_over_nc=0.005) -> LPConfig:
+    """3-D LWFA: dx = 0.05, dy = dz = 0.2, cfl 0.9."""
+    return LPConfig(ndim=3, nx=nx, ny=ny, nz=nz, k0=20.0, yx_rat=4.0, zx_rat=4.0, cfl=0.9,
+                    n_over_nc=n_over_nc)
+
+
+def lwfa_2d_case(nx=512, ny=256, seed=0):
+    """512x256, 2x2 ppc for x > 64 cells, a0 = 2, w0 = 8, tau 6 periods at x = 32 cells."""
+    cfg = lwfa_2d_config(nx, ny)
+    part, ebf = lwfa_case(cfg, ppc=(2, 2), a0=2.0, w0=8.0, tau_fwhm=6.0, xc=32 * cfg.dx,
+                          x_start_cell=nx / 8, seed=seed)
+    return cfg, part, ebf
+
+
+def lwfa_3d_case(nx=256, ny=128, nz=128, seed=0, ppc=(2, 2, 2)):
+    """256x128x128, n/nc = 0.005, 8 ppc, a0 = 3, w0 = 10 (pulse centred in x/4)."""
+    cfg = lwfa_3d_config(nx, ny, nz)
+    part, ebf = lwfa_case(cfg, ppc=ppc, a0=3.0, w0=10.0, tau_fwhm=6.0, xc=nx * cfg.dx / 4,
+                          x_start_cell=0.0, seed=seed)
+    return cfg, part, ebf

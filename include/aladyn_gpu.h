@@ -1,0 +1,149 @@
+/*
+ * aladyn_gpu.h -- C-ABI of the MI355X (gfx950) implementation of ALaDyn's
+ * laser-plasma PIC step (lpf2_evolve).
+ *
+ * The reference has no plugin API: the hot path is plain Fortran module calls
+ * inside lpf2_evolve (src/dynamics/pic_evolve_in_time.f90:43-107).  Each entry
+ * point below replaces one of those calls (or the whole step) and is bound
+ * from Fortran with bind(C) interfaces (aladyn_amd/fortran/aladyn_gpu.f90).
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a nonzero ALADYN_E* code
+ *    otherwise; aladyn_last_error() returns the message.  The Fortran shim
+ *    copies a nonzero code into `ier` so the reference's exit path
+ *    (ALaDyn.F90:99-102) fires.
+ *  - Host arrays are borrowed only during a call and use the reference's own
+ *    Fortran layouts (column-major):
+ *      part(ld, 2*curr_ndim+1)    x,y[,z],px,py[,pz],W       (struct_def.f90:26-28)
+ *      ebf(n1p, n2p, n3p, nfield) Ex,Ey,[Ez,Bx,By,]Bz        (fstruct_data.f90:27)
+ *      jc(n1p, n2p, n3p, curr_ndim)                          (fstruct_data.f90:28)
+ *    with n1p = nx+5, n2p = ny+5, n3p = nz+5 (1 in 2-D) (array_alloc.f90:56-80)
+ *    and the interior at Fortran indices ix1 = 3 .. ix2 = nx+2 (C index 2..).
+ *    W is the packed FP64 slot of precision_def.F90:33-40 (int16 charge,
+ *    int16 part_ind, FP32 weight).
+ *  - Device buffers belong to the context.  All work is queued on the
+ *    context's HIP stream; get_* and the reductions synchronise it.
+ *  - One host thread per context.  Contexts are independent.
+ */
+#ifndef ALADYN_GPU_H
+#define ALADYN_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALADYN_OK 0
+#define ALADYN_EINVAL 1   /* bad argument or unsupported configuration */
+#define ALADYN_EHIP 2     /* HIP runtime error */
+#define ALADYN_ENOMEM 3   /* device allocation failed */
+#define ALADYN_ESTATE 4   /* call out of order (e.g. step before put) */
+#define ALADYN_ERANGE 5   /* particle outside the box at put_species */
+
+#define ALADYN_PREC_F64 64 /* reference layout, FP64 throughout */
+#define ALADYN_PREC_F32 32 /* tile-sorted SoA, int32 cell + FP32 offset */
+
+#define ALADYN_MAX_SPECIES 4 /* spec(4) in pstruct_data.f90:39 */
+
+/* Run constants of one single-rank sub-domain, named as in common_param.f90
+ * and grid_param.f90.  The derived quantities the reference computes in
+ * set_grid / set_initial_param / set_field_param (indices, inverses, djc,
+ * se_coeff, box limits) are computed by aladyn_create from these. */
+typedef struct aladyn_config {
+  int32_t ndim;          /* 2 or 3 (2-D is P-polarised: nfield 3, curr_ndim 2) */
+  int32_t nx, ny, nz;    /* interior cells (nz ignored in 2-D) */
+  int32_t ibx, iby, ibz; /* boundary types; 0 (open) is supported */
+  int32_t der_ord;       /* 2, 3 or 4 */
+  int32_t precision;     /* ALADYN_PREC_F64 or ALADYN_PREC_F32 */
+  int32_t nsp;           /* species run per step (nsp_run), 1..4 */
+  double dx, dy, dz;     /* cell sizes (dz ignored in 2-D) */
+  double dt;             /* dt_loc */
+  double cfl;            /* only used by der_ord 3 */
+  double ompe;           /* 4*pi*rc0*n0_ref*1e-3 (set_init_param.f90:254-255) */
+  double lorentz_fact[ALADYN_MAX_SPECIES]; /* 1/m per species */
+  /* FP32 tile path tuning (0 = default / autotuned) */
+  int32_t tile[3];       /* tile extent in nodes per axis */
+  int32_t block;         /* threads per workgroup of the particle kernel */
+  int32_t sort_every;    /* steps between full re-layouts of the tile store */
+  int32_t reserved[5];
+} aladyn_config;
+
+typedef struct aladyn_ctx aladyn_ctx;
+
+/* Creates a context on the current HIP device.  Replaces the allocation of
+ * ebf/jc/ebfp in v_alloc (array_alloc.f90:56-125) for the device copy. */
+int aladyn_create(const aladyn_config *cfg, aladyn_ctx **out);
+int aladyn_destroy(aladyn_ctx *ctx);
+const char *aladyn_last_error(const aladyn_ctx *ctx);
+
+/* Copies species ic (0-based) in: the counterpart of filling spec(ic)%part
+ * (part_distribute, init_part_distrib.f90:2475-2553).  Positions must lie in
+ * the particle box (loc_*grid%gmin, gmax] (ALADYN_ERANGE otherwise). */
+int aladyn_put_species(aladyn_ctx *ctx, int32_t ic, const double *part,
+                       int64_t ld, int64_t np);
+/* Number of particles of species ic (loc_npart, mpi_var.F90:35). */
+int aladyn_species_count(aladyn_ctx *ctx, int32_t ic, int64_t *np);
+/* Copies species ic back into part(ld, 2*curr_ndim+1); *np gets the count
+ * (which shrinks when particles leave the box, cell_part_dist,
+ * mpi_part_interface.f90:525-753).  ld must be >= the count. */
+int aladyn_get_species(aladyn_ctx *ctx, int32_t ic, double *part, int64_t ld,
+                       int64_t *np);
+/* Field and current transfer in the ebf / jc layouts above. */
+int aladyn_put_fields(aladyn_ctx *ctx, const double *ebf);
+int aladyn_get_fields(aladyn_ctx *ctx, double *ebf);
+int aladyn_get_current(aladyn_ctx *ctx, double *jc);
+
+/* One lpf2_evolve(iter) (pic_evolve_in_time.f90:37-109) with part = .true.:
+ * pfields_prepare, jc = 0, per species {set_lpf_acc, field_charge_multiply,
+ * [init_lpf_momenta], lpf_momenta_and_positions, cell_part_dist,
+ * curr_accumulate}, curr_mpi_collect, advance_lpf_fields.  initial_time != 0
+ * applies the first-step momentum back-shift (ALaDyn.F90:67). */
+int aladyn_lpf2_evolve(aladyn_ctx *ctx, int32_t iter, int32_t initial_time);
+
+/* The stages of the step, for drivers that interleave their own work:
+ * pfields_prepare(ebf, nfield, 2, 2) (curr_and_fields_util.f90:156-174) */
+int aladyn_pfields_prepare(aladyn_ctx *ctx);
+/* jc(:,:,:,:) = 0 (pic_evolve_in_time.f90:73) */
+int aladyn_curr_clean(aladyn_ctx *ctx);
+/* Species loop body for species ic (pic_evolve_in_time.f90:77-89): gather
+ * (set_lpf_acc), field_charge_multiply, init_lpf_momenta if initial_time,
+ * lpf_momenta_and_positions, cell_part_dist(.false.), curr_accumulate
+ * (Esirkepov, iform 0).  Fused into one particle kernel. */
+int aladyn_particle_stage(aladyn_ctx *ctx, int32_t ic, int32_t initial_time);
+/* curr_mpi_collect(jc) on one rank (curr_and_fields_util.f90:104-152). */
+int aladyn_curr_mpi_collect(aladyn_ctx *ctx);
+/* advance_lpf_fields(ebf, jc, 0) (curr_and_fields_util.f90:176-260). */
+int aladyn_advance_lpf_fields(aladyn_ctx *ctx);
+
+/* energy_momenta for species ic (diag_part_and_fields.f90:172-219), FP64
+ * device reduction: ek[0..6] = ek(1:7), *ekmax = max(gamma-1). */
+int aladyn_energy_momenta(aladyn_ctx *ctx, int32_t ic, double ek[7],
+                          double *ekmax);
+/* Field energy per component as in Envar (diag_part_and_fields.f90:499-670,
+ * unstretched): out[c] = dgvol * sum_interior ebf(c)^2, FP64 reduction. */
+int aladyn_field_energy(aladyn_ctx *ctx, double out[6]);
+/* set_grid_charge for species ic (grid_part_util.f90:240-335), accumulated in
+ * FP64 on the device and copied into rho(n1p, n2p, n3p) (added to it). */
+int aladyn_grid_charge(aladyn_ctx *ctx, int32_t ic, double *rho);
+/* Discrete continuity residual of the last particle_stage on the device in
+ * FP64: r = (rho^{n+1} - rho^n) + (1/curr_ndim) sum_d D_d J_d(raw) on the
+ * interior, with rho from the particle positions before and after the stage
+ * and J the raw (pre-djc) current of that stage.  out[0] = max|r|,
+ * out[1] = max|rho^{n+1} - rho^n|.  Requires aladyn_set_diagnostics(ctx, 1)
+ * before the stage (it keeps rho^n and the raw current). */
+int aladyn_continuity_residual(aladyn_ctx *ctx, double out[2]);
+int aladyn_set_diagnostics(aladyn_ctx *ctx, int32_t enable);
+
+/* Waits for all queued work of the context. */
+int aladyn_synchronize(aladyn_ctx *ctx);
+/* Kernel/launch bookkeeping for measurement: average duration (ms) of the
+ * particle kernel over the steps since the last reset, measured with HIP
+ * events on the context stream. */
+int aladyn_kernel_timing(aladyn_ctx *ctx, int32_t reset, double *particle_ms,
+                         double *field_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

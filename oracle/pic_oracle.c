@@ -1,0 +1,1077 @@
+/*
+ * pic_oracle.c -- CPU FP64 restatement of ALaDyn's LP PIC step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library, never part of
+ * the product path.  Each routine cites the reference routine it restates and
+ * keeps the reference's operation order, so that compiled without FMA
+ * contraction (-ffp-contract=off) it reproduces the reference's rounding.
+ * Array layouts and the packed weight slot are described in pic_oracle.h.
+ */
+#include "pic_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Fortran ebf(i,j,k,c) / jc(i,j,k,c) with 1-based indices. */
+#define F4(a, i, j, k, c)                                                    \
+  (a)[((((int64_t)(c)-1) * g->n3p + ((k)-1)) * (int64_t)g->n2p + ((j)-1)) * \
+          (int64_t)g->n1p +                                                  \
+      ((i)-1)]
+/* Fortran part(n,c) with 0-based n and 1-based column c. */
+#define PC(a, n, c) (a)[((int64_t)(c)-1) * ld + (n)]
+
+static inline int imin(int a, int b) { return a < b ? a : b; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+
+void orc_grid_setup(orc_grid *g, double cfl) {
+  /* set_grid (set_grid_param.f90:52-225): x(i) = dx*(i-1),
+   * y(i) = dy*(i-1-ny/2), z likewise; interior index 3..n+2 (code_util.f90:37,
+   * set_fyzxgrid p_ind = min(sh, n_loc) .. n_loc+p_ind-1). */
+  if (g->ndim == 2) g->nz = 1;
+  g->nfield = g->ndim == 3 ? 6 : 3;
+  g->curr_ndim = g->ndim == 3 ? 3 : 2;
+  g->n1p = g->nx + 5;
+  g->n2p = g->ny + 5;
+  g->n3p = g->ndim == 3 ? g->nz + 5 : 1;
+  g->ix1 = 3;
+  g->ix2 = g->nx + 2;
+  g->jy1 = 3;
+  g->jy2 = g->ny + 2;
+  if (g->ndim == 3) {
+    g->kz1 = 3;
+    g->kz2 = g->nz + 2;
+  } else {
+    g->kz1 = 1;
+    g->kz2 = 1;
+  }
+  g->dx_inv = 1.0 / g->dx;
+  g->dy_inv = 1.0 / g->dy;
+  g->dz_inv = g->ndim == 3 ? 1.0 / g->dz : 1.0;
+  if (g->ndim == 2) g->dz = 1.0;
+  g->xmn = 0.0;
+  g->xmx = g->dx * (double)g->nx;
+  g->ymn = g->dy * (double)(-(g->ny / 2));
+  g->ymx = g->dy * (double)(g->ny - g->ny / 2);
+  if (g->ndim == 3) {
+    g->zmn = g->dz * (double)(-(g->nz / 2));
+    g->zmx = g->dz * (double)(g->nz - g->nz / 2);
+  } else {
+    g->zmn = 0.0;
+    g->zmx = 0.0;
+  }
+  /* djc (set_init_param.f90:92-102) */
+  g->djc[0] = g->dx;
+  g->djc[1] = 0.0;
+  g->djc[2] = 0.0;
+  if (g->ndim == 2) {
+    g->djc[0] = 0.5 * g->dx;
+    g->djc[1] = 0.5 * g->dy;
+  } else if (g->ndim == 3) {
+    g->djc[0] = g->dx / 3.;
+    g->djc[1] = g->dy / 3.;
+    g->djc[2] = g->dz / 3.;
+  }
+  /* se_coeff (grid_field_param.f90:36-99) */
+  double yx_rat = g->dy / g->dx;
+  double nu = cfl;
+  if (g->ndim > 1)
+    nu = cfl * yx_rat / sqrt(yx_rat * yx_rat + (double)g->ndim - 1.);
+  switch (g->der_ord) {
+    case 3:
+      g->se_coeff[0] = 1. + 0.125 * (1. - nu * nu);
+      g->se_coeff[1] = (1. - g->se_coeff[0]) / 3.;
+      break;
+    case 4:
+      g->se_coeff[0] = 1.125;
+      g->se_coeff[1] = (1. - g->se_coeff[0]) / 3.;
+      break;
+    default:
+      g->der_ord = 2;
+      g->se_coeff[0] = 1.;
+      g->se_coeff[1] = 0.;
+      break;
+  }
+}
+
+double orc_pack_weight(int charge, int part_ind, float wgh) {
+  unsigned char b[8];
+  int16_t c = (int16_t)charge, p = (int16_t)part_ind;
+  memcpy(b, &c, 2);
+  memcpy(b + 2, &p, 2);
+  memcpy(b + 4, &wgh, 4);
+  double w;
+  memcpy(&w, b, 8);
+  return w;
+}
+
+void orc_unpack_weight(double w, int *charge, int *part_ind, float *wgh) {
+  unsigned char b[8];
+  memcpy(b, &w, 8);
+  int16_t c, p;
+  float f;
+  memcpy(&c, b, 2);
+  memcpy(&p, b + 2, 2);
+  memcpy(&f, b + 4, 4);
+  if (charge) *charge = c;
+  if (part_ind) *part_ind = p;
+  if (wgh) *wgh = f;
+}
+
+/* qlh_*_spline, one axis (grid_part_lib.f90:110-143, 212-259): quadratic
+ * weights at integer nodes, linear at half-integer nodes, base i = ix-1. */
+static inline void qlh_axis(double xp, double a[3], double ah[2], int *i) {
+  double xx = 3.0 + xp; /* shx = 3 (grid_part_lib.f90:32) */
+  int ix = (int)(xx + 0.5);
+  double sx = xx - (double)ix;
+  double sx2 = sx * sx;
+  a[1] = 0.75 - sx2;
+  a[2] = 0.5 * (0.25 + sx2 + sx);
+  a[0] = 1. - a[1] - a[2];
+  ah[1] = sx + 0.5;
+  ah[0] = 1. - ah[1];
+  *i = ix - 1;
+}
+
+/* qden_*_wgh, one axis (grid_part_lib.f90:337-358, 388-417): ix NOT shifted. */
+static inline void qden_axis(double xp, double a[3], int *ix) {
+  double xx = 3.0 + xp;
+  int i = (int)(xx + 0.5);
+  double sx = xx - (double)i;
+  double sx2 = sx * sx;
+  a[1] = 0.75 - sx2;
+  a[2] = 0.5 * (0.25 + sx2 + sx);
+  a[0] = 1. - a[1] - a[2];
+  *ix = i;
+}
+
+void orc_gather(const orc_grid *g, int64_t np, const double *part, int64_t ld,
+                const double *ebf, double *ebfp) {
+  double ax1[3], axh[2], ay1[3], ayh[2], az1[3], azh[2];
+  int i, j, k;
+  if (g->ndim == 2) {
+    /* set_part2d_hcell_acc, nfield = 3 (grid_part_connect.f90:104-142) */
+    for (int64_t n = 0; n < np; n++) {
+      double xl = g->dx_inv * (PC(part, n, 1) - g->xmn);
+      double yl = g->dy_inv * (PC(part, n, 2) - g->ymn);
+      qlh_axis(xl, ax1, axh, &i);
+      qlh_axis(yl, ay1, ayh, &j);
+      const int ih = i, jh = j;
+      double ap1 = 0., ap2 = 0., ap3 = 0.;
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = j + j1;
+        double dvol = ay1[j1];
+        for (int i1 = 0; i1 <= 1; i1++) {
+          int i2 = i1 + ih;
+          double dvol1 = axh[i1] * dvol;
+          ap1 = ap1 + dvol1 * F4(ebf, i2, j2, 1, 1);
+        }
+      }
+      for (int j1 = 0; j1 <= 1; j1++) {
+        int j2 = jh + j1;
+        double dvol = ayh[j1];
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i + i1;
+          double dvol1 = ax1[i1] * dvol;
+          ap2 = ap2 + dvol1 * F4(ebf, i2, j2, 1, 2);
+        }
+        for (int i1 = 0; i1 <= 1; i1++) {
+          int i2 = i1 + ih;
+          double dvol1 = axh[i1] * dvol;
+          ap3 = ap3 + dvol1 * F4(ebf, i2, j2, 1, 3);
+        }
+      }
+      PC(ebfp, n, 1) = ap1;
+      PC(ebfp, n, 2) = ap2;
+      PC(ebfp, n, 3) = ap3;
+    }
+    return;
+  }
+  /* set_part3d_hcell_acc (grid_part_connect.f90:187-285) */
+  for (int64_t n = 0; n < np; n++) {
+    double xl = g->dx_inv * (PC(part, n, 1) - g->xmn);
+    double yl = g->dy_inv * (PC(part, n, 2) - g->ymn);
+    double zl = g->dz_inv * (PC(part, n, 3) - g->zmn);
+    qlh_axis(xl, ax1, axh, &i);
+    qlh_axis(yl, ay1, ayh, &j);
+    qlh_axis(zl, az1, azh, &k);
+    const int ih = i, jh = j, kh = k;
+    double ap[6] = {0., 0., 0., 0., 0., 0.};
+    for (int k1 = 0; k1 <= 2; k1++) {
+      int k2 = k + k1;
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = j + j1;
+        double dvol = ay1[j1] * az1[k1];
+        for (int i1 = 0; i1 <= 1; i1++) {
+          int i2 = i1 + ih;
+          ap[0] = ap[0] + axh[i1] * dvol * F4(ebf, i2, j2, k2, 1);
+        }
+      }
+      for (int j1 = 0; j1 <= 1; j1++) {
+        int j2 = jh + j1;
+        double dvol = ayh[j1] * az1[k1];
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i + i1;
+          ap[1] = ap[1] + ax1[i1] * dvol * F4(ebf, i2, j2, k2, 2);
+        }
+        for (int i1 = 0; i1 <= 1; i1++) {
+          int i2 = i1 + ih;
+          ap[5] = ap[5] + axh[i1] * dvol * F4(ebf, i2, j2, k2, 6);
+        }
+      }
+    }
+    for (int k1 = 0; k1 <= 1; k1++) {
+      int k2 = kh + k1;
+      for (int j1 = 0; j1 <= 1; j1++) {
+        int j2 = jh + j1;
+        double dvol = ayh[j1] * azh[k1];
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i1 + i;
+          ap[3] = ap[3] + ax1[i1] * dvol * F4(ebf, i2, j2, k2, 4);
+        }
+      }
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = j + j1;
+        double dvol = ay1[j1] * azh[k1];
+        for (int i1 = 0; i1 <= 1; i1++) {
+          int i2 = ih + i1;
+          ap[4] = ap[4] + axh[i1] * dvol * F4(ebf, i2, j2, k2, 5);
+        }
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i1 + i;
+          ap[2] = ap[2] + ax1[i1] * dvol * F4(ebf, i2, j2, k2, 3);
+        }
+      }
+    }
+    for (int c = 0; c < 6; c++) PC(ebfp, n, c + 1) = ap[c];
+  }
+}
+
+void orc_charge_multiply(const orc_grid *g, int64_t np, const double *part,
+                         int64_t ld, double *ebfp) {
+  const int ch = 2 * g->curr_ndim + 1; /* size(apt, 2) */
+  for (int64_t n = 0; n < np; n++) {
+    int charge;
+    orc_unpack_weight(PC(part, n, ch), &charge, NULL, NULL);
+    for (int c = 1; c <= g->nfield; c++)
+      PC(ebfp, n, c) = (double)charge * PC(ebfp, n, c);
+  }
+}
+
+void orc_init_momenta(const orc_grid *g, int64_t np, double *part, int64_t ld,
+                      const double *ebfp) {
+  const double dth_lp = 0.5 * g->dt;
+  const double alp = dth_lp * g->lorentz_fact;
+  if (g->curr_ndim == 2) {
+    for (int64_t n = 0; n < np; n++) {
+      double efp1 = -alp * PC(ebfp, n, 1);
+      double efp2 = -alp * PC(ebfp, n, 2);
+      double efp3 = -alp * PC(ebfp, n, 3);
+      double pp1 = PC(part, n, 3), pp2 = PC(part, n, 4);
+      double gam2 = 1. + (pp1 * pp1 + pp2 * pp2);
+      double gam_inv = 1. / sqrt(gam2);
+      double vp1 = pp1 * gam_inv, vp2 = pp2 * gam_inv;
+      PC(part, n, 3) = PC(part, n, 3) + efp1 + vp2 * efp3;
+      PC(part, n, 4) = PC(part, n, 4) + efp2 - vp1 * efp3;
+    }
+    return;
+  }
+  for (int64_t n = 0; n < np; n++) {
+    double pp1 = PC(part, n, 4), pp2 = PC(part, n, 5), pp3 = PC(part, n, 6);
+    double efp[6];
+    for (int c = 0; c < 6; c++) efp[c] = -alp * PC(ebfp, n, c + 1);
+    double gam2 = 1. + (pp1 * pp1 + pp2 * pp2 + pp3 * pp3);
+    double gam_inv = 1. / sqrt(gam2);
+    double vp1 = gam_inv * pp1, vp2 = gam_inv * pp2, vp3 = gam_inv * pp3;
+    PC(part, n, 4) = PC(part, n, 4) + efp[0] + vp2 * efp[5] - vp3 * efp[4];
+    PC(part, n, 5) = PC(part, n, 5) + efp[1] + vp3 * efp[3] - vp1 * efp[5];
+    PC(part, n, 6) = PC(part, n, 6) + efp[2] + vp1 * efp[4] - vp2 * efp[3];
+  }
+}
+
+void orc_push(const orc_grid *g, int64_t np, double *part, int64_t ld,
+              double *ebfp) {
+  const double dt_lp = g->dt;
+  const double dth_lp = 0.5 * dt_lp;
+  const double alp = dth_lp * g->lorentz_fact;
+  int ch;
+  if (g->curr_ndim == 2) {
+    ch = 5;
+    for (int64_t n = 0; n < np; n++) {
+      double pp1 = PC(part, n, 3), pp2 = PC(part, n, 4);
+      double efp1 = alp * PC(ebfp, n, 1), efp2 = alp * PC(ebfp, n, 2),
+             efp3 = alp * PC(ebfp, n, 3);
+      double vp1 = pp1 + efp1, vp2 = pp2 + efp2, vp3 = efp3;
+      double gam02 = 1. + (vp1 * vp1 + vp2 * vp2);
+      double b2 = vp3 * vp3;
+      gam02 = gam02 - b2;
+      double gam2 = 0.5 * (gam02 + sqrt(gam02 * gam02 + 4. * b2));
+      double gam = sqrt(gam2);
+      double vph1 = gam2 * vp1 + gam * vp2 * vp3;
+      double vph2 = gam2 * vp2 - gam * vp1 * vp3;
+      vph1 = vph1 / (gam2 + b2);
+      vph2 = vph2 / (gam2 + b2);
+      PC(part, n, 3) = 2. * vph1 - pp1;
+      PC(part, n, 4) = 2. * vph2 - pp2;
+      PC(ebfp, n, 3) = PC(part, n, 1);
+      PC(ebfp, n, 4) = PC(part, n, 2);
+      pp1 = PC(part, n, 3);
+      pp2 = PC(part, n, 4);
+      gam2 = 1. + (pp1 * pp1 + pp2 * pp2);
+      PC(ebfp, n, 5) = dt_lp / sqrt(gam2);
+      vp1 = PC(ebfp, n, 5) * pp1;
+      vp2 = PC(ebfp, n, 5) * pp2;
+      PC(ebfp, n, 1) = vp1;
+      PC(ebfp, n, 2) = vp2;
+      PC(part, n, 1) = PC(part, n, 1) + vp1;
+      PC(part, n, 2) = PC(part, n, 2) + vp2;
+    }
+  } else {
+    ch = 7;
+    for (int64_t n = 0; n < np; n++) {
+      double pp[3] = {PC(part, n, 4), PC(part, n, 5), PC(part, n, 6)};
+      double efp[6];
+      for (int c = 0; c < 6; c++) efp[c] = alp * PC(ebfp, n, c + 1);
+      double vp[3] = {pp[0] + efp[0], pp[1] + efp[1], pp[2] + efp[2]};
+      double bb[3] = {efp[3], efp[4], efp[5]};
+      double gam02 = 1. + (vp[0] * vp[0] + vp[1] * vp[1] + vp[2] * vp[2]);
+      double b2 = bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2];
+      double bv = bb[0] * vp[0] + bb[1] * vp[1] + bb[2] * vp[2];
+      gam02 = gam02 - b2;
+      double gam2 =
+          0.5 * (gam02 + sqrt(gam02 * gam02 + 4. * (b2 + bv * bv)));
+      double gam = sqrt(gam2);
+      double vph[3];
+      for (int c = 0; c < 3; c++) vph[c] = gam2 * vp[c] + bb[c] * bv;
+      vph[0] = vph[0] + gam * (vp[1] * bb[2] - vp[2] * bb[1]);
+      vph[1] = vph[1] + gam * (vp[2] * bb[0] - vp[0] * bb[2]);
+      vph[2] = vph[2] + gam * (vp[0] * bb[1] - vp[1] * bb[0]);
+      for (int c = 0; c < 3; c++) vph[c] = vph[c] / (b2 + gam2);
+      for (int c = 0; c < 3; c++) PC(part, n, 4 + c) = 2. * vph[c] - pp[c];
+      for (int c = 0; c < 3; c++) PC(ebfp, n, 4 + c) = PC(part, n, 1 + c);
+      for (int c = 0; c < 3; c++) pp[c] = PC(part, n, 4 + c);
+      gam2 = 1. + (pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);
+      PC(ebfp, n, 7) = dt_lp / sqrt(gam2);
+      for (int c = 0; c < 3; c++) {
+        double v = PC(ebfp, n, 7) * pp[c];
+        PC(ebfp, n, 1 + c) = v;
+        PC(part, n, 1 + c) = PC(part, n, 1 + c) + v;
+      }
+    }
+  }
+  /* iform < 2: the packed weight is kept in the stage buffer */
+  for (int64_t n = 0; n < np; n++) PC(ebfp, n, ch) = PC(part, n, ch);
+}
+
+/* reset_all_part_dist for one axis (mpi_part_interface.f90:467-523). */
+static int64_t part_dist_axis(int64_t np, double *part, int64_t ld,
+                              double *ebfp, int ndv, int cin, double xl,
+                              double xr, int ib) {
+  if (ib == 2) {
+    double dxp = xr - xl;
+    for (int64_t n = 0; n < np; n++) {
+      double xp = PC(part, n, cin);
+      if (xp < xl) PC(part, n, cin) = xp + dxp;
+      xp = PC(part, n, cin);
+      if (xp > xr) PC(part, n, cin) = xp - dxp;
+    }
+    return np;
+  }
+  int64_t p = 0;
+  for (int64_t n = 0; n < np; n++) {
+    double xp = PC(part, n, cin);
+    if (xp > xl && xp <= xr) {
+      if (p != n) {
+        for (int c = 1; c <= ndv; c++) {
+          PC(part, p, c) = PC(part, n, c);
+          PC(ebfp, p, c) = PC(ebfp, n, c);
+        }
+      }
+      p++;
+    }
+  }
+  return p;
+}
+
+int64_t orc_part_dist(const orc_grid *g, int64_t np, double *part, int64_t ld,
+                      double *ebfp) {
+  /* cell_part_dist(.false.) with one rank per direction
+   * (mpi_part_interface.f90:525-753): x, then y, then (3-D) z. */
+  const int ndv = 2 * g->curr_ndim + 1;
+  np = part_dist_axis(np, part, ld, ebfp, ndv, 1, g->xmn, g->xmx, g->ibx);
+  np = part_dist_axis(np, part, ld, ebfp, ndv, 2, g->ymn, g->ymx, g->iby);
+  if (g->ndim > 2)
+    np = part_dist_axis(np, part, ld, ebfp, ndv, 3, g->zmn, g->zmx, g->ibz);
+  return np;
+}
+
+/* One axis of the Esirkepov inversion shared by esirkepov_2d_curr and
+ * esirkepov_3d_curr (grid_part_connect.f90:1007-1025, 1235-1249): W1 placed
+ * into the 5-slot window at offset ih = i1-i0+1, curr = -cumsum(W1-W0),
+ * scaled by the FP32 weight. */
+static int64_t esk_skipped = 0;
+
+int64_t orc_esirkepov_skipped(void) { return esk_skipped; }
+
+static inline void esk_axis(const double a0[3], const double a1[3], int ih,
+                            float wght, double wh[5], double cur[5]) {
+  for (int s = 0; s < 5; s++) wh[s] = 0.0;
+  for (int s = 0; s <= 2; s++) wh[ih + s] = a1[s];
+  cur[0] = -wh[0];
+  for (int s = 1; s <= 3; s++) cur[s] = cur[s - 1] + a0[s - 1] - wh[s];
+  cur[4] = cur[3] - wh[4];
+  for (int s = 0; s < 5; s++) cur[s] = (double)wght * cur[s];
+}
+
+void orc_esirkepov(const orc_grid *g, int64_t np, const double *part,
+                   int64_t ld, double *ebfp, double *jc) {
+  if (np <= 0) return;
+  if (g->curr_ndim == 2) {
+    /* esirkepov_2d_curr, ndim 2, curr_ndim 2 (grid_part_connect.f90:979-1060) */
+    for (int64_t n = 0; n < np; n++) {
+      int charge;
+      float wgh;
+      orc_unpack_weight(PC(part, n, 5), &charge, NULL, &wgh);
+      float wght = (float)charge * wgh;
+      PC(ebfp, n, 5) = (double)wght;
+      PC(ebfp, n, 1) = g->dx_inv * (PC(part, n, 1) - g->xmn);
+      PC(ebfp, n, 2) = g->dy_inv * (PC(part, n, 2) - g->ymn);
+      PC(ebfp, n, 3) = g->dx_inv * (PC(ebfp, n, 3) - g->xmn);
+      PC(ebfp, n, 4) = g->dy_inv * (PC(ebfp, n, 4) - g->ymn);
+    }
+    for (int64_t n = 0; n < np; n++) {
+      double ax0[3], ay0[3], ax1[3], ay1[3], axh[5], ayh[5], currx[5],
+          curry[5];
+      int i, j, ii0, jj0;
+      float wght = (float)PC(ebfp, n, 5);
+      qden_axis(PC(ebfp, n, 3), ax0, &ii0);
+      qden_axis(PC(ebfp, n, 4), ay0, &jj0);
+      qden_axis(PC(ebfp, n, 1), ax1, &i);
+      qden_axis(PC(ebfp, n, 2), ay1, &j);
+      int ih = i - ii0 + 1;
+      if (ih < 0 || ih > 2 || j - jj0 + 1 < 0 || j - jj0 + 1 > 2) {
+        esk_skipped++; /* moved >= 1 cell: outside the scheme's validity */
+        continue;
+      }
+      esk_axis(ax0, ax1, ih, wght, axh, currx);
+      for (int s = 1; s <= 3; s++) axh[s] = axh[s] + ax0[s - 1];
+      int x0 = imin(ih, 1), x1 = imax(ih + 2, 3);
+      int jh = j - jj0 + 1;
+      esk_axis(ay0, ay1, jh, wght, ayh, curry);
+      for (int s = 1; s <= 3; s++) ayh[s] = ayh[s] + ay0[s - 1];
+      int y0 = imin(jh, 1), y1 = imax(jh + 2, 3);
+      jj0 = jj0 - 1;
+      jh = jj0 - 1;
+      ii0 = ii0 - 1;
+      ih = ii0 - 1;
+      for (int j1 = y0; j1 <= y1; j1++) {
+        int j2 = jh + j1;
+        for (int i1 = x0; i1 <= x1; i1++) {
+          int i2 = ih + i1;
+          F4(jc, i2, j2, 1, 1) = F4(jc, i2, j2, 1, 1) + ayh[j1] * currx[i1];
+        }
+      }
+      for (int j1 = y0; j1 <= y1; j1++) {
+        int j2 = jh + j1;
+        for (int i1 = x0; i1 <= x1; i1++) {
+          int i2 = ih + i1;
+          F4(jc, i2, j2, 1, 2) = F4(jc, i2, j2, 1, 2) + axh[i1] * curry[j1];
+        }
+      }
+    }
+    return;
+  }
+  /* esirkepov_3d_curr (grid_part_connect.f90:1168-1334) */
+  for (int64_t n = 0; n < np; n++) {
+    int charge;
+    float wgh;
+    orc_unpack_weight(PC(part, n, 7), &charge, NULL, &wgh);
+    float wght = (float)charge * wgh;
+    PC(ebfp, n, 7) = (double)wght;
+    PC(ebfp, n, 1) = g->dx_inv * (PC(part, n, 1) - g->xmn);
+    PC(ebfp, n, 2) = g->dy_inv * (PC(part, n, 2) - g->ymn);
+    PC(ebfp, n, 3) = g->dz_inv * (PC(part, n, 3) - g->zmn);
+    PC(ebfp, n, 4) = g->dx_inv * (PC(ebfp, n, 4) - g->xmn);
+    PC(ebfp, n, 5) = g->dy_inv * (PC(ebfp, n, 5) - g->ymn);
+    PC(ebfp, n, 6) = g->dz_inv * (PC(ebfp, n, 6) - g->zmn);
+  }
+  for (int64_t n = 0; n < np; n++) {
+    double ax0[3], ay0[3], az0[3], ax1[3], ay1[3], az1[3];
+    double axh[5], ayh[5], azh[5], axh0[5], axh1[5], ayh0[5], ayh1[5];
+    double currx[5], curry[5], currz[5];
+    int i, j, k, ii0, jj0, kk0;
+    float wght = (float)PC(ebfp, n, 7);
+    qden_axis(PC(ebfp, n, 4), ax0, &ii0);
+    qden_axis(PC(ebfp, n, 5), ay0, &jj0);
+    qden_axis(PC(ebfp, n, 6), az0, &kk0);
+    qden_axis(PC(ebfp, n, 1), ax1, &i);
+    qden_axis(PC(ebfp, n, 2), ay1, &j);
+    qden_axis(PC(ebfp, n, 3), az1, &k);
+    int ih = i - ii0 + 1;
+    if (ih < 0 || ih > 2 || j - jj0 + 1 < 0 || j - jj0 + 1 > 2 ||
+        k - kk0 + 1 < 0 || k - kk0 + 1 > 2) {
+      esk_skipped++; /* moved >= 1 cell: outside the scheme's validity */
+      continue;
+    }
+    esk_axis(ax0, ax1, ih, wght, axh, currx);
+    for (int s = 0; s < 5; s++) {
+      axh0[s] = 0.5 * axh[s];
+      axh1[s] = axh[s];
+    }
+    for (int s = 1; s <= 3; s++) {
+      axh0[s] = axh0[s] + ax0[s - 1];
+      axh1[s] = axh1[s] + 0.5 * ax0[s - 1];
+    }
+    int x0 = imin(ih, 1), x1 = imax(ih + 2, 3);
+    i = i - 1;
+    ii0 = ii0 - 1;
+    int jh = j - jj0 + 1;
+    esk_axis(ay0, ay1, jh, wght, ayh, curry);
+    for (int s = 0; s < 5; s++) {
+      ayh0[s] = 0.5 * ayh[s];
+      ayh1[s] = ayh[s];
+    }
+    for (int s = 1; s <= 3; s++) {
+      ayh0[s] = ayh0[s] + ay0[s - 1];
+      ayh1[s] = ayh1[s] + 0.5 * ay0[s - 1];
+    }
+    int y0 = imin(jh, 1), y1 = imax(jh + 2, 3);
+    jj0 = jj0 - 1;
+    j = j - 1;
+    int kh = k - kk0 + 1;
+    esk_axis(az0, az1, kh, wght, azh, currz);
+    kk0 = kk0 - 1;
+    k = k - 1;
+    int z0 = imin(kh, 1), z1 = imax(kh + 2, 3);
+    jh = jj0 - 1;
+    ih = ii0 - 1;
+    for (int k1 = 0; k1 <= 2; k1++) {
+      for (int j1 = y0; j1 <= y1; j1++) {
+        int j2 = jh + j1;
+        double dvol = ayh0[j1] * az0[k1];
+        double dvolh = ayh1[j1] * az1[k1];
+        for (int i1 = x0; i1 <= x1; i1++) {
+          int i2 = ih + i1;
+          F4(jc, i2, j2, kk0 + k1, 1) =
+              F4(jc, i2, j2, kk0 + k1, 1) + dvol * currx[i1];
+          F4(jc, i2, j2, k + k1, 1) =
+              F4(jc, i2, j2, k + k1, 1) + dvolh * currx[i1];
+        }
+      }
+    }
+    for (int k1 = 0; k1 <= 2; k1++) {
+      for (int j1 = y0; j1 <= y1; j1++) {
+        int j2 = jh + j1;
+        double dvol = curry[j1] * az0[k1];
+        double dvolh = curry[j1] * az1[k1];
+        for (int i1 = x0; i1 <= x1; i1++) {
+          int i2 = ih + i1;
+          F4(jc, i2, j2, kk0 + k1, 2) =
+              F4(jc, i2, j2, kk0 + k1, 2) + axh0[i1] * dvol;
+          F4(jc, i2, j2, k + k1, 2) =
+              F4(jc, i2, j2, k + k1, 2) + axh1[i1] * dvolh;
+        }
+      }
+    }
+    kh = kk0 - 1;
+    for (int k1 = z0; k1 <= z1; k1++) {
+      int k2 = kh + k1;
+      for (int j1 = 0; j1 <= 2; j1++) {
+        double dvol = ay0[j1] * currz[k1];
+        double dvolh = ay1[j1] * currz[k1];
+        for (int i1 = x0; i1 <= x1; i1++) {
+          int i2 = ih + i1;
+          F4(jc, i2, jj0 + j1, k2, 3) =
+              F4(jc, i2, jj0 + j1, k2, 3) + axh0[i1] * dvol;
+          F4(jc, i2, j + j1, k2, 3) =
+              F4(jc, i2, j + j1, k2, 3) + axh1[i1] * dvolh;
+        }
+      }
+    }
+  }
+}
+
+static const int x_parity[6] = {-1, 1, 1, -1, 1, 1};
+static const int y_parity[6] = {1, -1, 1, 1, -1, 1};
+static const int z_parity[6] = {1, 1, -1, 1, 1, -1};
+
+void orc_field_xyzbd(const orc_grid *g, double *ebf, int nc) {
+  /* field_xyzbd on a single rank: pex0, pex1 and all *_bd are true
+   * (mpi_field_interface.f90:53-198). */
+  int i1 = g->ix1, i2 = g->ix2, j1 = g->jy1, j2 = g->jy2, k1 = g->kz1,
+      k2 = g->kz2;
+  const int ix1 = g->ix1, ix2 = g->ix2, jy1 = g->jy1, jy2 = g->jy2,
+            kz1 = g->kz1, kz2 = g->kz2;
+  if (g->ibx < 2)
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iz = k1; iz <= k2; iz++)
+        for (int iy = j1; iy <= j2; iy++)
+          F4(ebf, ix1 - 1, iy, iz, ik) =
+              2. * F4(ebf, ix1, iy, iz, ik) - F4(ebf, ix1 + 1, iy, iz, ik);
+  i1 = i1 - 1;
+  if (g->ibx == 0)
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iz = k1; iz <= k2; iz++)
+        for (int iy = j1; iy <= j2; iy++)
+          F4(ebf, ix2 + 1, iy, iz, ik) =
+              2. * F4(ebf, ix2, iy, iz, ik) - F4(ebf, ix2 - 1, iy, iz, ik);
+  if (g->ibx == 1)
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iz = k1; iz <= k2; iz++)
+        for (int iy = j1; iy <= j2; iy++)
+          F4(ebf, ix2 + 1, iy, iz, ik) =
+              x_parity[ik - 1] * F4(ebf, ix2, iy, iz, ik);
+  i2 = i2 + 1;
+  if (g->ndim < 2) return;
+  if (g->iby == 0)
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iz = k1; iz <= k2; iz++)
+        for (int ix = i1; ix <= i2; ix++)
+          F4(ebf, ix, jy1 - 1, iz, ik) =
+              2. * F4(ebf, ix, jy1, iz, ik) - F4(ebf, ix, jy1 + 1, iz, ik);
+  if (g->iby == 1)
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iz = k1; iz <= k2; iz++)
+        for (int ix = i1; ix <= i2; ix++)
+          F4(ebf, ix, jy1 - 1, iz, ik) =
+              y_parity[ik - 1] * F4(ebf, ix, jy1 + 1, iz, ik);
+  j1 = j1 - 1;
+  if (g->iby == 0)
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iz = k1; iz <= k2; iz++)
+        for (int ix = i1; ix <= i2; ix++)
+          F4(ebf, ix, jy2 + 1, iz, ik) =
+              2. * F4(ebf, ix, jy2, iz, ik) - F4(ebf, ix, jy2 - 1, iz, ik);
+  if (g->iby == 1)
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iz = k1; iz <= k2; iz++)
+        for (int ix = i1; ix <= i2; ix++)
+          F4(ebf, ix, jy2 + 1, iz, ik) =
+              y_parity[ik - 1] * F4(ebf, ix, jy2 - 1, iz, ik);
+  j2 = j2 + 1;
+  if (g->ndim < 3) return;
+  if (g->ibz == 0)
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iy = j1; iy <= j2; iy++)
+        for (int ix = i1; ix <= i2; ix++)
+          F4(ebf, ix, iy, kz1 - 1, ik) =
+              2. * F4(ebf, ix, iy, kz1, ik) - F4(ebf, ix, iy, kz1 + 1, ik);
+  if (g->ibz == 1)
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iy = j1; iy <= j2; iy++)
+        for (int ix = i1; ix <= i2; ix++)
+          F4(ebf, ix, iy, kz1 - 1, ik) =
+              z_parity[ik - 1] * F4(ebf, ix, iy, kz1 + 1, ik);
+  k1 = k1 - 1;
+  if (g->ibz == 0)
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iy = j1; iy <= j2; iy++)
+        for (int ix = i1; ix <= i2; ix++)
+          F4(ebf, ix, iy, kz2 + 1, ik) =
+              2. * F4(ebf, ix, iy, kz2, ik) - F4(ebf, ix, iy, kz2 - 1, ik);
+  if (g->ibz == 1) /* the reference writes kz1+1 from kz1-1 here */
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iy = j1; iy <= j2; iy++)
+        for (int ix = i1; ix <= i2; ix++)
+          F4(ebf, ix, iy, kz1 + 1, ik) =
+              z_parity[ik - 1] * F4(ebf, ix, iy, kz1 - 1, ik);
+  k2 = k2 + 1;
+  (void)k2;
+}
+
+void orc_jc_xyzbd(const orc_grid *g, double *jc) {
+  /* jc_xyzbd on a single rank (mpi_curr_interface.f90:551-643). */
+  const int i1 = g->ix1, n1 = g->ix2, j1 = g->jy1, n2 = g->jy2, k1 = g->kz1,
+            n3 = g->kz2;
+  const int j2 = n2, k2 = n3, nc = g->curr_ndim;
+  if (g->ibx == 0) {
+    for (int ik = 1; ik <= nc; ik++)
+      for (int kk = k1; kk <= k2; kk++)
+        for (int jj = j1; jj <= j2; jj++) {
+          F4(jc, i1, jj, kk, ik) = F4(jc, i1, jj, kk, ik) + F4(jc, i1 - 1, jj, kk, ik);
+          F4(jc, i1 - 1, jj, kk, ik) = 0.0;
+          F4(jc, n1, jj, kk, ik) = F4(jc, n1, jj, kk, ik) +
+                                   F4(jc, n1 + 1, jj, kk, ik) +
+                                   F4(jc, n1 + 2, jj, kk, ik);
+          F4(jc, n1 + 1, jj, kk, ik) = 0.0;
+          F4(jc, n1 + 2, jj, kk, ik) = 0.0;
+        }
+  }
+  if (g->ndim < 2) return;
+  if (g->iby == 0) {
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iz = k1; iz <= k2; iz++)
+        for (int ix = i1; ix <= n1; ix++) {
+          F4(jc, ix, j1, iz, ik) = F4(jc, ix, j1, iz, ik) +
+                                   F4(jc, ix, j1 - 1, iz, ik) +
+                                   F4(jc, ix, j1 - 2, iz, ik);
+          F4(jc, ix, j1 - 2, iz, ik) = 0.0;
+          F4(jc, ix, j1 - 1, iz, ik) = 0.0;
+        }
+  }
+  if (g->iby == 1) {
+    for (int iz = k1; iz <= k2; iz++)
+      for (int ix = i1; ix <= n1; ix++) {
+        F4(jc, ix, j1 + 1, iz, 1) = F4(jc, ix, j1 + 1, iz, 1) + F4(jc, ix, j1 - 1, iz, 1);
+        F4(jc, ix, j1, iz, 2) = F4(jc, ix, j1, iz, 2) - F4(jc, ix, j1 - 1, iz, 2);
+      }
+  }
+  if (g->iby == 0) {
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iz = k1; iz <= k2; iz++)
+        for (int ix = i1; ix <= n1; ix++) {
+          F4(jc, ix, n2, iz, ik) = F4(jc, ix, n2, iz, ik) +
+                                   F4(jc, ix, n2 + 1, iz, ik) +
+                                   F4(jc, ix, n2 + 2, iz, ik);
+          F4(jc, ix, n2 + 1, iz, ik) = 0.0;
+          F4(jc, ix, n2 + 2, iz, ik) = 0.0;
+        }
+  }
+  if (g->ndim < 3) return;
+  if (g->ibz == 0) {
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iy = j1; iy <= j2; iy++)
+        for (int ix = i1; ix <= n1; ix++) {
+          F4(jc, ix, iy, k1, ik) = F4(jc, ix, iy, k1, ik) + F4(jc, ix, iy, k1 - 1, ik);
+          F4(jc, ix, iy, k1 - 1, ik) = 0.0;
+        }
+    for (int ik = 1; ik <= nc; ik++)
+      for (int iy = j1; iy <= j2; iy++)
+        for (int ix = i1; ix <= n1; ix++) {
+          F4(jc, ix, iy, n3, ik) = F4(jc, ix, iy, n3, ik) +
+                                   F4(jc, ix, iy, n3 + 1, ik) +
+                                   F4(jc, ix, iy, n3 + 2, ik);
+          F4(jc, ix, iy, n3 + 1, ik) = 0.0;
+          F4(jc, ix, iy, n3 + 2, ik) = 0.0;
+        }
+  }
+}
+
+void orc_curr_collect(const orc_grid *g, double *jc) {
+  /* curr_mpi_collect (curr_and_fields_util.f90:104-152), unstretched. */
+  orc_jc_xyzbd(g, jc);
+  const int64_t ncell = (int64_t)g->n1p * g->n2p * g->n3p;
+  for (int i = 0; i < g->ndim; i++)
+    for (int64_t c = 0; c < ncell; c++)
+      jc[(int64_t)i * ncell + c] = g->djc[i] * jc[(int64_t)i * ncell + c];
+}
+
+void orc_ef_bds(const orc_grid *g, double *ebf, double dtl) {
+  /* ef_bds(ef, dtl, 0) on a single rank (grid_fields.f90:1204-1354). */
+  const int nf = g->nfield, ix2 = g->ix2, jy2 = g->jy2, kz2 = g->kz2;
+  double aphx = 1, aphy = 1, aphz = 1;
+  if (g->ibx < 2) {
+    if (g->ibx == 0) {
+      aphx = 1.0 * g->dx_inv * dtl;
+      for (int k = g->kz1; k <= g->kz2; k++)
+        for (int j = g->jy1; j <= g->jy2; j++)
+          F4(ebf, ix2 + 1, j, k, 2) =
+              (2. * F4(ebf, ix2, j, k, nf) - (1. - aphx) * F4(ebf, ix2, j, k, 2)) /
+              (1. + aphx);
+    } else {
+      for (int k = g->kz1; k <= g->kz2; k++)
+        for (int j = g->jy1; j <= g->jy2; j++)
+          F4(ebf, ix2 + 1, j, k, 2) = F4(ebf, ix2, j, k, 2);
+    }
+    if (nf > 3) {
+      if (g->ibx == 0) {
+        for (int k = g->kz1; k <= g->kz2; k++)
+          for (int j = g->jy1; j <= g->jy2; j++)
+            F4(ebf, ix2 + 1, j, k, 3) =
+                -(2. * F4(ebf, ix2, j, k, 5) + (1. - aphx) * F4(ebf, ix2, j, k, 3)) /
+                (1. + aphx);
+      } else {
+        for (int k = g->kz1; k <= g->kz2; k++)
+          for (int j = g->jy1; j <= g->jy2; j++)
+            F4(ebf, ix2 + 1, j, k, 3) = F4(ebf, ix2, j, k, 3);
+      }
+    }
+  }
+  if (g->ndim < 2) return;
+  if (g->iby < 2) {
+    aphy = 1.0 * g->dy_inv * dtl;
+    for (int k = g->kz1; k <= g->kz2; k++)
+      for (int i = g->ix1; i <= g->ix2; i++)
+        F4(ebf, i, jy2 + 1, k, 1) =
+            -(2. * F4(ebf, i, jy2, k, nf) + (1. - aphy) * F4(ebf, i, jy2, k, 1)) /
+            (1. + aphy);
+    if (nf > 3)
+      for (int k = g->kz1; k <= g->kz2; k++)
+        for (int i = g->ix1; i <= g->ix2; i++)
+          F4(ebf, i, jy2 + 1, k, 3) =
+              (2. * F4(ebf, i, jy2, k, 4) - (1. - aphy) * F4(ebf, i, jy2, k, 3)) /
+              (1. + aphy);
+  }
+  if (g->ndim < 3) return;
+  if (g->ibz < 2) {
+    aphz = 1.0 * g->dz_inv * dtl;
+    for (int j = g->jy1; j <= g->jy2; j++)
+      for (int i = g->ix1; i <= g->ix2; i++) {
+        F4(ebf, i, j, kz2 + 1, 1) =
+            (2. * F4(ebf, i, j, kz2, 5) - (1. - aphz) * F4(ebf, i, j, kz2, 1)) /
+            (1. + aphz);
+        F4(ebf, i, j, kz2 + 1, 2) =
+            -(2. * F4(ebf, i, j, kz2, 4) + (1. - aphz) * F4(ebf, i, j, kz2, 2)) /
+            (1. + aphz);
+      }
+  }
+}
+
+void orc_bf_bds(const orc_grid *g, double *ebf, double dtl) {
+  /* bf_bds(ef, dtl, 0) on a single rank (grid_fields.f90:1069-1202). */
+  const int nf = g->nfield, ix1 = g->ix1, jy1 = g->jy1, kz1 = g->kz1;
+  if (g->ibx < 2) {
+    double aphx = 1.0 * g->dx_inv * dtl;
+    for (int k = g->kz1; k <= g->kz2; k++)
+      for (int j = g->jy1; j <= g->jy2; j++)
+        F4(ebf, ix1 - 1, j, k, nf) =
+            -(2. * F4(ebf, ix1, j, k, 2) + (1. - aphx) * F4(ebf, ix1, j, k, nf)) /
+            (1. + aphx);
+    if (nf > 3)
+      for (int k = g->kz1; k <= g->kz2; k++)
+        for (int j = g->jy1; j <= g->jy2; j++)
+          F4(ebf, ix1 - 1, j, k, 5) =
+              (2. * F4(ebf, ix1, j, k, 3) - (1. - aphx) * F4(ebf, ix1, j, k, 5)) /
+              (1. + aphx);
+  }
+  if (g->ndim < 2) return;
+  if (g->iby < 2) {
+    double aphy = 1.0 * g->dy_inv * dtl;
+    for (int k = g->kz1; k <= g->kz2; k++)
+      for (int i = g->ix1; i <= g->ix2; i++) {
+        double v = 2. * F4(ebf, i, jy1, k, 1) - (1. - aphy) * F4(ebf, i, jy1, k, nf);
+        F4(ebf, i, jy1 - 1, k, nf) = v / (1. + aphy);
+      }
+    if (nf > 3)
+      for (int k = g->kz1; k <= g->kz2; k++)
+        for (int i = g->ix1; i <= g->ix2; i++) {
+          double v = -2. * F4(ebf, i, jy1, k, 3) - (1. - aphy) * F4(ebf, i, jy1, k, 4);
+          F4(ebf, i, jy1 - 1, k, 4) = v / (1. + aphy);
+        }
+  }
+  if (g->ndim < 3) return;
+  if (g->ibz < 2) {
+    double aphz = 1.0 * g->dz_inv * dtl;
+    for (int j = g->jy1; j <= g->jy2; j++)
+      for (int i = g->ix1; i <= g->ix2; i++) {
+        double v = 2. * F4(ebf, i, j, kz1, 2) - (1. - aphz) * F4(ebf, i, j, kz1, 4);
+        F4(ebf, i, j, kz1 - 1, 4) = v / (1. + aphz);
+        v = -2. * F4(ebf, i, j, kz1, 1) - (1. - aphz) * F4(ebf, i, j, kz1, 5);
+        F4(ebf, i, j, kz1 - 1, 5) = v / (1. + aphz);
+      }
+  }
+}
+
+void orc_rote(const orc_grid *g, double *ebf, double dtf) {
+  /* rote (grid_fields.f90:1405-1478), unstretched metric = 1. */
+  const double aphx = dtf * g->dx_inv, aphy = dtf * g->dy_inv,
+               aphz = dtf * g->dz_inv;
+  const double aph1 = aphx * g->se_coeff[0], aph2 = aphx * g->se_coeff[1];
+  const int nf = g->nfield;
+  for (int k = g->kz1; k <= g->kz2; k++)
+    for (int j = g->jy1; j <= g->jy2; j++) {
+      const double sdhy = 1.0 * aphy;
+      for (int i = g->ix1; i <= g->ix2; i++)
+        F4(ebf, i, j, k, nf) =
+            F4(ebf, i, j, k, nf) -
+            aph1 * (F4(ebf, i + 1, j, k, 2) - F4(ebf, i, j, k, 2)) +
+            sdhy * (F4(ebf, i, j + 1, k, 1) - F4(ebf, i, j, k, 1)) -
+            aph2 * (F4(ebf, i + 2, j, k, 2) - F4(ebf, i - 1, j, k, 2));
+    }
+  if (nf < 6) return;
+  for (int k = g->kz1; k <= g->kz2; k++) {
+    const double sdhz = 1.0 * aphz;
+    for (int j = g->jy1; j <= g->jy2; j++) {
+      const double sdhy = 1.0 * aphy;
+      for (int i = g->ix1; i <= g->ix2; i++) {
+        F4(ebf, i, j, k, 4) =
+            F4(ebf, i, j, k, 4) -
+            sdhy * (F4(ebf, i, j + 1, k, 3) - F4(ebf, i, j, k, 3)) +
+            sdhz * (F4(ebf, i, j, k + 1, 2) - F4(ebf, i, j, k, 2));
+        F4(ebf, i, j, k, 5) =
+            F4(ebf, i, j, k, 5) +
+            aph1 * (F4(ebf, i + 1, j, k, 3) - F4(ebf, i, j, k, 3)) -
+            sdhz * (F4(ebf, i, j, k + 1, 1) - F4(ebf, i, j, k, 1)) +
+            aph2 * (F4(ebf, i + 2, j, k, 3) - F4(ebf, i - 1, j, k, 3));
+      }
+    }
+  }
+}
+
+void orc_rotb(const orc_grid *g, double *ebf, double dtf) {
+  /* rotb (grid_fields.f90:1480-1552), unstretched metric = 1. */
+  const double aphx = dtf * g->dx_inv, aphy = dtf * g->dy_inv,
+               aphz = dtf * g->dz_inv;
+  const double aph1 = aphx * g->se_coeff[0], aph2 = aphx * g->se_coeff[1];
+  const int nf = g->nfield;
+  for (int k = g->kz1; k <= g->kz2; k++)
+    for (int j = g->jy1; j <= g->jy2; j++) {
+      const double sdy = 1.0 * aphy;
+      for (int i = g->ix1; i <= g->ix2; i++) {
+        F4(ebf, i, j, k, 1) =
+            F4(ebf, i, j, k, 1) +
+            sdy * (F4(ebf, i, j, k, nf) - F4(ebf, i, j - 1, k, nf));
+        F4(ebf, i, j, k, 2) =
+            F4(ebf, i, j, k, 2) -
+            aph1 * (F4(ebf, i, j, k, nf) - F4(ebf, i - 1, j, k, nf)) -
+            aph2 * (F4(ebf, i + 1, j, k, nf) - F4(ebf, i - 2, j, k, nf));
+      }
+    }
+  if (nf < 6) return;
+  for (int k = g->kz1; k <= g->kz2; k++) {
+    const double sdz = aphz * 1.0;
+    for (int j = g->jy1; j <= g->jy2; j++) {
+      const double sdy = aphy * 1.0;
+      for (int i = g->ix1; i <= g->ix2; i++) {
+        F4(ebf, i, j, k, 1) =
+            F4(ebf, i, j, k, 1) -
+            sdz * (F4(ebf, i, j, k, 5) - F4(ebf, i, j, k - 1, 5));
+        F4(ebf, i, j, k, 2) =
+            F4(ebf, i, j, k, 2) +
+            sdz * (F4(ebf, i, j, k, 4) - F4(ebf, i, j, k - 1, 4));
+        F4(ebf, i, j, k, 3) =
+            F4(ebf, i, j, k, 3) +
+            aph1 * (F4(ebf, i, j, k, 5) - F4(ebf, i - 1, j, k, 5)) -
+            sdy * (F4(ebf, i, j, k, 4) - F4(ebf, i, j - 1, k, 4)) +
+            aph2 * (F4(ebf, i + 1, j, k, 5) - F4(ebf, i - 2, j, k, 5));
+      }
+    }
+  }
+}
+
+void orc_advance_fields(const orc_grid *g, double *ebf, const double *jc) {
+  /* advance_lpf_fields(ef, curr, 0) (curr_and_fields_util.f90:176-260). */
+  const double dt_lp = g->dt;
+  const double dth = 0.5 * dt_lp;
+  orc_ef_bds(g, ebf, 0.0);
+  orc_rote(g, ebf, dth);
+  orc_bf_bds(g, ebf, dt_lp);
+  orc_rotb(g, ebf, dt_lp);
+  for (int ik = 1; ik <= g->curr_ndim; ik++)
+    for (int k = g->kz1; k <= g->kz2; k++)
+      for (int j = g->jy1; j <= g->jy2; j++)
+        for (int i = g->ix1; i <= g->ix2; i++)
+          F4(ebf, i, j, k, ik) = F4(ebf, i, j, k, ik) - g->ompe * F4(jc, i, j, k, ik);
+  orc_ef_bds(g, ebf, dt_lp);
+  orc_rote(g, ebf, dth);
+}
+
+int64_t orc_particle_stage(const orc_grid *g, int64_t np, double *part,
+                           int64_t ld, double *ebfp, const double *ebf,
+                           double *jc, int initial_time) {
+  /* pic_evolve_in_time.f90:75-91 with part = .true. */
+  orc_gather(g, np, part, ld, ebf, ebfp);
+  orc_charge_multiply(g, np, part, ld, ebfp);
+  if (initial_time) orc_init_momenta(g, np, part, ld, ebfp);
+  orc_push(g, np, part, ld, ebfp);
+  np = orc_part_dist(g, np, part, ld, ebfp);
+  orc_esirkepov(g, np, part, ld, ebfp, jc);
+  return np;
+}
+
+int64_t orc_lpf2_evolve(const orc_grid *g, int64_t np, double *part,
+                        int64_t ld, double *ebfp, double *ebf, double *jc,
+                        int initial_time) {
+  /* lpf2_evolve (pic_evolve_in_time.f90:37-109), one species, part=.true. */
+  orc_field_xyzbd(g, ebf, g->nfield);
+  const int64_t njc = (int64_t)g->n1p * g->n2p * g->n3p * g->curr_ndim;
+  memset(jc, 0, (size_t)njc * sizeof(double));
+  np = orc_particle_stage(g, np, part, ld, ebfp, ebf, jc, initial_time);
+  orc_curr_collect(g, jc);
+  orc_advance_fields(g, ebf, jc);
+  return np;
+}
+
+void orc_grid_charge(const orc_grid *g, int64_t np, const double *part,
+                     int64_t ld, double *rho) {
+  /* set_grid_charge (grid_part_util.f90:240-335), den(:,:,:,1). */
+  const int ch = 2 * g->curr_ndim + 1;
+  for (int64_t n = 0; n < np; n++) {
+    int charge;
+    float wgh;
+    orc_unpack_weight(PC(part, n, ch), &charge, NULL, &wgh);
+    float wght = (float)charge * wgh;
+    double ax0[3], ay0[3], az0[3];
+    int i, j, k;
+    qden_axis(g->dx_inv * (PC(part, n, 1) - g->xmn), ax0, &i);
+    qden_axis(g->dy_inv * (PC(part, n, 2) - g->ymn), ay0, &j);
+    for (int s = 0; s < 3; s++) ax0[s] = (double)wght * ax0[s];
+    i = i - 1;
+    j = j - 1;
+    if (g->ndim == 2) {
+      for (int j1 = 0; j1 <= 2; j1++)
+        for (int i1 = 0; i1 <= 2; i1++) {
+          double dvol = ax0[i1] * ay0[j1];
+          F4(rho, i + i1, j + j1, 1, 1) = F4(rho, i + i1, j + j1, 1, 1) + dvol;
+        }
+    } else {
+      qden_axis(g->dz_inv * (PC(part, n, 3) - g->zmn), az0, &k);
+      k = k - 1;
+      for (int k1 = 0; k1 <= 2; k1++)
+        for (int j1 = 0; j1 <= 2; j1++) {
+          double dvol = az0[k1] * ay0[j1];
+          for (int i1 = 0; i1 <= 2; i1++)
+            F4(rho, i + i1, j + j1, k + k1, 1) =
+                F4(rho, i + i1, j + j1, k + k1, 1) + ax0[i1] * dvol;
+        }
+    }
+  }
+}
+
+void orc_energy_momenta(const orc_grid *g, int64_t np, const double *part,
+                        int64_t ld, double ek[7], double *ekmax) {
+  /* energy_momenta (diag_part_and_fields.f90:172-219). */
+  for (int c = 0; c < 7; c++) ek[c] = 0.0;
+  double emax = 0.0;
+  if (g->curr_ndim < 3) {
+    for (int64_t n = 0; n < np; n++) {
+      double vp1 = PC(part, n, 3), vp2 = PC(part, n, 4);
+      double gamm = sqrt(1. + vp1 * vp1 + vp2 * vp2);
+      int charge;
+      float wgh;
+      orc_unpack_weight(PC(part, n, 5), &charge, NULL, &wgh);
+      double gam1 = gamm - 1.;
+      ek[0] = ek[0] + (double)wgh * vp1;
+      ek[1] = ek[1] + (double)wgh * vp2;
+      ek[5] = ek[5] + (double)((float)charge * wgh);
+      ek[6] = ek[6] + (double)wgh * gam1;
+      if (gam1 > emax) emax = gam1;
+    }
+  } else {
+    for (int64_t n = 0; n < np; n++) {
+      double xp2 = PC(part, n, 2), xp3 = PC(part, n, 3);
+      double vp1 = PC(part, n, 4), vp2 = PC(part, n, 5), vp3 = PC(part, n, 6);
+      double gamm = sqrt(1. + vp1 * vp1 + vp2 * vp2 + vp3 * vp3);
+      int charge;
+      float wgh;
+      orc_unpack_weight(PC(part, n, 7), &charge, NULL, &wgh);
+      double gam1 = gamm - 1.;
+      ek[0] = ek[0] + vp1;
+      ek[1] = ek[1] + vp2;
+      ek[2] = ek[2] + vp3;
+      ek[3] = ek[3] + (double)wgh * (xp2 * vp3 - xp3 * vp2);
+      ek[5] = ek[5] + (double)((float)charge * wgh);
+      ek[6] = ek[6] + (double)wgh * gam1;
+      if (gam1 > emax) emax = gam1;
+    }
+  }
+  if (ekmax) *ekmax = emax;
+}
+
+void orc_field_energy(const orc_grid *g, const double *ebf, double out[6]) {
+  /* Envar field energy, unstretched branch (diag_part_and_fields.f90:499-670):
+   * i1 = ix1 .. nxp = ix2, j1 = jy1 .. nyp = jy2, k1 = kz1 .. nzp = kz2. */
+  double dgvol = g->dx * g->dy * g->dz;
+  if (g->ndim == 2) dgvol = g->dx * g->dy * g->dy;
+  for (int c = 0; c < 6; c++) out[c] = 0.0;
+  for (int ik = 1; ik <= g->nfield; ik++) {
+    double s = 0.0;
+    for (int iz = g->kz1; iz <= g->kz2; iz++)
+      for (int iy = g->jy1; iy <= g->jy2; iy++)
+        for (int ix = g->ix1; ix <= g->ix2; ix++) {
+          double v = F4(ebf, ix, iy, iz, ik);
+          s = s + v * v;
+        }
+    out[ik - 1] = dgvol * s;
+  }
+}

@@ -1,0 +1,134 @@
+/*
+ * pic_oracle.h -- CPU FP64 restatement of ALaDyn's laser-plasma (LP) PIC step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This oracle is the checker for the HIP library in
+ * aladyn_amd/csrc: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product path never calls it.
+ *
+ * Every array uses the reference's own Fortran layout (column-major, 1-based
+ * indices in the comments; the C code subtracts 1):
+ *   part(ld, 2*curr_ndim+1)  : x,y[,z], px,py[,pz], W   (struct_def.f90:26-28)
+ *   ebf(n1p, n2p, n3p, nfield): Ex,Ey,[Ez],[Bx,By],Bz   (fstruct_data.f90:27)
+ *   jc (n1p, n2p, n3p, curr_ndim)                       (fstruct_data.f90:28)
+ *   ebfp(ld, 2*curr_ndim+1) : per-particle stage buffer (pstruct_data.f90:31)
+ * W is the packed FP64 slot of precision_def.F90:33-40: int16 charge in bytes
+ * 0-1, int16 part_ind in bytes 2-3, FP32 weight in bytes 4-7.
+ *
+ * Parity pins (see tests/test_oracle_*.py):
+ *   - gather, push (incl. first-step back-shift), Esirkepov deposit and grid
+ *     charge are compared with the reference Fortran routines themselves,
+ *     compiled from the reference sources into oracle/_ref (when present),
+ *     and with closed forms / invariants (partition of unity, exact
+ *     interpolation of linear fields, uniform-E / pure-B push, continuity).
+ *   - field_xyzbd, jc_xyzbd, ef_bds/bf_bds, rote/rotb live in reference
+ *     modules that need an MPI library to compile (parallel.F90:35) and are
+ *     therefore pinned by mathematics only: Yee discrete dispersion of an exact
+ *     lattice eigenmode, div B conservation, the discrete Gauss law together
+ *     with Esirkepov continuity (pins the djc and ompe factors and the sign of
+ *     the source), exact extrapolation of linear fields, charge conservation of
+ *     the ghost fold, and the absorption of an outgoing wave by the Mur ABC.
+ */
+#ifndef ALADYN_PIC_ORACLE_H
+#define ALADYN_PIC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Grid/run constants of one (single-rank) sub-domain.  Field names follow
+ * common_param.f90 / grid_param.f90 so the meaning is the reference's. */
+typedef struct orc_grid {
+  int ndim;       /* 2 or 3 */
+  int curr_ndim;  /* 2 (2-D P-polarised) or 3 (3-D) */
+  int nfield;     /* 3 (2-D: Ex,Ey,Bz) or 6 (3-D) */
+  int nx, ny, nz; /* interior cells; nz = 1 in 2-D */
+  int n1p, n2p, n3p; /* array extents: nx+5, ny+5, nz+5 (1 in 2-D) */
+  int ix1, ix2, jy1, jy2, kz1, kz2; /* 1-based interior bounds */
+  int ibx, iby, ibz; /* boundary types: 0 open, 1 reflective, 2 periodic */
+  int der_ord;    /* 2, 3 or 4 (grid_field_param.f90:36-99) */
+  double dx, dy, dz, dx_inv, dy_inv, dz_inv;
+  double xmn, ymn, zmn; /* local origin (set_loc_grid_param) */
+  double xmx, ymx, zmx; /* particle box upper bounds (loc_*grid%gmax) */
+  double dt;      /* dt_loc */
+  double ompe;    /* 4*pi*rc0*n0_ref*1e-3 */
+  double djc[3];
+  double se_coeff[2];
+  double lorentz_fact; /* 1/m of the species being pushed */
+} orc_grid;
+
+/* Fills the derived members (extents, bounds, inverses, djc, se_coeff,
+ * box limits) from ndim, nx, ny, nz, dx, dy, dz, der_ord and cfl as
+ * set_grid / set_initial_param / set_field_param do
+ * (set_grid_param.f90:52-225, set_init_param.f90:76-102,151-162,
+ *  grid_field_param.f90:36-99).  cfl is only used for der_ord 3. */
+void orc_grid_setup(orc_grid *g, double cfl);
+
+/* Packed weight slot helpers (precision_def.F90:33-40). */
+double orc_pack_weight(int charge, int part_ind, float wgh);
+void orc_unpack_weight(double w, int *charge, int *part_ind, float *wgh);
+
+/* set_part2d_hcell_acc / set_part3d_hcell_acc (grid_part_connect.f90:89-285):
+ * ebfp(1:nfield) <- fields gathered at the particle positions. */
+void orc_gather(const orc_grid *g, int64_t np, const double *part, int64_t ld,
+                const double *ebf, double *ebfp);
+/* field_charge_multiply (curr_and_fields_util.f90:58-72). */
+void orc_charge_multiply(const orc_grid *g, int64_t np, const double *part,
+                         int64_t ld, double *ebfp);
+/* init_lpf_momenta (boris_push.f90:34-73): p^0 -> p^{-1/2}. */
+void orc_init_momenta(const orc_grid *g, int64_t np, double *part, int64_t ld,
+                      const double *ebfp);
+/* lpf_momenta_and_positions (boris_push.f90:75-168). */
+void orc_push(const orc_grid *g, int64_t np, double *part, int64_t ld,
+              double *ebfp);
+/* cell_part_dist(.false.) on one rank -> reset_all_part_dist per axis
+ * (mpi_part_interface.f90:467-523, 525-753).  Returns the new count. */
+int64_t orc_part_dist(const orc_grid *g, int64_t np, double *part, int64_t ld,
+                      double *ebfp);
+/* esirkepov_2d_curr (curr_ndim 2) / esirkepov_3d_curr
+ * (grid_part_connect.f90:956-1334): jc += raw Esirkepov current. */
+void orc_esirkepov(const orc_grid *g, int64_t np, const double *part,
+                   int64_t ld, double *ebfp, double *jc);
+/* Particles skipped by orc_esirkepov because they moved a full cell or more
+ * in one step (the reference has no guard there: its window would overflow). */
+int64_t orc_esirkepov_skipped(void);
+/* field_xyzbd (mpi_field_interface.f90:53-198). */
+void orc_field_xyzbd(const orc_grid *g, double *ebf, int nc);
+/* jc_xyzbd (mpi_curr_interface.f90:551-643) + djc normalisation
+ * (curr_and_fields_util.f90:104-152, unstretched, iform < 2). */
+void orc_curr_collect(const orc_grid *g, double *jc);
+void orc_jc_xyzbd(const orc_grid *g, double *jc);
+/* ef_bds / bf_bds with imbd = 0 (grid_fields.f90:1069-1354). */
+void orc_ef_bds(const orc_grid *g, double *ebf, double dtl);
+void orc_bf_bds(const orc_grid *g, double *ebf, double dtl);
+/* rote / rotb (grid_fields.f90:1405-1552). */
+void orc_rote(const orc_grid *g, double *ebf, double dtf);
+void orc_rotb(const orc_grid *g, double *ebf, double dtf);
+/* advance_lpf_fields(ef, curr, 0) (curr_and_fields_util.f90:176-260). */
+void orc_advance_fields(const orc_grid *g, double *ebf, const double *jc);
+/* The species-loop body of lpf2_evolve (pic_evolve_in_time.f90:75-91) with
+ * part = .true.: gather, charge multiply, [back-shift], push, drop, deposit.
+ * Returns the new particle count. */
+int64_t orc_particle_stage(const orc_grid *g, int64_t np, double *part,
+                           int64_t ld, double *ebfp, const double *ebf,
+                           double *jc, int initial_time);
+/* One full lpf2_evolve (pic_evolve_in_time.f90:37-109) for one species with
+ * part = .true. (curr_mpi_collect active).  Returns the new count. */
+int64_t orc_lpf2_evolve(const orc_grid *g, int64_t np, double *part,
+                        int64_t ld, double *ebfp, double *ebf, double *jc,
+                        int initial_time);
+/* set_grid_charge (grid_part_util.f90:240-335) for ndim 2/3: rho += q*w*S. */
+void orc_grid_charge(const orc_grid *g, int64_t np, const double *part,
+                     int64_t ld, double *rho);
+/* energy_momenta (diag_part_and_fields.f90:172-219): ek[0..6] as ek(1:7). */
+void orc_energy_momenta(const orc_grid *g, int64_t np, const double *part,
+                        int64_t ld, double ek[7], double *ekmax);
+/* Field energy of Envar (diag_part_and_fields.f90:485-670), unstretched:
+ * out[c] = dgvol * sum over ix1..ix2, jy1..nyp, kz1..nzp of ebf(c)^2. */
+void orc_field_energy(const orc_grid *g, const double *ebf, double out[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

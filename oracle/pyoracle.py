@@ -1,0 +1,238 @@
+"""ctypes wrappers of the CPU checkers -- TEST INFRASTRUCTURE ONLY.
+
+* Oracle: the C restatement (oracle/liboracle.so, pic_oracle.c).
+* RefHarness: the reference's own Fortran routines compiled with
+  oracle/ref_harness.f90 into oracle/_ref/libaladyn_ref.so (built only where
+  the reference sources exist; the built library travels with the tree).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline use this.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_LIB = HERE / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libaladyn_ref.so"
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int), ("curr_ndim", C.c_int), ("nfield", C.c_int),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("n1p", C.c_int), ("n2p", C.c_int), ("n3p", C.c_int),
+        ("ix1", C.c_int), ("ix2", C.c_int), ("jy1", C.c_int), ("jy2", C.c_int),
+        ("kz1", C.c_int), ("kz2", C.c_int),
+        ("ibx", C.c_int), ("iby", C.c_int), ("ibz", C.c_int),
+        ("der_ord", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("dx_inv", C.c_double), ("dy_inv", C.c_double), ("dz_inv", C.c_double),
+        ("xmn", C.c_double), ("ymn", C.c_double), ("zmn", C.c_double),
+        ("xmx", C.c_double), ("ymx", C.c_double), ("zmx", C.c_double),
+        ("dt", C.c_double), ("ompe", C.c_double),
+        ("djc", C.c_double * 3), ("se_coeff", C.c_double * 2),
+        ("lorentz_fact", C.c_double),
+    ]
+
+
+_D = C.POINTER(C.c_double)
+_G = C.POINTER(OrcGrid)
+_I64 = C.c_int64
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def _f(a):
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim == 1), "need Fortran-order f64"
+    return a
+
+
+class Oracle:
+    """The C restatement of the LP step on the reference layouts."""
+
+    def __init__(self, cfg, lorentz_fact: float | None = None, lib: Path | None = None):
+        p = Path(lib) if lib else ORACLE_LIB
+        if not p.exists():
+            raise ImportError(f"{p} missing: run `make -C oracle`")
+        L = C.CDLL(str(p))
+        sig = {
+            "orc_grid_setup": [_G, C.c_double],
+            "orc_gather": [_G, _I64, _D, _I64, _D, _D],
+            "orc_charge_multiply": [_G, _I64, _D, _I64, _D],
+            "orc_init_momenta": [_G, _I64, _D, _I64, _D],
+            "orc_push": [_G, _I64, _D, _I64, _D],
+            "orc_part_dist": [_G, _I64, _D, _I64, _D],
+            "orc_esirkepov": [_G, _I64, _D, _I64, _D, _D],
+            "orc_esirkepov_skipped": [],
+            "orc_field_xyzbd": [_G, _D, C.c_int],
+            "orc_curr_collect": [_G, _D],
+            "orc_jc_xyzbd": [_G, _D],
+            "orc_ef_bds": [_G, _D, C.c_double],
+            "orc_bf_bds": [_G, _D, C.c_double],
+            "orc_rote": [_G, _D, C.c_double],
+            "orc_rotb": [_G, _D, C.c_double],
+            "orc_advance_fields": [_G, _D, _D],
+            "orc_particle_stage": [_G, _I64, _D, _I64, _D, _D, _D, C.c_int],
+            "orc_lpf2_evolve": [_G, _I64, _D, _I64, _D, _D, _D, C.c_int],
+            "orc_grid_charge": [_G, _I64, _D, _I64, _D],
+            "orc_energy_momenta": [_G, _I64, _D, _I64, _D, _D],
+            "orc_field_energy": [_G, _D, _D],
+        }
+        for name, at in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = at
+            fn.restype = _I64 if name in ("orc_part_dist", "orc_particle_stage", "orc_lpf2_evolve",
+                                          "orc_esirkepov_skipped") else None
+        self.L = L
+        self.cfg = cfg
+        g = OrcGrid()
+        g.ndim, g.nx, g.ny, g.nz = cfg.ndim, cfg.nx, cfg.ny, cfg.nz
+        g.ibx, g.iby, g.ibz = cfg.ibx, cfg.iby, cfg.ibz
+        g.der_ord = cfg.der_ord
+        g.dx, g.dy, g.dz = cfg.dx, cfg.dy, cfg.dz
+        g.dt = cfg.dt
+        g.ompe = cfg.ompe
+        g.lorentz_fact = cfg.lorentz_fact[0] if lorentz_fact is None else lorentz_fact
+        L.orc_grid_setup(C.byref(g), cfg.cfl)
+        self.g = g
+
+    # array factories in the reference layouts
+    def zeros_fields(self):
+        return np.zeros(self.cfg.field_shape, dtype=np.float64, order="F")
+
+    def zeros_current(self):
+        return np.zeros(self.cfg.current_shape, dtype=np.float64, order="F")
+
+    def zeros_stage(self, n):
+        return np.zeros((max(n, 1), self.cfg.ncol), dtype=np.float64, order="F")
+
+    def _ld(self, part):
+        return max(part.shape[0], 1)
+
+    def gather(self, part, ebf):
+        ebfp = self.zeros_stage(part.shape[0])
+        self.L.orc_gather(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part), _p(_f(ebf)), _p(ebfp))
+        return ebfp
+
+    def charge_multiply(self, part, ebfp):
+        self.L.orc_charge_multiply(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part), _p(ebfp))
+
+    def init_momenta(self, part, ebfp):
+        self.L.orc_init_momenta(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part), _p(ebfp))
+
+    def push(self, part, ebfp):
+        self.L.orc_push(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part), _p(ebfp))
+
+    def part_dist(self, part, ebfp) -> int:
+        return int(self.L.orc_part_dist(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part), _p(ebfp)))
+
+    def esirkepov(self, part, ebfp, jc, np_=None):
+        n = part.shape[0] if np_ is None else np_
+        self.L.orc_esirkepov(C.byref(self.g), n, _p(_f(part)), self._ld(part), _p(ebfp), _p(_f(jc)))
+
+    def esirkepov_skipped(self) -> int:
+        return int(self.L.orc_esirkepov_skipped())
+
+    def field_xyzbd(self, ebf, nc=None):
+        self.L.orc_field_xyzbd(C.byref(self.g), _p(_f(ebf)), self.cfg.nfield if nc is None else nc)
+
+    def curr_collect(self, jc):
+        self.L.orc_curr_collect(C.byref(self.g), _p(_f(jc)))
+
+    def jc_xyzbd(self, jc):
+        self.L.orc_jc_xyzbd(C.byref(self.g), _p(_f(jc)))
+
+    def ef_bds(self, ebf, dtl):
+        self.L.orc_ef_bds(C.byref(self.g), _p(_f(ebf)), dtl)
+
+    def bf_bds(self, ebf, dtl):
+        self.L.orc_bf_bds(C.byref(self.g), _p(_f(ebf)), dtl)
+
+    def rote(self, ebf, dtf):
+        self.L.orc_rote(C.byref(self.g), _p(_f(ebf)), dtf)
+
+    def rotb(self, ebf, dtf):
+        self.L.orc_rotb(C.byref(self.g), _p(_f(ebf)), dtf)
+
+    def advance_fields(self, ebf, jc):
+        self.L.orc_advance_fields(C.byref(self.g), _p(_f(ebf)), _p(_f(jc)))
+
+    def particle_stage(self, part, ebf, jc, initial_time=False):
+        """Returns (part_new[:np], ebfp); part is modified in place."""
+        ebfp = self.zeros_stage(part.shape[0])
+        n = self.L.orc_particle_stage(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part),
+                                      _p(ebfp), _p(_f(ebf)), _p(_f(jc)), 1 if initial_time else 0)
+        return np.asfortranarray(part[:n]), ebfp
+
+    def lpf2_evolve(self, part, ebf, jc, initial_time=False):
+        """One full step; returns the surviving particles (a new array)."""
+        ebfp = self.zeros_stage(part.shape[0])
+        n = self.L.orc_lpf2_evolve(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part),
+                                   _p(ebfp), _p(_f(ebf)), _p(_f(jc)), 1 if initial_time else 0)
+        return np.asfortranarray(part[:n])
+
+    def grid_charge(self, part):
+        rho = np.zeros((self.cfg.n1p, self.cfg.n2p, self.cfg.n3p), dtype=np.float64, order="F")
+        self.L.orc_grid_charge(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part), _p(rho))
+        return rho
+
+    def energy_momenta(self, part):
+        ek = np.zeros(7)
+        emax = np.zeros(1)
+        self.L.orc_energy_momenta(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part), _p(ek), _p(emax))
+        return ek, float(emax[0])
+
+    def field_energy(self, ebf):
+        out = np.zeros(6)
+        self.L.orc_field_energy(C.byref(self.g), _p(_f(ebf)), _p(out))
+        return out[: self.cfg.nfield]
+
+
+class RefHarness:
+    """The reference's own gather / push / Esirkepov / grid-charge routines."""
+
+    def __init__(self, cfg, lorentz_fact: float | None = None, lib: Path | None = None):
+        p = Path(lib) if lib else REF_LIB
+        if not p.exists():
+            raise ImportError(f"{p} missing: run `make -C oracle ref` where the reference exists")
+        L = C.CDLL(str(p))
+        L.ref_setup.argtypes = [C.c_int] * 4 + [C.c_double] * 5
+        L.ref_gather.argtypes = [_I64, _D, _I64, _D, _D]
+        L.ref_push.argtypes = [_I64, _D, _I64, _D, C.c_int]
+        L.ref_esirkepov.argtypes = [_I64, _D, _I64, _D, _D]
+        L.ref_grid_charge.argtypes = [_I64, _D, _I64, _D]
+        L.ref_gather_push_deposit.argtypes = [_I64, _D, _I64, _D, _D, C.c_int]
+        for f in (L.ref_setup, L.ref_gather, L.ref_push, L.ref_esirkepov, L.ref_grid_charge,
+                  L.ref_gather_push_deposit):
+            f.restype = None
+        self.L = L
+        self.cfg = cfg
+        lf = cfg.lorentz_fact[0] if lorentz_fact is None else lorentz_fact
+        L.ref_setup(cfg.ndim, cfg.nx, cfg.ny, cfg.nz, cfg.dx, cfg.dy, cfg.dz, cfg.dt, lf)
+
+    def gather(self, part, ebf):
+        ebfp = np.zeros((max(part.shape[0], 1), self.cfg.ncol), order="F")
+        self.L.ref_gather(part.shape[0], _p(_f(part)), max(part.shape[0], 1), _p(_f(ebf)), _p(ebfp))
+        return ebfp
+
+    def push(self, part, ebfp, initial=False):
+        self.L.ref_push(part.shape[0], _p(_f(part)), max(part.shape[0], 1), _p(ebfp), 1 if initial else 0)
+
+    def esirkepov(self, part, ebfp, jc):
+        self.L.ref_esirkepov(part.shape[0], _p(_f(part)), max(part.shape[0], 1), _p(ebfp), _p(_f(jc)))
+
+    def grid_charge(self, part):
+        rho = np.zeros((self.cfg.n1p, self.cfg.n2p, self.cfg.n3p), order="F")
+        self.L.ref_grid_charge(part.shape[0], _p(_f(part)), max(part.shape[0], 1), _p(rho))
+        return rho
+
+    def gather_push_deposit(self, part, ebf, nsteps=1):
+        jc = np.zeros(self.cfg.current_shape, order="F")
+        self.L.ref_gather_push_deposit(part.shape[0], _p(_f(part)), max(part.shape[0], 1),
+                                       _p(_f(ebf)), _p(jc), nsteps)
+        return jc
