@@ -16,6 +16,46 @@
 
 using namespace aladyn;
 
+struct TimerRing {
+  static constexpr int kRing = 256;
+  hipEvent_t a[kRing] = {}, b[kRing] = {};
+  int64_t n = 0;      // pairs recorded since reset
+  int64_t base = 0;   // total pairs recorded (ring position)
+  int create() {
+    for (int i = 0; i < kRing; i++) {
+      if (hipEventCreate(&a[i]) != hipSuccess || hipEventCreate(&b[i]) != hipSuccess) return 1;
+    }
+    return 0;
+  }
+  void destroy() {
+    for (int i = 0; i < kRing; i++) {
+      if (a[i]) (void)hipEventDestroy(a[i]);
+      if (b[i]) (void)hipEventDestroy(b[i]);
+    }
+  }
+  hipEvent_t begin() const { return a[base % kRing]; }
+  hipEvent_t end() const { return b[base % kRing]; }
+  void commit() {
+    base++;
+    n++;
+  }
+  // average ms over the last min(n, kRing) pairs (stream must be idle)
+  double average() const {
+    const int64_t m = n < kRing ? n : kRing;
+    double sum = 0;
+    int64_t cnt = 0;
+    for (int64_t q = 1; q <= m; q++) {
+      const int i = (int)((base - q) % kRing);
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, a[i], b[i]) == hipSuccess) {
+        sum += ms;
+        cnt++;
+      }
+    }
+    return cnt ? sum / cnt : 0.0;
+  }
+};
+
 struct SpeciesF64 {
   double* part = nullptr;  // part(cap, 2D+1) on the device
   double* tmp = nullptr;   // compaction target, same shape
@@ -54,10 +94,9 @@ struct aladyn_ctx {
   double* rho1 = nullptr;
   void* jraw = nullptr;
   bool have_resid = false;
-  // timing
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  double part_ms = 0, field_ms = 0;
-  int64_t n_timed = 0;
+  // timing: event pairs around the particle kernel and the field solve,
+  // recorded on the context stream (ring of kRing steps)
+  TimerRing tpart, tfield;
 };
 
 static int fail(aladyn_ctx* c, int code, const char* fmt, ...) {
@@ -176,7 +215,8 @@ extern "C" int aladyn_create(const aladyn_config* cfg, aladyn_ctx** out) {
   *out = c;
   HIPCK(c, hipGetDevice(&c->dev));
   HIPCK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  for (auto& e : c->ev) HIPCK(c, hipEventCreate(&e));
+  if (c->tpart.create() || c->tfield.create())
+    return fail(c, ALADYN_EHIP, "event creation failed");
   const DevGrid& g = c->g;
   const size_t nf = (size_t)g.ncell * g.nfield, nj = (size_t)g.ncell * g.curr_ndim;
   if (f64(c)) {
@@ -223,8 +263,8 @@ extern "C" int aladyn_destroy(aladyn_ctx* c) {
   (void)hipFree(c->rho0);
   (void)hipFree(c->rho1);
   (void)hipFree(c->jraw);
-  for (auto& e : c->ev)
-    if (e) (void)hipEventDestroy(e);
+  c->tpart.destroy();
+  c->tfield.destroy();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return ALADYN_OK;
@@ -442,17 +482,18 @@ extern "C" int aladyn_particle_stage(aladyn_ctx* c, int32_t ic, int32_t initial_
                             (size_t)g.ncell * g.curr_ndim * esz, hipMemcpyDeviceToDevice,
                             c->stream));
   }
-  HIPCK(c, hipEventRecord(c->ev[0], c->stream));
   if (f64(c)) {
     SpeciesF64& s = c->sp64[ic];
     if (s.np > 0) {
       HIPCK(c, hipMemsetAsync(c->d_ndead, 0, sizeof(int), c->stream));
+      HIPCK(c, hipEventRecord(c->tpart.begin(), c->stream));
       particles_f64(g, s.part, s.cap, s.np, c->ebf64, c->jc64, alp, initial_time ? 1 : 0,
                     c->d_ndead, c->stream);
       HIPCK(c, hipGetLastError());
+      HIPCK(c, hipEventRecord(c->tpart.end(), c->stream));
+      c->tpart.commit();
       int ndead = 0;
       HIPCK(c, hipMemcpyAsync(&ndead, c->d_ndead, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      HIPCK(c, hipEventRecord(c->ev[1], c->stream));
       HIPCK(c, hipStreamSynchronize(c->stream));
       if (ndead > 0) {
         compact_f64(g, s.part, s.tmp, s.cap, s.np, c->d_blk, c->d_count, c->stream);
@@ -460,15 +501,14 @@ extern "C" int aladyn_particle_stage(aladyn_ctx* c, int32_t ic, int32_t initial_
         std::swap(s.part, s.tmp);
         s.np -= ndead;
       }
-    } else {
-      HIPCK(c, hipEventRecord(c->ev[1], c->stream));
     }
   } else {
     std::string msg;
     int r = tile_store_step(&c->ts[ic], c->tg, g, c->ebf32, c->jc32, c->patch, (float)alp,
-                            initial_time ? 1 : 0, c->j_is_zero, c->stream, &msg);
+                            initial_time ? 1 : 0, c->j_is_zero, c->stream, c->tpart.begin(),
+                            c->tpart.end(), &msg);
     if (r) return fail(c, r, "%s", msg.c_str());
-    HIPCK(c, hipEventRecord(c->ev[1], c->stream));
+    if (c->ts[ic].loaded) c->tpart.commit();
   }
   c->j_is_zero = false;
   if (c->diag) {
@@ -507,10 +547,11 @@ extern "C" int aladyn_lpf2_evolve(aladyn_ctx* c, int32_t iter, int32_t initial_t
   if (int r = aladyn_curr_clean(c)) return r;
   for (int ic = 0; ic < c->cfg.nsp; ic++)
     if (int r = aladyn_particle_stage(c, ic, initial_time)) return r;
-  HIPCK(c, hipEventRecord(c->ev[2], c->stream));
   if (int r = aladyn_curr_mpi_collect(c)) return r;
+  HIPCK(c, hipEventRecord(c->tfield.begin(), c->stream));
   if (int r = aladyn_advance_lpf_fields(c)) return r;
-  HIPCK(c, hipEventRecord(c->ev[3], c->stream));
+  HIPCK(c, hipEventRecord(c->tfield.end(), c->stream));
+  c->tfield.commit();
   return ALADYN_OK;
 }
 
@@ -607,12 +648,12 @@ extern "C" int aladyn_kernel_timing(aladyn_ctx* c, int32_t reset, double* partic
                                     double* field_ms, int64_t* launches) {
   if (!c) return ALADYN_EINVAL;
   HIPCK(c, hipStreamSynchronize(c->stream));
-  float pm = 0, fm = 0;
-  if (hipEventElapsedTime(&pm, c->ev[0], c->ev[1]) != hipSuccess) pm = 0;
-  if (hipEventElapsedTime(&fm, c->ev[2], c->ev[3]) != hipSuccess) fm = 0;
-  if (particle_ms) *particle_ms = pm;
-  if (field_ms) *field_ms = fm;
-  if (launches) *launches = c->n_timed;
-  if (reset) c->n_timed = 0;
+  if (particle_ms) *particle_ms = c->tpart.average();
+  if (field_ms) *field_ms = c->tfield.average();
+  if (launches) *launches = c->tpart.n;
+  if (reset) {
+    c->tpart.n = 0;
+    c->tfield.n = 0;
+  }
   return ALADYN_OK;
 }

@@ -1,29 +1,24 @@
-// particles_f32.hip -- the FP32 tile path (see tile_store.hpp for the store).
-//
-// One workgroup per tile:
-//   1. stages the tile's field patch (nodes T0-1 .. T0+T per axis, all
-//      nfield components) from HBM into LDS and zeroes an LDS current patch
-//      (nodes T0-2 .. T0+T+1: the 5-slot Esirkepov window of any particle
-//      whose nearest node lies in the tile, grid_part_connect.f90:1008);
-//   2. streams the tile's particles: gather from LDS with the alternating-
-//      order shapes, charge multiply, [first-step back-shift], exact-gamma
-//      push, box test, Esirkepov deposit into the LDS patch (LDS atomics,
-//      no global atomics in this loop), and writes each survivor into the
-//      segment of its new tile in the other buffer;
-//   3. writes its current patch to a per-tile partial-sum buffer.
-// A fold kernel then sums, for every grid node, the (at most 2 per axis)
-// tile patches covering it with compensated (Kahan) summation and writes jc.
-// Overflowed particles (destination segment full) take a global-memory
-// kernel the next step; it deposits with global atomics after the fold.
+// tile_store.hip -- host side of the FP32 tile path (see tile_store.hpp):
+// geometry, the per-step launch sequence, the fold of the per-tile current
+// patches, the global-memory kernel for overflowed particles, the re-layout
+// (counting sort with fresh capacities), transfers and FP64 diagnostics.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
 #include <vector>
 
-#include "lp_kernels.hpp"
-#include "tile_store.hpp"
+#include "tile_device.hpp"
 
 namespace aladyn {
+
+// tile_kernel.hip
+bool tile_shape_supported(int D, int tx, int ty, int tz);
+int tile_lds_bytes(int D, int tx, int ty, int tz);
+hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepConst& sc,
+                              const float* ebf, const RecPtr& A, const RecPtr& B,
+                              const int* seg_start, const int* seg_cap, const int* cnt_cur,
+                              int* cnt_next, int* ovf_cur, int* ovf_next, float* patch,
+                              hipStream_t s);
 
 // ------------------------------------------------------------ geometry
 static int bits_for(int n) {  // bits to hold values 0..n
@@ -34,10 +29,9 @@ static int bits_for(int n) {  // bits to hold values 0..n
 
 const char* tile_geom_error(int code) {
   switch (code) {
-    case 1: return "tile extent must be >= 4 nodes per tiled axis";
+    case 1: return "tile shape has no compiled kernel (3-D: 8x8x8, 8x8x4, 16x8x4; 2-D: 16x16, 32x16)";
     case 2: return "grid too large for the 31-bit packed cell index";
-    case 3: return "tile patches exceed 64 KiB of LDS; use smaller tiles";
-    case 4: return "block must be a multiple of 64 in [64, 1024]";
+    case 3: return "tile patches exceed the LDS of one workgroup";
     default: return "invalid tile geometry";
   }
 }
@@ -48,8 +42,7 @@ int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg) {
   const int def[3] = {D == 3 ? 8 : 16, D == 3 ? 8 : 16, D == 3 ? 8 : 1};
   for (int d = 0; d < 3; d++) tg->T[d] = cfg.tile[d] > 0 ? cfg.tile[d] : def[d];
   if (D == 2) tg->T[2] = 1;
-  for (int d = 0; d < D; d++)
-    if (tg->T[d] < 4) return 1;
+  if (!tile_shape_supported(D, tg->T[0], tg->T[1], tg->T[2])) return 1;
   // particle nearest nodes span Fortran indices 3 .. n+3 (x_loc in (0, n])
   const int n[3] = {g.nx, g.ny, g.nz};
   for (int d = 0; d < 3; d++) tg->nt[d] = d < D ? (n[d] + 1 + tg->T[d] - 1) / tg->T[d] : 1;
@@ -64,210 +57,12 @@ int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg) {
   tg->by = bits_for(g.n2p);
   const int bz = D == 3 ? bits_for(g.n3p) : 0;
   if (tg->bx + tg->by + bz > 31) return 2;
-  tg->block = cfg.block > 0 ? cfg.block : 256;
-  if (tg->block % 64 || tg->block < 64 || tg->block > 1024) return 4;
+  tg->block = tg->T[0] * tg->T[1] * tg->T[2];  // one thread per node-cell
   tg->sort_every = cfg.sort_every > 0 ? cfg.sort_every : 16;
-  tg->lds_bytes = (g.nfield * tg->gvol + g.curr_ndim * tg->pvol) * (int)sizeof(float);
-  if (tg->lds_bytes > 64 * 1024) return 3;
+  tg->lds_bytes = tile_lds_bytes(D, tg->T[0], tg->T[1], tg->T[2]);
+  if (tg->lds_bytes > 160 * 1024) return 3;
   return 0;
 }
-
-// ------------------------------------------------------- record helpers
-struct Rec {
-  int N[3];
-  float s[3];
-  float p[3];
-  float w;
-  int id;
-};
-
-template <int D>
-__device__ inline void load_rec(const RecPtr& r, int64_t n, Rec& q, int bx, int by) {
-  const int cell = r.cell[n];
-  q.N[0] = cell & ((1 << bx) - 1);
-  q.N[1] = (cell >> bx) & ((1 << by) - 1);
-  q.N[2] = D == 3 ? (cell >> (bx + by)) : 1;
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    q.s[d] = r.s[d][n];
-    q.p[d] = r.p[d][n];
-  }
-  q.w = r.w[n];
-  q.id = r.id ? r.id[n] : 0;
-}
-
-template <int D>
-__device__ inline void store_rec(const RecPtr& r, int64_t n, const Rec& q, int bx, int by) {
-  r.cell[n] = q.N[0] | (q.N[1] << bx) | (D == 3 ? (q.N[2] << (bx + by)) : 0);
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    r.s[d][n] = q.s[d];
-    r.p[d][n] = q.p[d];
-  }
-  r.w[n] = q.w;
-  if (r.id) r.id[n] = q.id;
-}
-
-// Fields at the particle with the reference's alternating-order shapes
-// (set_part2d_hcell_acc / set_part3d_hcell_acc, grid_part_connect.f90:89-285).
-// F(c, i, j, k) reads component c at local node (i, j, k) where (i, j, k)
-// are already relative to the accessor's origin; (bi, bj, bk) is the local
-// index of node N - 1.
-template <int D, class FA>
-__device__ inline void gather(int bi, int bj, int bk, const float s[3], const FA& F, float ef[6]) {
-  float a[3][3], ah[3][2];
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    qweights<float>(s[d], a[d]);
-    lweights<float>(s[d], ah[d]);
-  }
-  if constexpr (D == 2) {
-    float e0 = 0.f, e1 = 0.f, e2 = 0.f;
-#pragma unroll
-    for (int j1 = 0; j1 <= 2; j1++)
-#pragma unroll
-      for (int i1 = 0; i1 <= 1; i1++) e0 += (ah[0][i1] * a[1][j1]) * F(0, bi + i1, bj + j1, bk);
-#pragma unroll
-    for (int j1 = 0; j1 <= 1; j1++) {
-#pragma unroll
-      for (int i1 = 0; i1 <= 2; i1++) e1 += (a[0][i1] * ah[1][j1]) * F(1, bi + i1, bj + j1, bk);
-#pragma unroll
-      for (int i1 = 0; i1 <= 1; i1++) e2 += (ah[0][i1] * ah[1][j1]) * F(2, bi + i1, bj + j1, bk);
-    }
-    ef[0] = e0;
-    ef[1] = e1;
-    ef[2] = e2;
-  } else {
-    float ap[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int k1 = 0; k1 <= 2; k1++) {
-#pragma unroll
-      for (int j1 = 0; j1 <= 2; j1++) {
-        const float dv = a[1][j1] * a[2][k1];
-#pragma unroll
-        for (int i1 = 0; i1 <= 1; i1++) ap[0] += ah[0][i1] * dv * F(0, bi + i1, bj + j1, bk + k1);
-      }
-#pragma unroll
-      for (int j1 = 0; j1 <= 1; j1++) {
-        const float dv = ah[1][j1] * a[2][k1];
-#pragma unroll
-        for (int i1 = 0; i1 <= 2; i1++) ap[1] += a[0][i1] * dv * F(1, bi + i1, bj + j1, bk + k1);
-#pragma unroll
-        for (int i1 = 0; i1 <= 1; i1++) ap[5] += ah[0][i1] * dv * F(5, bi + i1, bj + j1, bk + k1);
-      }
-    }
-#pragma unroll
-    for (int k1 = 0; k1 <= 1; k1++) {
-#pragma unroll
-      for (int j1 = 0; j1 <= 1; j1++) {
-        const float dv = ah[1][j1] * ah[2][k1];
-#pragma unroll
-        for (int i1 = 0; i1 <= 2; i1++) ap[3] += a[0][i1] * dv * F(3, bi + i1, bj + j1, bk + k1);
-      }
-#pragma unroll
-      for (int j1 = 0; j1 <= 2; j1++) {
-        const float dv = a[1][j1] * ah[2][k1];
-#pragma unroll
-        for (int i1 = 0; i1 <= 1; i1++) ap[4] += ah[0][i1] * dv * F(4, bi + i1, bj + j1, bk + k1);
-#pragma unroll
-        for (int i1 = 0; i1 <= 2; i1++) ap[2] += a[0][i1] * dv * F(2, bi + i1, bj + j1, bk + k1);
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 6; c++) ef[c] = ap[c];
-  }
-}
-
-// Charge multiply + [init_lpf_momenta] + lpf_momenta_and_positions in FP32
-// (curr_and_fields_util.f90:58-72, boris_push.f90:34-168).  Returns
-// dt/gamma^{n+1/2}.
-template <int D>
-__device__ inline float push(float p[3], float ef[6], float qch, float alp, float dt, int initial) {
-  const int nf = D == 2 ? 3 : 6;
-#pragma unroll
-  for (int c = 0; c < nf; c++) ef[c] = qch * ef[c];
-  if constexpr (D == 2) {
-    if (initial) {
-      const float e1 = -alp * ef[0], e2 = -alp * ef[1], e3 = -alp * ef[2];
-      const float gi = 1.f / sqrtf(1.f + (p[0] * p[0] + p[1] * p[1]));
-      const float v1 = p[0] * gi, v2 = p[1] * gi;
-      p[0] = p[0] + e1 + v2 * e3;
-      p[1] = p[1] + e2 - v1 * e3;
-    }
-    const float v1 = p[0] + alp * ef[0], v2 = p[1] + alp * ef[1], v3 = alp * ef[2];
-    const float b2 = v3 * v3;
-    const float gam02 = (1.f + (v1 * v1 + v2 * v2)) - b2;
-    const float gam2 = 0.5f * (gam02 + sqrtf(gam02 * gam02 + 4.f * b2));
-    const float gam = sqrtf(gam2);
-    const float r = 1.f / (gam2 + b2);
-    const float h1 = (gam2 * v1 + gam * v2 * v3) * r;
-    const float h2 = (gam2 * v2 - gam * v1 * v3) * r;
-    p[0] = 2.f * h1 - p[0];
-    p[1] = 2.f * h2 - p[1];
-    return dt / sqrtf(1.f + (p[0] * p[0] + p[1] * p[1]));
-  } else {
-    if (initial) {
-      float e[6];
-#pragma unroll
-      for (int c = 0; c < 6; c++) e[c] = -alp * ef[c];
-      const float gi = 1.f / sqrtf(1.f + (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
-      const float v1 = gi * p[0], v2 = gi * p[1], v3 = gi * p[2];
-      p[0] = p[0] + e[0] + v2 * e[5] - v3 * e[4];
-      p[1] = p[1] + e[1] + v3 * e[3] - v1 * e[5];
-      p[2] = p[2] + e[2] + v1 * e[4] - v2 * e[3];
-    }
-    const float vp[3] = {p[0] + alp * ef[0], p[1] + alp * ef[1], p[2] + alp * ef[2]};
-    const float bb[3] = {alp * ef[3], alp * ef[4], alp * ef[5]};
-    const float b2 = bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2];
-    const float bv = bb[0] * vp[0] + bb[1] * vp[1] + bb[2] * vp[2];
-    const float gam02 = (1.f + (vp[0] * vp[0] + vp[1] * vp[1] + vp[2] * vp[2])) - b2;
-    const float gam2 = 0.5f * (gam02 + sqrtf(gam02 * gam02 + 4.f * (b2 + bv * bv)));
-    const float gam = sqrtf(gam2);
-    const float r = 1.f / (b2 + gam2);
-    float vph[3];
-    vph[0] = gam2 * vp[0] + bb[0] * bv + gam * (vp[1] * bb[2] - vp[2] * bb[1]);
-    vph[1] = gam2 * vp[1] + bb[1] * bv + gam * (vp[2] * bb[0] - vp[0] * bb[2]);
-    vph[2] = gam2 * vp[2] + bb[2] * bv + gam * (vp[0] * bb[1] - vp[1] * bb[0]);
-#pragma unroll
-    for (int c = 0; c < 3; c++) p[c] = 2.f * (vph[c] * r) - p[c];
-    return dt / sqrtf(1.f + (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
-  }
-}
-
-// x^{n+1} = x^n + dt v: offset update with an exact integer carry; the box
-// test is (x - gmin) > 0 and (x - gmin) <= n*dx, i.e. node 3 + (0, n] in
-// local units (mpi_part_interface.f90:494-498), evaluated without rounding.
-template <int D>
-__device__ inline bool move_and_test(Rec& q, float dtg, const float inv[3], const int nn[3]) {
-  bool keep = true;
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    float s = q.s[d] + (dtg * inv[d]) * q.p[d];
-    int N = q.N[d];
-    if (s >= 0.5f) {
-      N += 1;
-      s -= 1.f;
-    } else if (s < -0.5f) {
-      N -= 1;
-      s += 1.f;
-    }
-    q.N[d] = N;
-    q.s[d] = s;
-    const bool fin = fabsf(s) <= 0.5f;  // false for NaN
-    const bool lo = (N > 3) || (N == 3 && s > 0.f);
-    const bool hi = (N < nn[d] + 3) || (N == nn[d] + 3 && s <= 0.f);
-    keep = keep && fin && lo && hi;
-  }
-  return keep;
-}
-
-struct LdsAcc {
-  float* j;
-  int ox, oy, oz, jx, jxy, pvol;
-  __device__ void operator()(int c, int i, int jj, int k, float v) const {
-    atomicAdd(j + c * pvol + (k - oz) * jxy + (jj - oy) * jx + (i - ox), v);
-  }
-};
 
 struct GlobalAcc32 {
   float* jc;
@@ -277,113 +72,10 @@ struct GlobalAcc32 {
   }
 };
 
-struct LdsField {
-  const float* f;
-  int gx, gxy, gvol;
-  __device__ float operator()(int c, int i, int j, int k) const {
-    return f[c * gvol + k * gxy + j * gx + i];
-  }
-};
-
-struct GlobalField {
-  const float* f;
-  const DevGrid* g;
-  __device__ float operator()(int c, int i, int j, int k) const {
-    return f[(int64_t)c * g->ncell + fofs(*g, i, j, k)];
-  }
-};
-
-struct StepConst {
-  float alp, qch, dt;
-  float inv[3];
-  int nn[3];
-  int initial;
-  int64_t ovf_base;
-};
-
-// --------------------------------------------------------- tile kernel
-template <int D>
-__global__ __launch_bounds__(256) void k_tile_particles(
-    TileGeom tg, DevGrid g, StepConst sc, const float* __restrict__ ebf, RecPtr A, RecPtr B,
-    const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
-    const int* __restrict__ cnt_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_next,
-    float* __restrict__ patch) {
-  extern __shared__ float lds[];
-  __shared__ int hist[27];
-  __shared__ int hbase[27];
-  constexpr int nf = D == 3 ? 6 : 3;
-  float* Fl = lds;
-  float* Jl = lds + nf * tg.gvol;
-  const int t = blockIdx.x;
-  const int tx = t % tg.nt[0], ty = (t / tg.nt[0]) % tg.nt[1], tz = t / (tg.nt[0] * tg.nt[1]);
-  const int T0[3] = {3 + tx * tg.T[0], 3 + ty * tg.T[1], D == 3 ? 3 + tz * tg.T[2] : 1};
-  const int G0[3] = {T0[0] - 1, T0[1] - 1, D == 3 ? T0[2] - 1 : 1};
-  const int J0[3] = {T0[0] - 2, T0[1] - 2, D == 3 ? T0[2] - 2 : 1};
-  const int gxy = tg.G[0] * tg.G[1];
-  for (int q = threadIdx.x; q < nf * tg.gvol; q += blockDim.x) {
-    const int c = q / tg.gvol, r = q - c * tg.gvol;
-    const int li = r % tg.G[0], lj = (r / tg.G[0]) % tg.G[1], lk = r / gxy;
-    const int i = G0[0] + li, j = G0[1] + lj, k = G0[2] + lk;
-    float v = 0.f;
-    if (i <= g.n1p && j <= g.n2p && k <= g.n3p) v = ebf[(int64_t)c * g.ncell + fofs(g, i, j, k)];
-    Fl[q] = v;
-  }
-  for (int q = threadIdx.x; q < D * tg.pvol; q += blockDim.x) Jl[q] = 0.f;
-  __syncthreads();
-  const LdsField F{Fl, tg.G[0], gxy, tg.gvol};
-  const LdsAcc acc{Jl, J0[0], J0[1], J0[2], tg.J[0], tg.J[0] * tg.J[1], tg.pvol};
-  const int cnt = min(cnt_cur[t], seg_cap[t]);
-  const int64_t base = seg_start[t];
-  for (int it = 0; it < cnt; it += blockDim.x) {
-    if (threadIdx.x < 27) hist[threadIdx.x] = 0;
-    __syncthreads();
-    const int n = it + threadIdx.x;
-    bool keep = false;
-    int code = 0, lrank = 0;
-    Rec q;
-    if (n < cnt) {
-      load_rec<D>(A, base + n, q, tg.bx, tg.by);
-      float ef[6];
-      gather<D>(q.N[0] - 1 - G0[0], q.N[1] - 1 - G0[1], D == 3 ? q.N[2] - 1 - G0[2] : 0, q.s, F, ef);
-      const int io[3] = {q.N[0], q.N[1], q.N[2]};
-      const float so[3] = {q.s[0], q.s[1], q.s[2]};
-      const float dtg = push<D>(q.p, ef, sc.qch, sc.alp, sc.dt, sc.initial);
-      keep = move_and_test<D>(q, dtg, sc.inv, sc.nn);
-      if (keep) {
-        esirkepov<D, float>(io, so, q.N, q.s, sc.qch * q.w, acc);
-        int dc[3] = {0, 0, 0};
-        dc[0] = (q.N[0] - 3) / tg.T[0] - tx;
-        dc[1] = (q.N[1] - 3) / tg.T[1] - ty;
-        if (D == 3) dc[2] = (q.N[2] - 3) / tg.T[2] - tz;
-        code = (dc[0] + 1) + 3 * (dc[1] + 1) + 9 * (dc[2] + 1);
-        lrank = atomicAdd(&hist[code], 1);
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x < 27) {
-      const int h = hist[threadIdx.x];
-      if (h > 0) {
-        const int c = threadIdx.x;
-        const int dtile = (tx + c % 3 - 1) + tg.nt[0] * ((ty + (c / 3) % 3 - 1) + tg.nt[1] * (tz + c / 9 - 1));
-        hbase[c] = atomicAdd(&cnt_next[dtile], h);
-      }
-    }
-    __syncthreads();
-    if (keep) {
-      const int dtile = (tx + code % 3 - 1) +
-                        tg.nt[0] * ((ty + (code / 3) % 3 - 1) + tg.nt[1] * (tz + code / 9 - 1));
-      const int slot = hbase[code] + lrank;
-      const int64_t dst = slot < seg_cap[dtile] ? (int64_t)seg_start[dtile] + slot
-                                                : sc.ovf_base + atomicAdd(ovf_next, 1);
-      store_rec<D>(B, dst, q, tg.bx, tg.by);
-    }
-  }
-  __syncthreads();
-  float* P = patch + (int64_t)t * D * tg.pvol;
-  for (int q = threadIdx.x; q < D * tg.pvol; q += blockDim.x) P[q] = Jl[q];
-}
-
 // ----------------------------------------------------- overflow kernel
+// Particles whose destination segment was full: gather from HBM, deposit
+// with global atomics (after the fold has written jc), re-bin like the tile
+// kernel.  Runs every step; usually there is nothing to do.
 template <int D>
 __global__ __launch_bounds__(256) void k_ovf_particles(
     TileGeom tg, DevGrid g, StepConst sc, const float* __restrict__ ebf, RecPtr A, RecPtr B,
@@ -391,13 +83,16 @@ __global__ __launch_bounds__(256) void k_ovf_particles(
     const int* __restrict__ ovf_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_next,
     float* __restrict__ jc) {
   const int cnt = *ovf_cur;
-  const GlobalField F{ebf, &g};
   const GlobalAcc32 acc{jc, &g};
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < cnt; n += gridDim.x * blockDim.x) {
     Rec q;
     load_rec<D>(A, sc.ovf_base + n, q, tg.bx, tg.by);
+    const int i0 = q.N[0] - 1, j0 = q.N[1] - 1, k0 = D == 3 ? q.N[2] - 1 : 1;
+    const auto F = [&](int c, int i1, int j1, int k1) -> float {
+      return ebf[(int64_t)c * g.ncell + fofs(g, i0 + i1, j0 + j1, D == 3 ? k0 + k1 : 1)];
+    };
     float ef[6];
-    gather<D>(q.N[0] - 1, q.N[1] - 1, D == 3 ? q.N[2] - 1 : 1, q.s, F, ef);
+    gather<D>(q.s, F, ef);
     const int io[3] = {q.N[0], q.N[1], q.N[2]};
     const float so[3] = {q.s[0], q.s[1], q.s[2]};
     const float dtg = push<D>(q.p, ef, sc.qch, sc.alp, sc.dt, sc.initial);
@@ -405,7 +100,7 @@ __global__ __launch_bounds__(256) void k_ovf_particles(
     esirkepov<D, float>(io, so, q.N, q.s, sc.qch * q.w, acc);
     const int dtile = (q.N[0] - 3) / tg.T[0] +
                       tg.nt[0] * ((q.N[1] - 3) / tg.T[1] + tg.nt[1] * (D == 3 ? (q.N[2] - 3) / tg.T[2] : 0));
-    const int slot = atomicAdd(&cnt_next[dtile], 1);
+    const int slot = reserve_slot(dtile, cnt_next);
     const int64_t dst = slot < seg_cap[dtile] ? (int64_t)seg_start[dtile] + slot
                                               : sc.ovf_base + atomicAdd(ovf_next, 1);
     store_rec<D>(B, dst, q, tg.bx, tg.by);
@@ -414,7 +109,7 @@ __global__ __launch_bounds__(256) void k_ovf_particles(
 
 // ------------------------------------------------------------ fold kernel
 // jc(node) [+]= sum over the tile patches covering node (at most two tiles
-// per axis since T >= 4), compensated summation.
+// per axis since T >= 4), compensated (Kahan) summation.
 __global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g, const float* __restrict__ patch,
                                               float* __restrict__ jc, int accumulate) {
   const int D = g.ndim;
@@ -435,7 +130,7 @@ __global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g, const floa
       int thi = (u + 2) / tg.T[d];
       if (thi > tg.nt[d] - 1) thi = tg.nt[d] - 1;
       for (int tt = (thi > 0 ? thi - 1 : 0); tt <= thi; tt++) {
-        const int o = 1 + tt * tg.T[d];  // patch origin J0 = T0 - 2
+        const int o = 1 + tt * tg.T[d];  // patch origin T0 - 2
         const int l = idx[d] - o;
         if (l >= 0 && l < tg.J[d]) {
           tl[d][nt[d]] = tt;
@@ -472,13 +167,6 @@ static unsigned grid_for(int64_t n, int64_t cap = 65535LL * 8) {
   return (unsigned)b;
 }
 
-__device__ inline int tile_of(const TileGeom& tg, int cell, int D) {
-  const int i = cell & ((1 << tg.bx) - 1);
-  const int j = (cell >> tg.bx) & ((1 << tg.by) - 1);
-  const int k = D == 3 ? (cell >> (tg.bx + tg.by)) : 3;
-  return (i - 3) / tg.T[0] + tg.nt[0] * ((j - 3) / tg.T[1] + tg.nt[1] * (D == 3 ? (k - 3) / tg.T[2] : 0));
-}
-
 __device__ inline int block_scan_excl(int v, int* sh, int& total) {
   const int t = threadIdx.x;
   sh[t] = v;
@@ -495,20 +183,18 @@ __device__ inline int block_scan_excl(int v, int* sh, int& total) {
   return ex;
 }
 
-// out[t] = exclusive scan of f(in[t]) for t < n, out[n] = total; one workgroup.
-// mode 0: f = min(cnt, cap); mode 1: f = capacity for count (cnt + cnt/4 + 16)
-__global__ __launch_bounds__(kB) void k_scan_tiles(const int* cnt, const int* cap, int n, int mode,
-                                                    int* out, int* cap_out) {
+// out[t] = exclusive scan of v[t], out[n] = total (one workgroup); with
+// cap_out, v is turned into the capacity v + v/4 + 16 first.
+__global__ __launch_bounds__(kB) void k_scan_tiles(const int* v_in, int n, int* out, int* cap_out) {
   __shared__ int sh[kB];
   long long carry = 0;
   for (int b0 = 0; b0 < n; b0 += kB) {
     const int t = b0 + threadIdx.x;
     int v = 0;
     if (t < n) {
-      if (mode == 0) {
-        v = min(cnt[t], cap[t]);
-      } else {
-        v = cnt[t] + cnt[t] / 4 + 16;
+      v = v_in[t];
+      if (cap_out) {
+        v = v + v / 4 + 16;
         cap_out[t] = v;
       }
     }
@@ -520,29 +206,43 @@ __global__ __launch_bounds__(kB) void k_scan_tiles(const int* cnt, const int* ca
   if (threadIdx.x == 0) out[n] = (int)carry;
 }
 
-template <int D>
-__device__ inline void copy_rec(const RecPtr& a, int64_t i, const RecPtr& b, int64_t o) {
-  b.cell[o] = a.cell[i];
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    b.s[d][o] = a.s[d][i];
-    b.p[d][o] = a.p[d][i];
+// live (non-hole) records per tile: live[t], and the overflow count at [ntiles]
+__global__ __launch_bounds__(kB) void k_count_live(RecPtr A, const int* seg_start, const int* seg_cap,
+                                                   const int* cnt, int ntiles, const int* ovf_cnt,
+                                                   int* live) {
+  __shared__ int sh[kB];
+  const int t = blockIdx.x;
+  if (t == ntiles) {
+    if (threadIdx.x == 0) live[ntiles] = *ovf_cnt;
+    return;
   }
-  b.w[o] = a.w[i];
-  if (a.id && b.id) b.id[o] = a.id[i];
+  const int n = min(cnt[t], seg_cap[t]);
+  int c = 0;
+  for (int q = threadIdx.x; q < n; q += kB) c += A.cell[(int64_t)seg_start[t] + q] != kHole;
+  int total;
+  block_scan_excl(c, sh, total);
+  if (threadIdx.x == 0) live[t] = total;
 }
 
-// live records of A (tile segments, then overflow) -> contiguous U
+// live records of A (tile segments in order, then the overflow list) -> U
 template <int D>
 __global__ __launch_bounds__(kB) void k_gather_live(RecPtr A, RecPtr U, const int* seg_start,
                                                     const int* seg_cap, const int* cnt,
                                                     const int* off, int ntiles, int64_t ovf_base,
                                                     const int* ovf_cnt) {
+  __shared__ int sh[kB];
   const int t = blockIdx.x;
   if (t < ntiles) {
     const int n = min(cnt[t], seg_cap[t]);
-    for (int q = threadIdx.x; q < n; q += blockDim.x)
-      copy_rec<D>(A, (int64_t)seg_start[t] + q, U, (int64_t)off[t] + q);
+    int dst = off[t];
+    for (int q0 = 0; q0 < n; q0 += kB) {
+      const int q = q0 + threadIdx.x;
+      const bool ok = q < n && A.cell[(int64_t)seg_start[t] + q] != kHole;
+      int total;
+      const int ex = block_scan_excl(ok ? 1 : 0, sh, total);
+      if (ok) copy_rec<D>(A, (int64_t)seg_start[t] + q, U, (int64_t)dst + ex);
+      dst += total;
+    }
   } else {
     const int n = *ovf_cnt;
     const int64_t o0 = off[ntiles];
@@ -555,7 +255,7 @@ template <int D>
 __global__ void k_hist(TileGeom tg, RecPtr U, int64_t n, int* hcount) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x)
-    atomicAdd(&hcount[tile_of(tg, U.cell[q], D)], 1);
+    (void)reserve_slot(tile_of(tg, U.cell[q], D), hcount);
 }
 
 template <int D>
@@ -564,7 +264,7 @@ __global__ void k_scatter(TileGeom tg, RecPtr U, int64_t n, RecPtr A, const int*
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
     const int t = tile_of(tg, U.cell[q], D);
-    const int slot = atomicAdd(&fillc[t], 1);
+    const int slot = reserve_slot(t, fillc);
     copy_rec<D>(U, q, A, (int64_t)seg_start[t] + slot);
   }
 }
@@ -672,7 +372,7 @@ static int sort_into_cur(TileStore* ts, const TileGeom& tg, int64_t n, hipStream
   int* h = ts->count[cur];
   fill<int>(h, tg.ntiles, 0, s);
   if (n > 0) k_hist<D><<<grid_for(n), kB, 0, s>>>(tg, ts->buf[oth], n, h);
-  k_scan_tiles<<<1, kB, 0, s>>>(h, nullptr, tg.ntiles, 1, ts->seg_start, ts->seg_cap);
+  k_scan_tiles<<<1, kB, 0, s>>>(h, tg.ntiles, ts->seg_start, ts->seg_cap);
   int total = 0;
   TS_CK(hipMemcpyAsync(&total, ts->seg_start + tg.ntiles, sizeof(int), hipMemcpyDeviceToHost, s));
   TS_CK(hipStreamSynchronize(s));
@@ -727,7 +427,7 @@ int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const do
     TS_CK(hipMalloc(&ts->seg_start, sizeof(int) * (tg.ntiles + 1)));
     TS_CK(hipMalloc(&ts->seg_cap, sizeof(int) * (tg.ntiles + 1)));
     TS_CK(hipMalloc(&ts->ovf_count, sizeof(int) * 2));
-    TS_CK(hipMalloc(&ts->scratch, sizeof(int) * (tg.ntiles + 1)));
+    TS_CK(hipMalloc(&ts->scratch, sizeof(int) * (tg.ntiles + 2)));
     TS_CK(hipMalloc(&ts->d_total, 64));
     TS_CK(hipHostMalloc(&ts->h_ovf, 2 * sizeof(int), hipHostMallocDefault));
     TS_CK(hipEventCreateWithFlags(&ts->ovf_ev, hipEventDisableTiming));
@@ -764,16 +464,18 @@ template <int D>
 static int gather_live(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t* n,
                        std::string* msg) {
   const int cur = ts->cur, oth = 1 - cur;
-  k_scan_tiles<<<1, kB, 0, s>>>(ts->count[cur], ts->seg_cap, tg.ntiles, 0, ts->scratch, nullptr);
+  int* live = ts->count[oth];  // free between steps
+  k_count_live<<<tg.ntiles + 1, kB, 0, s>>>(ts->buf[cur], ts->seg_start, ts->seg_cap,
+                                            ts->count[cur], tg.ntiles, ts->ovf_count + cur, live);
+  k_scan_tiles<<<1, kB, 0, s>>>(live, tg.ntiles + 1, ts->scratch, nullptr);
   k_gather_live<D><<<tg.ntiles + 64, kB, 0, s>>>(ts->buf[cur], ts->buf[oth], ts->seg_start,
                                                  ts->seg_cap, ts->count[cur], ts->scratch,
                                                  tg.ntiles, ts->ovf_base, ts->ovf_count + cur);
   TS_CK(hipGetLastError());
-  int h[2] = {0, 0};
-  TS_CK(hipMemcpyAsync(&h[0], ts->scratch + tg.ntiles, sizeof(int), hipMemcpyDeviceToHost, s));
-  TS_CK(hipMemcpyAsync(&h[1], ts->ovf_count + cur, sizeof(int), hipMemcpyDeviceToHost, s));
+  int tot = 0;
+  TS_CK(hipMemcpyAsync(&tot, ts->scratch + tg.ntiles + 1, sizeof(int), hipMemcpyDeviceToHost, s));
   TS_CK(hipStreamSynchronize(s));
-  *n = (int64_t)h[0] + h[1];
+  *n = tot;
   return 0;
 }
 
@@ -791,15 +493,16 @@ int tile_store_count(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t* 
                      std::string* msg) {
   *np = 0;
   if (!ts->loaded) return 0;
-  // live = sum over tiles of min(count, cap) + overflow
-  const int cur = ts->cur;
-  k_scan_tiles<<<1, kB, 0, s>>>(ts->count[cur], ts->seg_cap, tg.ntiles, 0, ts->scratch, nullptr);
+  const int cur = ts->cur, oth = 1 - cur;
+  int* live = ts->count[oth];
+  k_count_live<<<tg.ntiles + 1, kB, 0, s>>>(ts->buf[cur], ts->seg_start, ts->seg_cap,
+                                            ts->count[cur], tg.ntiles, ts->ovf_count + cur, live);
+  k_scan_tiles<<<1, kB, 0, s>>>(live, tg.ntiles + 1, ts->scratch, nullptr);
   TS_CK(hipGetLastError());
-  int h[2] = {0, 0};
-  TS_CK(hipMemcpyAsync(&h[0], ts->scratch + tg.ntiles, sizeof(int), hipMemcpyDeviceToHost, s));
-  TS_CK(hipMemcpyAsync(&h[1], ts->ovf_count + cur, sizeof(int), hipMemcpyDeviceToHost, s));
+  int tot = 0;
+  TS_CK(hipMemcpyAsync(&tot, ts->scratch + tg.ntiles + 1, sizeof(int), hipMemcpyDeviceToHost, s));
   TS_CK(hipStreamSynchronize(s));
-  *np = (int64_t)h[0] + h[1];
+  *np = tot;
   return 0;
 }
 
@@ -852,7 +555,7 @@ int tile_store_get(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* 
 
 int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const float* ebf,
                     float* jc, float* patch, float alp, int initial, bool j_is_zero,
-                    hipStream_t s, std::string* msg) {
+                    hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg) {
   if (!ts->loaded) return 0;
   const int D = g.ndim, cur = ts->cur, nxt = 1 - cur;
   StepConst sc;
@@ -869,15 +572,11 @@ int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const f
   sc.ovf_base = ts->ovf_base;
   fill<int>(ts->count[nxt], tg.ntiles, 0, s);
   fill<int>(ts->ovf_count + nxt, 1, 0, s);
-  if (D == 2)
-    k_tile_particles<2><<<tg.ntiles, tg.block, tg.lds_bytes, s>>>(
-        tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt], ts->seg_start, ts->seg_cap, ts->count[cur],
-        ts->count[nxt], ts->ovf_count + nxt, patch);
-  else
-    k_tile_particles<3><<<tg.ntiles, tg.block, tg.lds_bytes, s>>>(
-        tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt], ts->seg_start, ts->seg_cap, ts->count[cur],
-        ts->count[nxt], ts->ovf_count + nxt, patch);
-  TS_CK(hipGetLastError());
+  if (ev_begin) TS_CK(hipEventRecord(ev_begin, s));
+  TS_CK(launch_tile_kernel(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt], ts->seg_start, ts->seg_cap,
+                           ts->count[cur], ts->count[nxt], ts->ovf_count + cur,
+                           ts->ovf_count + nxt, patch, s));
+  if (ev_end) TS_CK(hipEventRecord(ev_end, s));
   k_fold<<<grid_for(g.ncell, 8192), kB, 0, s>>>(tg, g, patch, jc, j_is_zero ? 0 : 1);
   if (D == 2)
     k_ovf_particles<2><<<512, kB, 0, s>>>(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt], ts->seg_start,

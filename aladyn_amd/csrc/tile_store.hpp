@@ -84,7 +84,7 @@ int tile_store_get(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* 
 // jc(raw) = (j_is_zero ? 0 : jc) + this species' Esirkepov current.
 int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const float* ebf,
                     float* jc, float* patch, float alp, int initial, bool j_is_zero,
-                    hipStream_t s, std::string* msg);
+                    hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg);
 int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
                         std::string* msg);
 void tile_store_grid_charge(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* rho,

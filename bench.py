@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Throughput of the LP PIC step (gather + push + Esirkepov + Yee) on 1 GPU.
+
+Workload: 3-D LWFA, grid 256x128x128, dx = 0.05, dy = dz = 0.2 (micron),
+n/nc = 0.005, 8 particles per cell on a 2x2x2 sub-grid with seeded jitter
+(33,554,432 electrons), linearly polarised Gaussian pulse a0 = 3, waist 10.
+A step is one full lpf2_evolve: field ghosts, gather, push, removal,
+Esirkepov deposit, current fold/normalisation and the Yee update.
+
+The FP32 tile path is the measured value; the FP64 reference-layout path runs
+the same workload for the FP32/FP64 ratio.  Inputs are resident in HBM before
+timing starts.  `roofline` refers to the dominant kernel (the FP32 particle
+kernel), its duration measured with HIP events on the context stream over the
+timed steps; `cpu_baseline` times the reference's own Fortran gather + push +
+deposit (oracle/_ref, built from the reference sources) on one host core over
+a bounded particle sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "particle-steps/s (gather+Boris+Esirkepov), FP32 vs FP64 on 1 GPU; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_path(cfg, part, ebf, precision, steps, warmup):
+    from aladyn_amd import LPStep
+
+    gpu = LPStep(cfg, precision)
+    gpu.put_species(0, part)
+    gpu.put_fields(ebf)
+    gpu.synchronize()
+    for it in range(warmup):
+        gpu.lpf2_evolve(it, initial_time=(it == 0))
+    gpu.synchronize()
+    n0 = gpu.species_count(0)
+    gpu.kernel_timing(reset=True)
+    t0 = time.perf_counter()
+    for it in range(warmup, warmup + steps):
+        gpu.lpf2_evolve(it, initial_time=(it == 0))
+    gpu.synchronize()
+    t1 = time.perf_counter()
+    part_ms, field_ms = gpu.kernel_timing()
+    n1 = gpu.species_count(0)
+    ek, _ = gpu.energy_momenta(0)
+    fe = gpu.field_energy()
+    gpu.close()
+    wall = t1 - t0
+    n_avg = 0.5 * (n0 + n1)
+    return dict(wall=wall, ms_per_step=1e3 * wall / steps, n0=n0, n1=n1,
+                pps=n_avg * steps / wall, part_ms=part_ms, field_ms=field_ms,
+                kinetic=float(ek[6]), field_energy=float(fe.sum()))
+
+
+def cpu_baseline(cfg, part, ebf, budget_s=15.0):
+    """Reference Fortran gather+push+deposit on one core over a sample."""
+    import numpy as np
+
+    from oracle.pyoracle import REF_LIB, RefHarness
+
+    if not REF_LIB.exists():
+        return None
+    ref = RefHarness(cfg)
+    n = 200_000
+    sub = np.asfortranarray(part[:n])
+    t0 = time.perf_counter()
+    ref.gather_push_deposit(sub.copy(order="F"), ebf, 1)
+    t_probe = time.perf_counter() - t0
+    steps = max(1, min(20, int(budget_s / max(t_probe, 1e-3))))
+    t0 = time.perf_counter()
+    ref.gather_push_deposit(sub.copy(order="F"), ebf, steps)
+    t = time.perf_counter() - t0
+    return dict(value=n * steps / t, unit="particle-steps/s", cores=1, kind="reference",
+                sample=f"{n} particles of this workload x {steps} steps (gather+push+Esirkepov, "
+                       f"FP64, reference Fortran routines compiled -O2 -fdefault-real-8), {t:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--ny", type=int, default=128)
+    ap.add_argument("--nz", type=int, default=128)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the step would shard by spatial domain; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from aladyn_amd import PREC_F32, PREC_F64
+    from aladyn_amd.inputs import lwfa_3d_case
+
+    t = time.perf_counter()
+    cfg, part, ebf = lwfa_3d_case(args.nx, args.ny, args.nz, seed=0)
+    log(f"inputs: {part.shape[0]} particles, grid {cfg.nx}x{cfg.ny}x{cfg.nz} "
+        f"({time.perf_counter() - t:.1f} s)")
+
+    r32 = run_path(cfg, part, ebf, PREC_F32, args.steps, args.warmup)
+    log(f"fp32: {r32['ms_per_step']:.3f} ms/step, {r32['pps']:.4g} particle-steps/s, "
+        f"particle kernel {r32['part_ms']:.3f} ms, fields {r32['field_ms']:.3f} ms")
+    r64 = None
+    if not args.no_fp64:
+        r64 = run_path(cfg, part, ebf, PREC_F64, max(5, args.steps // 5), 2)
+        log(f"fp64: {r64['ms_per_step']:.3f} ms/step, {r64['pps']:.4g} particle-steps/s, "
+            f"particle kernel {r64['part_ms']:.3f} ms, fields {r64['field_ms']:.3f} ms")
+
+    n_p = 0.5 * (r32["n0"] + r32["n1"])
+    cells = cfg.nx * cfg.ny * cfg.nz
+    # dominant kernel = FP32 particle kernel: particle state 60 B/particle
+    # (32 B read, 28 B written) + field gather 24 B/cell + current 12 B/cell
+    kern_bytes = n_p * 60.0 + cells * (24.0 + 12.0)
+    achieved = kern_bytes / (r32["part_ms"] * 1e-3) / 1e9 if r32["part_ms"] > 0 else 0.0
+    step_bytes = cfg.algorithmic_bytes(n_p, 32)
+    step_gbs = step_bytes / (r32["ms_per_step"] * 1e-3) / 1e9
+
+    cpu = None if args.no_cpu else cpu_baseline(cfg, part, ebf)
+    if cpu:
+        log(f"cpu baseline: {cpu['value']:.4g} particle-steps/s ({cpu['sample']})")
+
+    out = {
+        "metric": METRIC,
+        "value": r32["pps"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded plasma + Gaussian laser, no checkpoint)",
+        "config": {
+            "workload": f"3-D LWFA {cfg.nx}x{cfg.ny}x{cfg.nz}, dx=0.05 dy=dz=0.2, n/nc=0.005, "
+                        f"8 ppc (2x2x2, jittered, seed 0), a0=3 w0=10, quadratic shapes, Yee 2nd order",
+            "particles": int(r32["n0"]),
+            "cells": cells,
+            "path": "fp32 tile-sorted (int32 cell + fp32 offset)",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_tile_particles<3>",
+            "kernel_ms": r32["part_ms"],
+            "algorithmic_bytes_per_launch": kern_bytes,
+        },
+        "step_hbm_frac": step_gbs / HBM_PEAK_GBS,
+        "cpu_baseline": cpu,
+    }
+    if r64:
+        out["fp64"] = {"value": r64["pps"], "ms_per_step": r64["ms_per_step"],
+                       "particle_kernel_ms": r64["part_ms"], "field_ms": r64["field_ms"]}
+        out["fp32_over_fp64"] = r32["pps"] / r64["pps"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
