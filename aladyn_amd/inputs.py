@@ -167,8 +167,14 @@ def lwfa_2d_config(nx=512, ny=256) -> LPConfig:
 
 
 def lwfa_3d_config(nx=256, ny=128, nz=128, n_over_nc=0.005) -> LPConfig:
-    """3-D LWFA: dx = 0.05, dy = dz = 0.2, cfl 0.9."""
-    return LPConfig(ndim=3, nx=nx, ny=ny, nz=nz, k0=20.0, yx_rat=4.0, zx_rat=4.0, cfl=0.9,
+    """3-D LWFA: dx = 0.05, dy = dz = 0.2, cfl 0.8.
+
+    cfl 0.8 is the value of the reference's example inputs
+    (examples/input_lwfa.nml:122).  With open boundaries the first-order Mur
+    conditions meet at the box corners, and there the 3-D Yee update grows
+    exponentially at cfl 0.9 (a property of the scheme, reproduced by the
+    oracle; tests/test_oracle_math.py checks 0.8 is stable)."""
+    return LPConfig(ndim=3, nx=nx, ny=ny, nz=nz, k0=20.0, yx_rat=4.0, zx_rat=4.0, cfl=0.8,
                     n_over_nc=n_over_nc)
 
 
