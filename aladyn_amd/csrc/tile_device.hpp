@@ -44,6 +44,28 @@ __device__ inline void load_rec(const RecPtr& r, int64_t n, Rec& q, int bx, int 
   q.id = r.id ? r.id[n] : 0;
 }
 
+// The same record with the packed cell left undecoded, so that a load can be
+// issued one iteration ahead of its use (the registers hold raw values).
+template <int D>
+__device__ inline void load_raw(const RecPtr& r, int64_t n, Rec& q) {
+  q.N[0] = r.cell[n];
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    q.s[d] = r.s[d][n];
+    q.p[d] = r.p[d][n];
+  }
+  q.w = r.w[n];
+  q.id = r.id ? r.id[n] : 0;
+}
+
+template <int D>
+__device__ inline void decode_cell(Rec& q, int bx, int by) {
+  const int cell = q.N[0];
+  q.N[0] = cell & ((1 << bx) - 1);
+  q.N[1] = (cell >> bx) & ((1 << by) - 1);
+  q.N[2] = D == 3 ? (cell >> (bx + by)) : 1;
+}
+
 template <int D>
 __device__ inline void store_rec(const RecPtr& r, int64_t n, const Rec& q, int bx, int by) {
   r.cell[n] = q.N[0] | (q.N[1] << bx) | (D == 3 ? (q.N[2] << (bx + by)) : 0);
@@ -68,10 +90,34 @@ __device__ inline void copy_rec(const RecPtr& a, int64_t i, const RecPtr& b, int
   if (a.id && b.id) b.id[o] = a.id[i];
 }
 
+// Separable contraction sum_{i,j,k} wx[i] wy[j] wz[k] F(c, i, j, k):
+// x first, then y, then z (NX*NY*NZ + NY*NZ + NZ multiply-adds instead of
+// forming every 3-D weight product).
+template <int NX, int NY, int NZ, class FA>
+__device__ inline float contract(const FA& F, int c, const float* wx, const float* wy,
+                                 const float* wz) {
+  float r = 0.f;
+#pragma unroll
+  for (int k = 0; k < NZ; k++) {
+    float u = 0.f;
+#pragma unroll
+    for (int j = 0; j < NY; j++) {
+      float t = wx[0] * F(c, 0, j, k);
+#pragma unroll
+      for (int i = 1; i < NX; i++) t = fmaf(wx[i], F(c, i, j, k), t);
+      u = j == 0 ? wy[0] * t : fmaf(wy[j], t, u);
+    }
+    r = k == 0 ? (NZ == 1 ? u : wz[0] * u) : fmaf(wz[k], u, r);
+  }
+  return r;
+}
+
 // Fields at the particle with the reference's alternating-order shapes
 // (set_part2d_hcell_acc / set_part3d_hcell_acc, grid_part_connect.f90:89-285):
-// quadratic weights on integer-node axes, linear on half-node axes.
-// F(c, i1, j1, k1) returns component c at node (N-1+i1, N-1+j1, N-1+k1).
+// quadratic weights on integer-node axes, linear on half-node axes
+// (Ex: half x; Ey: half y; Ez: half z; Bx: half y,z; By: half x,z;
+// Bz: half x,y).  F(c, i1, j1, k1) returns component c at node
+// (N-1+i1, N-1+j1, N-1+k1).
 template <int D, class FA>
 __device__ inline void gather(const float s[3], const FA& F, float ef[6]) {
   float a[3][3], ah[3][2];
@@ -81,59 +127,17 @@ __device__ inline void gather(const float s[3], const FA& F, float ef[6]) {
     lweights<float>(s[d], ah[d]);
   }
   if constexpr (D == 2) {
-    float e0 = 0.f, e1 = 0.f, e2 = 0.f;
-#pragma unroll
-    for (int j1 = 0; j1 <= 2; j1++)
-#pragma unroll
-      for (int i1 = 0; i1 <= 1; i1++) e0 += (ah[0][i1] * a[1][j1]) * F(0, i1, j1, 0);
-#pragma unroll
-    for (int j1 = 0; j1 <= 1; j1++) {
-#pragma unroll
-      for (int i1 = 0; i1 <= 2; i1++) e1 += (a[0][i1] * ah[1][j1]) * F(1, i1, j1, 0);
-#pragma unroll
-      for (int i1 = 0; i1 <= 1; i1++) e2 += (ah[0][i1] * ah[1][j1]) * F(2, i1, j1, 0);
-    }
-    ef[0] = e0;
-    ef[1] = e1;
-    ef[2] = e2;
+    const float one = 1.f;
+    ef[0] = contract<2, 3, 1>(F, 0, ah[0], a[1], &one);
+    ef[1] = contract<3, 2, 1>(F, 1, a[0], ah[1], &one);
+    ef[2] = contract<2, 2, 1>(F, 2, ah[0], ah[1], &one);
   } else {
-    float ap[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int k1 = 0; k1 <= 2; k1++) {
-#pragma unroll
-      for (int j1 = 0; j1 <= 2; j1++) {
-        const float dv = a[1][j1] * a[2][k1];
-#pragma unroll
-        for (int i1 = 0; i1 <= 1; i1++) ap[0] += ah[0][i1] * dv * F(0, i1, j1, k1);
-      }
-#pragma unroll
-      for (int j1 = 0; j1 <= 1; j1++) {
-        const float dv = ah[1][j1] * a[2][k1];
-#pragma unroll
-        for (int i1 = 0; i1 <= 2; i1++) ap[1] += a[0][i1] * dv * F(1, i1, j1, k1);
-#pragma unroll
-        for (int i1 = 0; i1 <= 1; i1++) ap[5] += ah[0][i1] * dv * F(5, i1, j1, k1);
-      }
-    }
-#pragma unroll
-    for (int k1 = 0; k1 <= 1; k1++) {
-#pragma unroll
-      for (int j1 = 0; j1 <= 1; j1++) {
-        const float dv = ah[1][j1] * ah[2][k1];
-#pragma unroll
-        for (int i1 = 0; i1 <= 2; i1++) ap[3] += a[0][i1] * dv * F(3, i1, j1, k1);
-      }
-#pragma unroll
-      for (int j1 = 0; j1 <= 2; j1++) {
-        const float dv = a[1][j1] * ah[2][k1];
-#pragma unroll
-        for (int i1 = 0; i1 <= 1; i1++) ap[4] += ah[0][i1] * dv * F(4, i1, j1, k1);
-#pragma unroll
-        for (int i1 = 0; i1 <= 2; i1++) ap[2] += a[0][i1] * dv * F(2, i1, j1, k1);
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 6; c++) ef[c] = ap[c];
+    ef[0] = contract<2, 3, 3>(F, 0, ah[0], a[1], a[2]);
+    ef[1] = contract<3, 2, 3>(F, 1, a[0], ah[1], a[2]);
+    ef[2] = contract<3, 3, 2>(F, 2, a[0], a[1], ah[2]);
+    ef[3] = contract<3, 2, 2>(F, 3, a[0], ah[1], ah[2]);
+    ef[4] = contract<2, 3, 2>(F, 4, ah[0], a[1], ah[2]);
+    ef[5] = contract<2, 2, 3>(F, 5, ah[0], ah[1], a[2]);
   }
 }
 
@@ -250,6 +254,13 @@ __device__ inline int reserve_one(int* counter) {
   if (lane == leader) b = atomicAdd(counter, __popcll(act));
   b = __shfl(b, leader);
   return b + __popcll(act & ((1ull << lane) - 1ull));
+}
+
+// Maximum of v over the wavefront.
+__device__ inline int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
 }
 
 __device__ inline int tile_of(const TileGeom& tg, int cell, int D) {

@@ -1,22 +1,25 @@
 // tile_kernel.hip -- the FP32 particle kernel of the tile path.
 //
-// One workgroup per tile of TX x TY x TZ node-cells and one thread per cell
-// (the tile shape is a template parameter, so every LDS address below is a
-// per-thread base plus an immediate offset).  Per workgroup:
+// One workgroup of 1024 threads per tile of TX x TY x TZ node-cells
+// (the tile shape is a template parameter, so LDS offsets inside a stencil
+// are immediates).  Per workgroup:
 //   1. stage the field patch (nodes T0-1 .. T0+T per axis, all components)
-//      from HBM into LDS and zero an LDS current patch (nodes T0-2 .. T0+T+1,
-//      the reach of the 5-slot Esirkepov window of grid_part_connect.f90:1008);
+//      from HBM into LDS and zero an FP64 LDS current patch (nodes
+//      T0-2 .. T0+T+1, the reach of the 5-slot Esirkepov window of
+//      grid_part_connect.f90:1008);
 //   2. counting-sort the tile's particles by cell in LDS (chunks of kChunk);
-//   3. each thread walks the particles of its own cell: gather from LDS,
-//      charge multiply, [back-shift], exact-gamma push, carry, box test;
-//      a particle that keeps its nearest node on every axis adds its
-//      Esirkepov current to registers (54 values in 3-D, 12 in 2-D), one that
-//      changes node goes through the union-window deposit into LDS;
-//      survivors are written to the other buffer: stayers into slots
-//      reserved for the chunk with one global atomic, movers into their new
-//      tile with wave-aggregated atomics, overflow into the global list;
-//   4. flush the register currents to LDS and the LDS patch to HBM.
-// No global float atomics are used.
+//   3. split the sorted chunk into equal consecutive slices, one per thread,
+//      and walk them in lock step: gather from LDS, charge multiply,
+//      [back-shift], exact-gamma push, carry, box test; a particle that keeps
+//      its nearest node deposits its 2x3x3-per-component Esirkepov current
+//      with LDS FP64 atomics, one that changes node is queued per wave and
+//      deposited through the union window when 64 are waiting; survivors are
+//      written to the other buffer: stayers into slots reserved per chunk,
+//      movers into their new tile with wave-aggregated atomics, overflow into
+//      the global list;
+//   4. write the LDS current patch to HBM (FP32) for the fold.
+// LDS accumulation is FP64 because LDS FP32 atomic adds are far slower than
+// FP64 ones on gfx950; there are no global float atomics.
 #include "tile_device.hpp"
 
 namespace aladyn {
@@ -34,7 +37,7 @@ struct TileArgs {
 // W0 - W1 (the fourth vanishes by the partition of unity).  jb points at the
 // LDS patch node of the union base; offsets are compile-time.
 template <int D, int JX, int JY, int PVOL>
-__device__ inline void deposit_general(float* jb, const float s0[3], const float s1[3],
+__device__ inline void deposit_general(double* jb, const float s0[3], const float s1[3],
                                        const int o0[3], const int o1[3], float wq) {
   float W0[3][4], W1[3][4];
 #pragma unroll
@@ -63,8 +66,8 @@ __device__ inline void deposit_general(float* jb, const float s0[3], const float
     for (int f = 0; f < 3; f++)
 #pragma unroll
       for (int m = 0; m < 4; m++) {
-        atomicAdd(jb + (f + JX * m), F[0][f] * (W0[1][m] + W1[1][m]));
-        atomicAdd(jb + PVOL + (m + JX * f), F[1][f] * (W0[0][m] + W1[0][m]));
+        atomicAdd(jb + (f + JX * m), (double)(F[0][f] * (W0[1][m] + W1[1][m])));
+        atomicAdd(jb + PVOL + (m + JX * f), (double)(F[1][f] * (W0[0][m] + W1[0][m])));
       }
   } else {
     // J_x: flux_x (x) [W0z (W0y + W1y/2) + W1z (W1y + W0y/2)], etc.
@@ -79,37 +82,141 @@ __device__ inline void deposit_general(float* jb, const float s0[3], const float
         const float txy = x0 * W0[1][l] + x1 * W1[1][l];
 #pragma unroll
         for (int f = 0; f < 3; f++) {
-          atomicAdd(jb + (f + JX * (m + JY * l)), F[0][f] * tyz);
-          atomicAdd(jb + PVOL + (m + JX * (f + JY * l)), F[1][f] * txz);
-          atomicAdd(jb + 2 * PVOL + (m + JX * (l + JY * f)), F[2][f] * txy);
+          atomicAdd(jb + (f + JX * (m + JY * l)), (double)(F[0][f] * tyz));
+          atomicAdd(jb + PVOL + (m + JX * (f + JY * l)), (double)(F[1][f] * txz));
+          atomicAdd(jb + 2 * PVOL + (m + JX * (l + JY * f)), (double)(F[2][f] * txy));
         }
       }
     }
   }
 }
 
-template <int D, int TX, int TY, int TZ>
-__global__ __launch_bounds__(TX* TY* TZ) void k_tile(
+// Esirkepov deposit (grid_part_connect.f90:979-1060, 1168-1334) of a
+// particle whose nearest node N is the same before and after the move: per
+// axis the 3-node windows coincide, W0[0]-W1[0] = (s1-s0)(1-s0-s1)/2 and
+// W1[2]-W0[2] = (s1-s0)(1+s0+s1)/2 are the only nonzero fluxes, so the
+// current has 2x3x3 values per component (2x3 in 2-D).  j1 points at the LDS
+// patch node N-1; offsets are compile-time.
+template <int D, int JX, int JY, int PVOL>
+__device__ inline void deposit_same_node(double* j1, const float s0[3], const float s1[3],
+                                         float wq) {
+  float a0[3][3], a1[3][3], fl[3][2];
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    qweights<float>(s0[d], a0[d]);
+    qweights<float>(s1[d], a1[d]);
+    const float h = 0.5f * wq * (s1[d] - s0[d]);
+    fl[d][0] = h * ((1.f - s0[d]) - s1[d]);
+    fl[d][1] = h * ((1.f + s0[d]) + s1[d]);
+  }
+  if constexpr (D == 2) {
+#pragma unroll
+    for (int mm = 0; mm < 3; mm++) {
+      const float ty2 = a0[1][mm] + a1[1][mm];
+      const float tx2 = a0[0][mm] + a1[0][mm];
+#pragma unroll
+      for (int f = 0; f < 2; f++) {
+        atomicAdd(j1 + (f + JX * mm), (double)(fl[0][f] * ty2));
+        atomicAdd(j1 + PVOL + (mm + JX * f), (double)(fl[1][f] * tx2));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int mm = 0; mm < 3; mm++) {
+      const float y0 = a0[1][mm] + 0.5f * a1[1][mm], y1 = a1[1][mm] + 0.5f * a0[1][mm];
+      const float x0 = a0[0][mm] + 0.5f * a1[0][mm], x1 = a1[0][mm] + 0.5f * a0[0][mm];
+#pragma unroll
+      for (int l = 0; l < 3; l++) {
+        const float tyz = y0 * a0[2][l] + y1 * a1[2][l];
+        const float txz = x0 * a0[2][l] + x1 * a1[2][l];
+        const float txy = x0 * a0[1][l] + x1 * a1[1][l];
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+          atomicAdd(j1 + (f + JX * (mm + JY * l)), (double)(fl[0][f] * tyz));
+          atomicAdd(j1 + PVOL + (mm + JX * (f + JY * l)), (double)(fl[1][f] * txz));
+          atomicAdd(j1 + 2 * PVOL + (mm + JX * (l + JY * f)), (double)(fl[2][f] * txy));
+        }
+      }
+    }
+  }
+}
+
+// LDS layout of the tile kernel (all sizes compile-time):
+//   Fl  float  [NF][GVOL]   field patch, nodes T0-1 .. T0+T per axis
+//   Jl  double [D][PVOL]    current patch, nodes T0-2 .. T0+T+1 per axis
+//   Q   int    [NW][8][QCAP] per-wave queue of node-changing particles
+//   offs, fillc int [NCL+1], [NCL]; misc int [2]; perm u16 [kChunk]
+template <int D, int TX, int TY, int TZ, int NT>
+struct TileShape {
+  static constexpr int NCL = TX * TY * TZ, NF = D == 3 ? 6 : 3, NW = NT / 64;
+  static constexpr int GX = TX + 2, GY = TY + 2, GZ = D == 3 ? TZ + 2 : 1, GVOL = GX * GY * GZ;
+  static constexpr int JX = TX + 4, JY = TY + 4, JZ = D == 3 ? TZ + 4 : 1, PVOL = JX * JY * JZ;
+  static constexpr int QCAP = 128, QW = 8;
+  static constexpr int F_WORDS = (NF * GVOL + 1) & ~1;
+  static constexpr int J_WORDS = 2 * D * PVOL;
+  static constexpr int Q_WORDS = NW * QW * QCAP;
+  static constexpr int I_WORDS = 2 * NCL + 3;
+  static constexpr int BYTES = 4 * (F_WORDS + J_WORDS + Q_WORDS + I_WORDS) + 2 * kChunk;
+  static_assert(NT % 64 == 0 && NT >= NCL, "threads: whole wavefronts, at least one per cell");
+};
+
+// Drains n <= 64 queued node-changing particles, one per lane, through the
+// union-window deposit.  Queue entry words: s0[3], s1[3], w*q, meta (local
+// cell | (sign+1) << (10 + 2d) per axis).
+template <int D, int TX, int TY, int TZ, int NT>
+__device__ inline void drain_queue(const int* qw, int head, int n, double* Jl) {
+  using S = TileShape<D, TX, TY, TZ, NT>;
+  const int lane = (int)__lane_id();
+  if (lane < n) {
+    const int e = (head + lane) & (S::QCAP - 1);
+    float s0[3] = {0.f, 0.f, 0.f}, s1[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      s0[d] = __int_as_float(qw[d * S::QCAP + e]);
+      s1[d] = __int_as_float(qw[(3 + d) * S::QCAP + e]);
+    }
+    const float wq = __int_as_float(qw[6 * S::QCAP + e]);
+    const int meta = qw[7 * S::QCAP + e];
+    const int lc = meta & 1023;
+    int o0[3] = {0, 0, 0}, o1[3] = {0, 0, 0}, sh[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int sig = ((meta >> (10 + 2 * d)) & 3) - 1;
+      o0[d] = sig < 0 ? 1 : 0;
+      o1[d] = sig > 0 ? 1 : 0;
+      sh[d] = sig < 0 ? -1 : 0;
+    }
+    const int cx = lc % TX, cy = (lc / TX) % TY, cz = D == 3 ? lc / (TX * TY) : 0;
+    // union base N-1+min(0,sig) sits at patch offset 1+min(0,sig) from node N-2
+    double* ub = Jl + (cx + 1 + sh[0]) +
+                 S::JX * ((cy + 1 + sh[1]) + S::JY * (D == 3 ? cz + 1 + sh[2] : 0));
+    deposit_general<D, S::JX, S::JY, S::PVOL>(ub, s0, s1, o0, o1, wq);
+  }
+}
+
+template <int D, int TX, int TY, int TZ, int NT>
+__global__ __launch_bounds__(NT) void k_tile(
     TileArgs ta, StepConst sc, const float* __restrict__ ebf, RecPtr A, RecPtr B,
     const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
     const int* __restrict__ cnt_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_cur,
     int* __restrict__ ovf_next, float* __restrict__ patch) {
-  constexpr int NCL = TX * TY * TZ, NF = D == 3 ? 6 : 3;
-  constexpr int GX = TX + 2, GY = TY + 2, GZ = D == 3 ? TZ + 2 : 1, GVOL = GX * GY * GZ;
-  constexpr int JX = TX + 4, JY = TY + 4, JZ = D == 3 ? TZ + 4 : 1, PVOL = JX * JY * JZ;
+  using S = TileShape<D, TX, TY, TZ, NT>;
+  constexpr int NCL = S::NCL, NF = S::NF, GX = S::GX, GY = S::GY, GVOL = S::GVOL;
+  constexpr int JX = S::JX, JY = S::JY, PVOL = S::PVOL, QCAP = S::QCAP;
   extern __shared__ float lds[];
   float* Fl = lds;
-  float* Jl = Fl + NF * GVOL;
-  unsigned short* perm = (unsigned short*)(Jl + D * PVOL);
-  int* offs = (int*)(perm + kChunk);  // NCL + 1
-  int* fillc = offs + NCL + 1;        // NCL
-  int* misc = fillc + NCL;            // [0] chunk slot base, [1] stayers
+  double* Jl = (double*)(lds + S::F_WORDS);
+  int* Q = (int*)(lds + S::F_WORDS + S::J_WORDS);
+  int* offs = Q + S::Q_WORDS;  // NCL + 1
+  int* fillc = offs + NCL + 1;  // NCL
+  int* misc = fillc + NCL;      // [0] chunk slot base, [1] stayers
+  unsigned short* perm = (unsigned short*)(misc + 2);
 
   const int t = blockIdx.x;
   const int tx = t % ta.nt0, ty = (t / ta.nt0) % ta.nt1, tz = t / (ta.nt0 * ta.nt1);
   const int T0x = 3 + tx * TX, T0y = 3 + ty * TY, T0z = D == 3 ? 3 + tz * TZ : 1;
   // field patch: nodes T0-1 .. T0+T
-  for (int q = threadIdx.x; q < NF * GVOL; q += NCL) {
+  for (int q = threadIdx.x; q < NF * GVOL; q += NT) {
     const int c = q / GVOL, r = q - c * GVOL;
     const int li = r % GX, lj = (r / GX) % GY, lk = r / (GX * GY);
     const int i = T0x - 1 + li, j = T0y - 1 + lj, k = D == 3 ? T0z - 1 + lk : 1;
@@ -118,25 +225,14 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_tile(
       v = ebf[(int64_t)c * ta.ncell + (i - 1) + (int64_t)ta.n1p * ((j - 1) + (int64_t)ta.n2p * (k - 1))];
     Fl[q] = v;
   }
-  for (int q = threadIdx.x; q < D * PVOL; q += NCL) Jl[q] = 0.f;
+  for (int q = threadIdx.x; q < D * PVOL; q += NT) Jl[q] = 0.0;
 
   const int me = threadIdx.x;
-  const int cx = me % TX, cy = (me / TX) % TY, cz = D == 3 ? me / (TX * TY) : 0;
-  const int Nc[3] = {T0x + cx, T0y + cy, D == 3 ? T0z + cz : 1};
-  // LDS field patch base at node N-1 of this cell, current patch base at N-2
-  const float* fb = Fl + cx + GX * (cy + GY * cz);
-  float* jb2 = Jl + cx + JX * (cy + JY * cz);
-  const auto F = [&](int c, int i1, int j1, int k1) -> float {
-    return fb[c * GVOL + i1 + GX * (j1 + GY * k1)];
-  };
-  // register currents of non-crossing particles, node offsets from N-1
-  float jx[2][3][3], jy[3][2][3], jz[3][3][2];
-#pragma unroll
-  for (int a = 0; a < 18; a++) {
-    (&jx[0][0][0])[a] = 0.f;
-    (&jy[0][0][0])[a] = 0.f;
-    (&jz[0][0][0])[a] = 0.f;
-  }
+  const int lane = (int)__lane_id();
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  int* qw = Q + (me >> 6) * (S::QW * QCAP);
+  int qhead = 0, qcount = 0;  // wave-uniform ring state
+
   const int cap_t = seg_cap[t];
   const int cnt = min(cnt_cur[t], cap_t);
   const int64_t base = seg_start[t];
@@ -145,9 +241,9 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_tile(
   for (int c0 = 0; c0 < cnt; c0 += kChunk) {
     const int m = min(kChunk, cnt - c0);
     __syncthreads();  // patches staged / previous chunk consumed
-    offs[me] = 0;
+    if (me < NCL) offs[me] = 0;
     __syncthreads();
-    for (int q = me; q < m; q += NCL) {
+    for (int q = me; q < m; q += NT) {
       const int cell = A.cell[base + c0 + q];
       if (cell == kHole) continue;
       const int li = (cell & mi) - T0x, lj = ((cell >> ta.bx) & mj) - T0y;
@@ -157,17 +253,17 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_tile(
     }
     __syncthreads();
     {  // exclusive scan of offs[0..NCL) (Hillis-Steele through fillc)
-      const int v = offs[me];
-      fillc[me] = v;
+      const int v = me < NCL ? offs[me] : 0;
+      if (me < NCL) fillc[me] = v;
       __syncthreads();
 #pragma unroll 1
       for (int off = 1; off < NCL; off <<= 1) {
-        const int add = me >= off ? fillc[me - off] : 0;
+        const int add = (me < NCL && me >= off) ? fillc[me - off] : 0;
         __syncthreads();
-        fillc[me] += add;
+        if (me < NCL) fillc[me] += add;
         __syncthreads();
       }
-      offs[me] = fillc[me] - v;
+      if (me < NCL) offs[me] = fillc[me] - v;
       if (me == NCL - 1) {
         offs[NCL] = fillc[me];
         // slots for every valid particle of the chunk in this tile
@@ -175,10 +271,10 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_tile(
         misc[1] = 0;
       }
       __syncthreads();
-      fillc[me] = 0;
+      if (me < NCL) fillc[me] = 0;
       __syncthreads();
     }
-    for (int q = me; q < m; q += NCL) {
+    for (int q = me; q < m; q += NT) {
       const int cell = A.cell[base + c0 + q];
       if (cell == kHole) continue;
       const int li = (cell & mi) - T0x, lj = ((cell >> ta.bx) & mj) - T0y;
@@ -194,130 +290,120 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_tile(
     }
     __syncthreads();
     const int chunk_base = misc[0];
-    const int p_end = offs[me + 1];
-    for (int pi = offs[me]; pi < p_end; pi++) {
-      Rec q;
-      load_rec<D>(A, base + c0 + perm[pi], q, ta.bx, ta.by);
-      float ef[6];
-      gather<D>(q.s, F, ef);
-      const float so[3] = {q.s[0], q.s[1], q.s[2]};
-      const float dtg = push<D>(q.p, ef, sc.qch, sc.alp, sc.dt, sc.initial);
-      if (!move_and_test<D>(q, dtg, sc.inv, sc.nn)) continue;
-      const float wq = sc.qch * q.w;
-      bool cross = q.N[0] != Nc[0] || q.N[1] != Nc[1];
-      if (D == 3) cross = cross || q.N[2] != Nc[2];
-      if (!cross) {
-        float a0[3][3], a1[3][3], fl[3][2];
+    const int nval = offs[NCL];
+    // balanced split of the cell-sorted chunk: thread me takes the W
+    // consecutive sorted positions [me*W, me*W+W), so every thread has the
+    // same trip count whatever the density, and at each step the lanes of a
+    // wave sit W positions apart, i.e. on different cells (no two lanes of an
+    // LDS atomic hit the same address unless one cell holds more than W).
+    const int W = (nval + NT - 1) / NT;
+    const int q0 = me * W, q1 = min(q0 + W, nval);
+    const int64_t cb = base + c0;
+    Rec nxt = {};
+    if (q0 < q1) load_raw<D>(A, cb + perm[q0], nxt);
+    for (int r = 0; r < W; r++) {
+      const int qi = q0 + r;
+      Rec q = nxt;
+      if (qi + 1 < q1) load_raw<D>(A, cb + perm[qi + 1], nxt);
+      bool keep = false, cross = false;
+      float so[3] = {0.f, 0.f, 0.f}, wq = 0.f;
+      int lc = 0, N0[3] = {0, 0, 0};
+      if (qi < q1) {
+        decode_cell<D>(q, ta.bx, ta.by);
+        const int li = q.N[0] - T0x, lj = q.N[1] - T0y, lk = D == 3 ? q.N[2] - T0z : 0;
+        lc = li + TX * (lj + TY * lk);
+        // LDS field patch base at node N-1 of the particle's cell
+        const float* fb = Fl + li + GX * (lj + GY * lk);
+        const auto F = [&](int c, int i1, int j1, int k1) -> float {
+          return fb[c * GVOL + i1 + GX * (j1 + GY * k1)];
+        };
+        float ef[6];
+        gather<D>(q.s, F, ef);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          so[d] = q.s[d];
+          N0[d] = q.N[d];
+        }
+        const float dtg = push<D>(q.p, ef, sc.qch, sc.alp, sc.dt, sc.initial);
+        keep = move_and_test<D>(q, dtg, sc.inv, sc.nn);
+        wq = sc.qch * q.w;
+        cross = q.N[0] != N0[0] || q.N[1] != N0[1];
+        if (D == 3) cross = cross || q.N[2] != N0[2];
+        if (keep && !cross) {
+          // current patch starts at node T0-2: node N-1 is at local offset +1
+          double* j1 = Jl + (li + 1) + JX * ((lj + 1) + JY * (D == 3 ? lk + 1 : 0));
+          deposit_same_node<D, JX, JY, PVOL>(j1, so, q.s, wq);
+        }
+      }
+      // node-changing particles go to the wave's queue
+      const unsigned long long cm = __ballot(keep && cross);
+      if (keep && cross) {
+        const int e = (qhead + qcount + __popcll(cm & lt_mask)) & (QCAP - 1);
+        int meta = lc;
 #pragma unroll
         for (int d = 0; d < D; d++) {
-          qweights<float>(so[d], a0[d]);
-          qweights<float>(q.s[d], a1[d]);
-          // W0[0]-W1[0] = (s1-s0)(1-s0-s1)/2, -(W0[2]-W1[2]) = (s1-s0)(1+s0+s1)/2
-          const float h = 0.5f * wq * (q.s[d] - so[d]);
-          fl[d][0] = h * ((1.f - so[d]) - q.s[d]);
-          fl[d][1] = h * ((1.f + so[d]) + q.s[d]);
+          qw[d * QCAP + e] = __float_as_int(so[d]);
+          qw[(3 + d) * QCAP + e] = __float_as_int(q.s[d]);
+          meta |= (q.N[d] - N0[d] + 1) << (10 + 2 * d);
         }
-        if constexpr (D == 2) {
-#pragma unroll
-          for (int mm = 0; mm < 3; mm++) {
-            const float ty2 = a0[1][mm] + a1[1][mm];
-            const float tx2 = a0[0][mm] + a1[0][mm];
-#pragma unroll
-            for (int f = 0; f < 2; f++) {
-              jx[f][mm][0] += fl[0][f] * ty2;
-              jy[mm][f][0] += fl[1][f] * tx2;
-            }
-          }
+        qw[6 * QCAP + e] = __float_as_int(wq);
+        qw[7 * QCAP + e] = meta;
+      }
+      qcount += __popcll(cm);
+      if (keep) {
+        // write the survivor into the other buffer
+        const int dtile = (q.N[0] - 3) / TX +
+                          ta.nt0 * ((q.N[1] - 3) / TY + ta.nt1 * (D == 3 ? (q.N[2] - 3) / TZ : 0));
+        int64_t dst;
+        if (dtile == t) {
+          const int slot = chunk_base + reserve_one(&misc[1]);
+          dst = slot < cap_t ? base + slot : sc.ovf_base + atomicAdd(ovf_next, 1);
         } else {
-#pragma unroll
-          for (int mm = 0; mm < 3; mm++) {
-            const float y0 = a0[1][mm] + 0.5f * a1[1][mm], y1 = a1[1][mm] + 0.5f * a0[1][mm];
-            const float x0 = a0[0][mm] + 0.5f * a1[0][mm], x1 = a1[0][mm] + 0.5f * a0[0][mm];
-#pragma unroll
-            for (int l = 0; l < 3; l++) {
-              const float tyz = y0 * a0[2][l] + y1 * a1[2][l];
-              const float txz = x0 * a0[2][l] + x1 * a1[2][l];
-              const float txy = x0 * a0[1][l] + x1 * a1[1][l];
-#pragma unroll
-              for (int f = 0; f < 2; f++) {
-                jx[f][mm][l] += fl[0][f] * tyz;
-                jy[mm][f][l] += fl[1][f] * txz;
-                jz[mm][l][f] += fl[2][f] * txy;
-              }
-            }
-          }
+          const int slot = reserve_slot(dtile, cnt_next);
+          dst = slot < seg_cap[dtile] ? (int64_t)seg_start[dtile] + slot
+                                      : sc.ovf_base + atomicAdd(ovf_next, 1);
         }
-      } else {
-        int o0[3] = {0, 0, 0}, o1[3] = {0, 0, 0}, sh[3] = {0, 0, 0};
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-          const int sig = q.N[d] - Nc[d];  // -1, 0, +1
-          o0[d] = sig < 0 ? 1 : 0;
-          o1[d] = sig > 0 ? 1 : 0;
-          sh[d] = sig < 0 ? -1 : 0;  // union base = N-1 + min(0, sig) -> patch N-2 offset
-        }
-        float* ub = jb2 + (1 + sh[0]) + JX * ((1 + sh[1]) + JY * (D == 3 ? 1 + sh[2] : 0));
-        deposit_general<D, JX, JY, PVOL>(ub, so, q.s, o0, o1, wq);
+        store_rec<D>(B, dst, q, ta.bx, ta.by);
       }
-      // write the survivor into the other buffer
-      const int dtile = (q.N[0] - 3) / TX +
-                        ta.nt0 * ((q.N[1] - 3) / TY + ta.nt1 * (D == 3 ? (q.N[2] - 3) / TZ : 0));
-      int64_t dst;
-      if (dtile == t) {
-        const int slot = chunk_base + reserve_one(&misc[1]);
-        dst = slot < cap_t ? base + slot : sc.ovf_base + atomicAdd(ovf_next, 1);
-      } else {
-        const int slot = reserve_slot(dtile, cnt_next);
-        dst = slot < seg_cap[dtile] ? (int64_t)seg_start[dtile] + slot
-                                    : sc.ovf_base + atomicAdd(ovf_next, 1);
+      if (qcount >= 64) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        drain_queue<D, TX, TY, TZ, NT>(qw, qhead, 64, Jl);
+        qhead = (qhead + 64) & (QCAP - 1);
+        qcount -= 64;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
       }
-      store_rec<D>(B, dst, q, ta.bx, ta.by);
     }
     __syncthreads();
     // reserved slots not taken by stayers become holes
     {
       const int nstay = misc[1], nval = offs[NCL];
-      for (int q = nstay + me; q < nval; q += NCL) {
+      for (int q = nstay + me; q < nval; q += NT) {
         const int slot = chunk_base + q;
         if (slot < cap_t) B.cell[base + slot] = kHole;
       }
     }
   }
-  // flush the register currents (node N-1 is patch offset +1 from jb2)
-  {
-    float* j1 = jb2 + 1 + JX * (1 + JY * (D == 3 ? 1 : 0));
-    if constexpr (D == 2) {
-#pragma unroll
-      for (int f = 0; f < 2; f++)
-#pragma unroll
-        for (int mm = 0; mm < 3; mm++) {
-          atomicAdd(j1 + (f + JX * mm), jx[f][mm][0]);
-          atomicAdd(j1 + PVOL + (mm + JX * f), jy[mm][f][0]);
-        }
-    } else {
-#pragma unroll
-      for (int mm = 0; mm < 3; mm++)
-#pragma unroll
-        for (int l = 0; l < 3; l++)
-#pragma unroll
-          for (int f = 0; f < 2; f++) {
-            atomicAdd(j1 + (f + JX * (mm + JY * l)), jx[f][mm][l]);
-            atomicAdd(j1 + PVOL + (mm + JX * (f + JY * l)), jy[mm][f][l]);
-            atomicAdd(j1 + 2 * PVOL + (mm + JX * (l + JY * f)), jz[mm][l][f]);
-          }
-    }
+  if (qcount > 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    drain_queue<D, TX, TY, TZ, NT>(qw, qhead, qcount, Jl);
   }
   __syncthreads();
   float* P = patch + (int64_t)t * D * PVOL;
-  for (int q = threadIdx.x; q < D * PVOL; q += NCL) P[q] = Jl[q];
+  for (int q = threadIdx.x; q < D * PVOL; q += NT) P[q] = (float)Jl[q];
 }
+
+// threads per workgroup: 1024 (16 wavefronts) so that one tile per CU keeps
+// four wavefronts on every SIMD while the LDS patches fit
+static constexpr int kTileThreads = 1024;
 
 template <int D, int TX, int TY, int TZ>
 static int lds_bytes_for() {
-  constexpr int NCL = TX * TY * TZ, NF = D == 3 ? 6 : 3;
-  constexpr int GVOL = (TX + 2) * (TY + 2) * (D == 3 ? TZ + 2 : 1);
-  constexpr int PVOL = (TX + 4) * (TY + 4) * (D == 3 ? TZ + 4 : 1);
-  return (NF * GVOL + D * PVOL) * 4 + kChunk * 2 + (2 * NCL + 3) * 4;
+  return TileShape<D, TX, TY, TZ, kTileThreads>::BYTES;
 }
 
 template <int D, int TX, int TY, int TZ>
@@ -325,8 +411,9 @@ static hipError_t launch(const TileGeom& tg, const TileArgs& ta, const StepConst
                          const float* ebf, const RecPtr& A, const RecPtr& B, const int* ss,
                          const int* sc_, const int* cc, int* cn, int* oc, int* on, float* patch,
                          hipStream_t s) {
-  k_tile<D, TX, TY, TZ><<<tg.ntiles, TX * TY * TZ, lds_bytes_for<D, TX, TY, TZ>(), s>>>(
-      ta, sc, ebf, A, B, ss, sc_, cc, cn, oc, on, patch);
+  k_tile<D, TX, TY, TZ, kTileThreads>
+      <<<tg.ntiles, kTileThreads, lds_bytes_for<D, TX, TY, TZ>(), s>>>(ta, sc, ebf, A, B, ss, sc_,
+                                                                      cc, cn, oc, on, patch);
   return hipGetLastError();
 }
 
@@ -346,6 +433,8 @@ int tile_lds_bytes(int D, int tx, int ty, int tz) {
   if (tx == 16) return lds_bytes_for<2, 16, 16, 1>();
   return lds_bytes_for<2, 32, 16, 1>();
 }
+
+int tile_threads() { return kTileThreads; }
 
 hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepConst& sc,
                               const float* ebf, const RecPtr& A, const RecPtr& B,

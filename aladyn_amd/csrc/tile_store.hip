@@ -14,6 +14,7 @@ namespace aladyn {
 // tile_kernel.hip
 bool tile_shape_supported(int D, int tx, int ty, int tz);
 int tile_lds_bytes(int D, int tx, int ty, int tz);
+int tile_threads();
 hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepConst& sc,
                               const float* ebf, const RecPtr& A, const RecPtr& B,
                               const int* seg_start, const int* seg_cap, const int* cnt_cur,
@@ -57,7 +58,7 @@ int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg) {
   tg->by = bits_for(g.n2p);
   const int bz = D == 3 ? bits_for(g.n3p) : 0;
   if (tg->bx + tg->by + bz > 31) return 2;
-  tg->block = tg->T[0] * tg->T[1] * tg->T[2];  // one thread per node-cell
+  tg->block = tile_threads();
   tg->sort_every = cfg.sort_every > 0 ? cfg.sort_every : 16;
   tg->lds_bytes = tile_lds_bytes(D, tg->T[0], tg->T[1], tg->T[2]);
   if (tg->lds_bytes > 160 * 1024) return 3;
@@ -183,8 +184,13 @@ __device__ inline int block_scan_excl(int v, int* sh, int& total) {
   return ex;
 }
 
+// Segment capacity of a tile holding v particles at a re-layout: half again
+// as many slots (HBM is plentiful; unused slack is never read) so that tiles
+// gaining particles between re-layouts rarely spill into the overflow list.
+__host__ __device__ inline int64_t tile_capacity(int64_t v) { return v + v / 2 + 64; }
+
 // out[t] = exclusive scan of v[t], out[n] = total (one workgroup); with
-// cap_out, v is turned into the capacity v + v/4 + 16 first.
+// cap_out, v is turned into its segment capacity first.
 __global__ __launch_bounds__(kB) void k_scan_tiles(const int* v_in, int n, int* out, int* cap_out) {
   __shared__ int sh[kB];
   long long carry = 0;
@@ -194,7 +200,7 @@ __global__ __launch_bounds__(kB) void k_scan_tiles(const int* v_in, int n, int* 
     if (t < n) {
       v = v_in[t];
       if (cap_out) {
-        v = v + v / 4 + 16;
+        v = (int)tile_capacity(v);
         cap_out[t] = v;
       }
     }
@@ -364,7 +370,7 @@ void tile_store_free(TileStore* ts) {
 }
 
 // Counting sort of the n records of U (stored in buffer 1-cur) into buffer
-// cur with capacities cnt + cnt/4 + 16 per tile.
+// cur with capacities tile_capacity(cnt) per tile.
 template <int D>
 static int sort_into_cur(TileStore* ts, const TileGeom& tg, int64_t n, hipStream_t s,
                          std::string* msg) {
@@ -408,7 +414,7 @@ int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const do
       return ALADYN_EINVAL;
     }
   }
-  const int64_t need = np + np / 4 + 16LL * tg.ntiles + 64;  // segments
+  const int64_t need = tile_capacity(np) + 64LL * tg.ntiles;  // segments (bounds the sum)
   const int64_t alloc = need + (np > 0 ? np : 1);            // + overflow region
   if (alloc >= (1LL << 31)) {
     *msg = "FP32 tile path: particle count exceeds the 31-bit slot range";
@@ -591,7 +597,7 @@ int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const f
   ts->steps_since_sort++;
   // lazily watch the overflow count; re-layout on cadence or on growth
   bool relayout = ts->steps_since_sort >= tg.sort_every;
-  if (hipEventQuery(ts->ovf_ev) == hipSuccess && ts->h_ovf[0] > (int)(ts->ovf_cap / 64 + 1024))
+  if (hipEventQuery(ts->ovf_ev) == hipSuccess && ts->h_ovf[0] > (int)(ts->ovf_cap / 512 + 1024))
     relayout = true;
   TS_CK(hipMemcpyAsync(ts->h_ovf, ts->ovf_count + nxt, sizeof(int), hipMemcpyDeviceToHost, s));
   TS_CK(hipEventRecord(ts->ovf_ev, s));
