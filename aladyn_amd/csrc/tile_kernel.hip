@@ -91,61 +91,98 @@ __device__ inline void deposit_general(double* jb, const float s0[3], const floa
   }
 }
 
-// Esirkepov deposit (grid_part_connect.f90:979-1060, 1168-1334) of a
-// particle whose nearest node N is the same before and after the move: per
-// axis the 3-node windows coincide, W0[0]-W1[0] = (s1-s0)(1-s0-s1)/2 and
-// W1[2]-W0[2] = (s1-s0)(1+s0+s1)/2 are the only nonzero fluxes, so the
-// current has 2x3x3 values per component (2x3 in 2-D).  j1 points at the LDS
-// patch node N-1; offsets are compile-time.
-template <int D, int JX, int JY, int PVOL>
-__device__ inline void deposit_same_node(double* j1, const float s0[3], const float s1[3],
-                                         float wq) {
-  float a0[3][3], a1[3][3], fl[3][2];
+// Esirkepov current (grid_part_connect.f90:979-1060, 1168-1334) of particles
+// whose nearest node N is the same before and after the move, summed in
+// registers for one cell: per axis the 3-node windows coincide,
+// W0[0]-W1[0] = (s1-s0)(1-s0-s1)/2 and W1[2]-W0[2] = (s1-s0)(1+s0+s1)/2 are
+// the only nonzero fluxes, so the current has 2x3x3 values per component
+// (2x3 in 2-D), offsets from node N-1.
+template <int D>
+struct CellAcc {
+  float jx[2][3][3], jy[3][2][3], jz[3][3][2];
+
+  __device__ inline void zero() {
 #pragma unroll
-  for (int d = 0; d < D; d++) {
-    qweights<float>(s0[d], a0[d]);
-    qweights<float>(s1[d], a1[d]);
-    const float h = 0.5f * wq * (s1[d] - s0[d]);
-    fl[d][0] = h * ((1.f - s0[d]) - s1[d]);
-    fl[d][1] = h * ((1.f + s0[d]) + s1[d]);
-  }
-  if constexpr (D == 2) {
-#pragma unroll
-    for (int mm = 0; mm < 3; mm++) {
-      const float ty2 = a0[1][mm] + a1[1][mm];
-      const float tx2 = a0[0][mm] + a1[0][mm];
-#pragma unroll
-      for (int f = 0; f < 2; f++) {
-        atomicAdd(j1 + (f + JX * mm), (double)(fl[0][f] * ty2));
-        atomicAdd(j1 + PVOL + (mm + JX * f), (double)(fl[1][f] * tx2));
-      }
+    for (int a = 0; a < 18; a++) {
+      (&jx[0][0][0])[a] = 0.f;
+      (&jy[0][0][0])[a] = 0.f;
+      (&jz[0][0][0])[a] = 0.f;
     }
-  } else {
+  }
+
+  __device__ inline void add(const float s0[3], const float s1[3], float wq) {
+    float a0[3][3], a1[3][3], fl[3][2];
 #pragma unroll
-    for (int mm = 0; mm < 3; mm++) {
-      const float y0 = a0[1][mm] + 0.5f * a1[1][mm], y1 = a1[1][mm] + 0.5f * a0[1][mm];
-      const float x0 = a0[0][mm] + 0.5f * a1[0][mm], x1 = a1[0][mm] + 0.5f * a0[0][mm];
+    for (int d = 0; d < D; d++) {
+      qweights<float>(s0[d], a0[d]);
+      qweights<float>(s1[d], a1[d]);
+      const float h = 0.5f * wq * (s1[d] - s0[d]);
+      fl[d][0] = h * ((1.f - s0[d]) - s1[d]);
+      fl[d][1] = h * ((1.f + s0[d]) + s1[d]);
+    }
+    if constexpr (D == 2) {
 #pragma unroll
-      for (int l = 0; l < 3; l++) {
-        const float tyz = y0 * a0[2][l] + y1 * a1[2][l];
-        const float txz = x0 * a0[2][l] + x1 * a1[2][l];
-        const float txy = x0 * a0[1][l] + x1 * a1[1][l];
+      for (int mm = 0; mm < 3; mm++) {
+        const float ty2 = a0[1][mm] + a1[1][mm];
+        const float tx2 = a0[0][mm] + a1[0][mm];
 #pragma unroll
         for (int f = 0; f < 2; f++) {
-          atomicAdd(j1 + (f + JX * (mm + JY * l)), (double)(fl[0][f] * tyz));
-          atomicAdd(j1 + PVOL + (mm + JX * (f + JY * l)), (double)(fl[1][f] * txz));
-          atomicAdd(j1 + 2 * PVOL + (mm + JX * (l + JY * f)), (double)(fl[2][f] * txy));
+          jx[f][mm][0] = fmaf(fl[0][f], ty2, jx[f][mm][0]);
+          jy[mm][f][0] = fmaf(fl[1][f], tx2, jy[mm][f][0]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int mm = 0; mm < 3; mm++) {
+        const float y0 = a0[1][mm] + 0.5f * a1[1][mm], y1 = a1[1][mm] + 0.5f * a0[1][mm];
+        const float x0 = a0[0][mm] + 0.5f * a1[0][mm], x1 = a1[0][mm] + 0.5f * a0[0][mm];
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+          const float tyz = y0 * a0[2][l] + y1 * a1[2][l];
+          const float txz = x0 * a0[2][l] + x1 * a1[2][l];
+          const float txy = x0 * a0[1][l] + x1 * a1[1][l];
+#pragma unroll
+          for (int f = 0; f < 2; f++) {
+            jx[f][mm][l] = fmaf(fl[0][f], tyz, jx[f][mm][l]);
+            jy[mm][f][l] = fmaf(fl[1][f], txz, jy[mm][f][l]);
+            jz[mm][l][f] = fmaf(fl[2][f], txy, jz[mm][l][f]);
+          }
         }
       }
     }
   }
-}
+
+  // adds the sums into the FP64 LDS patch; j1 points at node N-1
+  template <int JX, int JY, int PVOL>
+  __device__ inline void flush(double* j1) const {
+    if constexpr (D == 2) {
+#pragma unroll
+      for (int f = 0; f < 2; f++)
+#pragma unroll
+        for (int mm = 0; mm < 3; mm++) {
+          atomicAdd(j1 + (f + JX * mm), (double)jx[f][mm][0]);
+          atomicAdd(j1 + PVOL + (mm + JX * f), (double)jy[mm][f][0]);
+        }
+    } else {
+#pragma unroll
+      for (int mm = 0; mm < 3; mm++)
+#pragma unroll
+        for (int l = 0; l < 3; l++)
+#pragma unroll
+          for (int f = 0; f < 2; f++) {
+            atomicAdd(j1 + (f + JX * (mm + JY * l)), (double)jx[f][mm][l]);
+            atomicAdd(j1 + PVOL + (mm + JX * (f + JY * l)), (double)jy[mm][f][l]);
+            atomicAdd(j1 + 2 * PVOL + (mm + JX * (l + JY * f)), (double)jz[mm][l][f]);
+          }
+    }
+  }
+};
 
 // LDS layout of the tile kernel (all sizes compile-time):
 //   Fl  float  [NF][GVOL]   field patch, nodes T0-1 .. T0+T per axis
 //   Jl  double [D][PVOL]    current patch, nodes T0-2 .. T0+T+1 per axis
 //   Q   int    [NW][8][QCAP] per-wave queue of node-changing particles
-//   offs, fillc int [NCL+1], [NCL]; misc int [2]; perm u16 [kChunk]
+//   offs, fillc int [NCL+1], [NCL]; misc int [3]; perm u16 [kChunk]
 template <int D, int TX, int TY, int TZ, int NT>
 struct TileShape {
   static constexpr int NCL = TX * TY * TZ, NF = D == 3 ? 6 : 3, NW = NT / 64;
@@ -155,7 +192,7 @@ struct TileShape {
   static constexpr int F_WORDS = (NF * GVOL + 1) & ~1;
   static constexpr int J_WORDS = 2 * D * PVOL;
   static constexpr int Q_WORDS = NW * QW * QCAP;
-  static constexpr int I_WORDS = 2 * NCL + 3;
+  static constexpr int I_WORDS = 2 * NCL + 4;
   static constexpr int BYTES = 4 * (F_WORDS + J_WORDS + Q_WORDS + I_WORDS) + 2 * kChunk;
   static_assert(NT % 64 == 0 && NT >= NCL, "threads: whole wavefronts, at least one per cell");
 };
@@ -199,7 +236,7 @@ __global__ __launch_bounds__(NT) void k_tile(
     TileArgs ta, StepConst sc, const float* __restrict__ ebf, RecPtr A, RecPtr B,
     const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
     const int* __restrict__ cnt_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_cur,
-    int* __restrict__ ovf_next, float* __restrict__ patch) {
+    int* __restrict__ ovf_next, float* __restrict__ patch, int* __restrict__ grow) {
   using S = TileShape<D, TX, TY, TZ, NT>;
   constexpr int NCL = S::NCL, NF = S::NF, GX = S::GX, GY = S::GY, GVOL = S::GVOL;
   constexpr int JX = S::JX, JY = S::JY, PVOL = S::PVOL, QCAP = S::QCAP;
@@ -209,7 +246,7 @@ __global__ __launch_bounds__(NT) void k_tile(
   int* Q = (int*)(lds + S::F_WORDS + S::J_WORDS);
   int* offs = Q + S::Q_WORDS;  // NCL + 1
   int* fillc = offs + NCL + 1;  // NCL
-  int* misc = fillc + NCL;      // [0] chunk slot base, [1] stayers
+  int* misc = fillc + NCL;      // [0] chunk slot base, [1] stayers, [2] max per cell
   unsigned short* perm = (unsigned short*)(misc + 2);
 
   const int t = blockIdx.x;
@@ -237,11 +274,20 @@ __global__ __launch_bounds__(NT) void k_tile(
   const int cnt = min(cnt_cur[t], cap_t);
   const int64_t base = seg_start[t];
   const int mi = (1 << ta.bx) - 1, mj = (1 << ta.by) - 1, shk = ta.bx + ta.by;
+  // LDS current patch node N-1 of local cell lc (the patch starts at T0-2)
+  const auto patch_node = [&](int lc) -> double* {
+    const int li = lc % TX, lj = (lc / TX) % TY, lk = D == 3 ? lc / (TX * TY) : 0;
+    return Jl + (li + 1) + JX * ((lj + 1) + JY * (D == 3 ? lk + 1 : 0));
+  };
+  CellAcc<D> acc;
+  acc.zero();
+  int acc_lc = -1;
 
   for (int c0 = 0; c0 < cnt; c0 += kChunk) {
     const int m = min(kChunk, cnt - c0);
     __syncthreads();  // patches staged / previous chunk consumed
     if (me < NCL) offs[me] = 0;
+    if (me == 0) misc[2] = 0;
     __syncthreads();
     for (int q = me; q < m; q += NT) {
       const int cell = A.cell[base + c0 + q];
@@ -255,6 +301,7 @@ __global__ __launch_bounds__(NT) void k_tile(
     {  // exclusive scan of offs[0..NCL) (Hillis-Steele through fillc)
       const int v = me < NCL ? offs[me] : 0;
       if (me < NCL) fillc[me] = v;
+      if (v > 0) atomicMax(&misc[2], v);
       __syncthreads();
 #pragma unroll 1
       for (int off = 1; off < NCL; off <<= 1) {
@@ -291,13 +338,21 @@ __global__ __launch_bounds__(NT) void k_tile(
     __syncthreads();
     const int chunk_base = misc[0];
     const int nval = offs[NCL];
-    // balanced split of the cell-sorted chunk: thread me takes the W
-    // consecutive sorted positions [me*W, me*W+W), so every thread has the
-    // same trip count whatever the density, and at each step the lanes of a
-    // wave sit W positions apart, i.e. on different cells (no two lanes of an
-    // LDS atomic hit the same address unless one cell holds more than W).
+    // balanced split of the cell-sorted chunk into NT slices of W consecutive
+    // sorted positions, so every thread has the same trip count whatever the
+    // density.  The lanes of a wave take slices sigma apart (sigma a power of
+    // two dividing NW), so their particles are sigma*W sorted positions apart:
+    // on different cells when no cell holds more than sigma*W (no two lanes
+    // of an LDS atomic on one address), and on nearby cells when sigma is as
+    // small as that allows (stencil addresses spread over the LDS banks).
+    // sigma is the smallest such stride for the fullest cell of the chunk.
+    constexpr int NW = S::NW;
     const int W = (nval + NT - 1) / NT;
-    const int q0 = me * W, q1 = min(q0 + W, nval);
+    int sigma = 1;
+    while (sigma < NW && sigma * W < misc[2]) sigma <<= 1;
+    const int wv = me >> 6;
+    const int sl = (wv / sigma) * (64 * sigma) + (wv % sigma) + sigma * lane;
+    const int q0 = sl * W, q1 = min(q0 + W, nval);
     const int64_t cb = base + c0;
     Rec nxt = {};
     if (q0 < q1) load_raw<D>(A, cb + perm[q0], nxt);
@@ -330,9 +385,14 @@ __global__ __launch_bounds__(NT) void k_tile(
         cross = q.N[0] != N0[0] || q.N[1] != N0[1];
         if (D == 3) cross = cross || q.N[2] != N0[2];
         if (keep && !cross) {
-          // current patch starts at node T0-2: node N-1 is at local offset +1
-          double* j1 = Jl + (li + 1) + JX * ((lj + 1) + JY * (D == 3 ? lk + 1 : 0));
-          deposit_same_node<D, JX, JY, PVOL>(j1, so, q.s, wq);
+          // a thread's slice runs through the cells in order: sum in
+          // registers while the cell stays, add to LDS when it changes
+          if (lc != acc_lc) {
+            if (acc_lc >= 0) acc.template flush<JX, JY, PVOL>(patch_node(acc_lc));
+            acc.zero();
+            acc_lc = lc;
+          }
+          acc.add(so, q.s, wq);
         }
       }
       // node-changing particles go to the wave's queue
@@ -355,13 +415,19 @@ __global__ __launch_bounds__(NT) void k_tile(
         const int dtile = (q.N[0] - 3) / TX +
                           ta.nt0 * ((q.N[1] - 3) / TY + ta.nt1 * (D == 3 ? (q.N[2] - 3) / TZ : 0));
         int64_t dst;
+        int slot, cap;
         if (dtile == t) {
-          const int slot = chunk_base + reserve_one(&misc[1]);
-          dst = slot < cap_t ? base + slot : sc.ovf_base + atomicAdd(ovf_next, 1);
+          slot = chunk_base + reserve_one(&misc[1]);
+          cap = cap_t;
+          dst = base + slot;
         } else {
-          const int slot = reserve_slot(dtile, cnt_next);
-          dst = slot < seg_cap[dtile] ? (int64_t)seg_start[dtile] + slot
-                                      : sc.ovf_base + atomicAdd(ovf_next, 1);
+          slot = reserve_slot(dtile, cnt_next);
+          cap = seg_cap[dtile];
+          dst = (int64_t)seg_start[dtile] + slot;
+        }
+        if (slot >= cap) {  // segment full: overflow list, and a larger segment next re-layout
+          dst = sc.ovf_base + atomicAdd(ovf_next, 1);
+          grow[dtile] = 1;
         }
         store_rec<D>(B, dst, q, ta.bx, ta.by);
       }
@@ -392,14 +458,16 @@ __global__ __launch_bounds__(NT) void k_tile(
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     drain_queue<D, TX, TY, TZ, NT>(qw, qhead, qcount, Jl);
   }
+  if (acc_lc >= 0) acc.template flush<JX, JY, PVOL>(patch_node(acc_lc));
   __syncthreads();
   float* P = patch + (int64_t)t * D * PVOL;
   for (int q = threadIdx.x; q < D * PVOL; q += NT) P[q] = (float)Jl[q];
 }
 
-// threads per workgroup: 1024 (16 wavefronts) so that one tile per CU keeps
-// four wavefronts on every SIMD while the LDS patches fit
-static constexpr int kTileThreads = 1024;
+// threads per workgroup: 512 (8 wavefronts, one thread per cell of an 8^3
+// tile); the register sums of the current need more than the 128 VGPRs a
+// 1024-thread workgroup allows
+static constexpr int kTileThreads = 512;
 
 template <int D, int TX, int TY, int TZ>
 static int lds_bytes_for() {
@@ -410,10 +478,10 @@ template <int D, int TX, int TY, int TZ>
 static hipError_t launch(const TileGeom& tg, const TileArgs& ta, const StepConst& sc,
                          const float* ebf, const RecPtr& A, const RecPtr& B, const int* ss,
                          const int* sc_, const int* cc, int* cn, int* oc, int* on, float* patch,
-                         hipStream_t s) {
+                         int* grow, hipStream_t s) {
   k_tile<D, TX, TY, TZ, kTileThreads>
       <<<tg.ntiles, kTileThreads, lds_bytes_for<D, TX, TY, TZ>(), s>>>(ta, sc, ebf, A, B, ss, sc_,
-                                                                      cc, cn, oc, on, patch);
+                                                                      cc, cn, oc, on, patch, grow);
   return hipGetLastError();
 }
 
@@ -440,13 +508,13 @@ hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepCo
                               const float* ebf, const RecPtr& A, const RecPtr& B,
                               const int* seg_start, const int* seg_cap, const int* cnt_cur,
                               int* cnt_next, int* ovf_cur, int* ovf_next, float* patch,
-                              hipStream_t s) {
+                              int* grow, hipStream_t s) {
   TileArgs ta{tg.nt[0], tg.nt[1], tg.bx, tg.by, g.n1p, g.n2p, g.n3p, g.ncell};
   const int* T = tg.T;
 #define ALADYN_TILE_CASE(DD, X, Y, Z)                                                        \
   if (g.ndim == DD && T[0] == X && T[1] == Y && (DD == 2 || T[2] == Z))                      \
     return launch<DD, X, Y, Z>(tg, ta, sc, ebf, A, B, seg_start, seg_cap, cnt_cur, cnt_next, \
-                               ovf_cur, ovf_next, patch, s);
+                               ovf_cur, ovf_next, patch, grow, s);
   ALADYN_TILE_CASE(3, 8, 8, 8)
   ALADYN_TILE_CASE(3, 8, 8, 4)
   ALADYN_TILE_CASE(3, 16, 8, 4)

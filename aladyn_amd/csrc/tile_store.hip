@@ -19,7 +19,7 @@ hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepCo
                               const float* ebf, const RecPtr& A, const RecPtr& B,
                               const int* seg_start, const int* seg_cap, const int* cnt_cur,
                               int* cnt_next, int* ovf_cur, int* ovf_next, float* patch,
-                              hipStream_t s);
+                              int* grow, hipStream_t s);
 
 // ------------------------------------------------------------ geometry
 static int bits_for(int n) {  // bits to hold values 0..n
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void k_ovf_particles(
     TileGeom tg, DevGrid g, StepConst sc, const float* __restrict__ ebf, RecPtr A, RecPtr B,
     const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
     const int* __restrict__ ovf_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_next,
-    float* __restrict__ jc) {
+    float* __restrict__ jc, int* __restrict__ grow) {
   const int cnt = *ovf_cur;
   const GlobalAcc32 acc{jc, &g};
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < cnt; n += gridDim.x * blockDim.x) {
@@ -102,8 +102,11 @@ __global__ __launch_bounds__(256) void k_ovf_particles(
     const int dtile = (q.N[0] - 3) / tg.T[0] +
                       tg.nt[0] * ((q.N[1] - 3) / tg.T[1] + tg.nt[1] * (D == 3 ? (q.N[2] - 3) / tg.T[2] : 0));
     const int slot = reserve_slot(dtile, cnt_next);
-    const int64_t dst = slot < seg_cap[dtile] ? (int64_t)seg_start[dtile] + slot
-                                              : sc.ovf_base + atomicAdd(ovf_next, 1);
+    int64_t dst = (int64_t)seg_start[dtile] + slot;
+    if (slot >= seg_cap[dtile]) {
+      dst = sc.ovf_base + atomicAdd(ovf_next, 1);
+      grow[dtile] = 1;
+    }
     store_rec<D>(B, dst, q, tg.bx, tg.by);
   }
 }
@@ -184,15 +187,44 @@ __device__ inline int block_scan_excl(int v, int* sh, int& total) {
   return ex;
 }
 
-// Segment capacity of a tile holding v particles at a re-layout: half again
-// as many slots (HBM is plentiful; unused slack is never read) so that tiles
-// gaining particles between re-layouts rarely spill into the overflow list.
-__host__ __device__ inline int64_t tile_capacity(int64_t v) { return v + v / 2 + 64; }
+// Segment capacities at a re-layout.  The segments of one buffer share a
+// budget of kSegBudget records per particle (HBM is plentiful and slack that
+// no particle uses is never read): a tile holding v particles gets
+// v + v/2 + 64 slots, or 2v + 256 if it spilled since the last re-layout,
+// and the rest of the budget is spread evenly over all tiles.  The even share
+// matters where the density moves fast: a tile swept by the dense rear of a
+// plasma wake gains thousands of particles per step from a low count.
+static constexpr int64_t kSegBudgetNum = 5, kSegBudgetDen = 2;  // 2.5 slots per particle
+__host__ __device__ inline int64_t tile_base_capacity(int64_t v, bool grown) {
+  return grown ? 2 * v + 256 : v + v / 2 + 64;
+}
+// slots of the segment area for np particles in ntiles tiles (bounds every
+// sum of base capacities, since sum v <= np)
+static inline int64_t seg_budget(int64_t np, int64_t ntiles) {
+  return np * kSegBudgetNum / kSegBudgetDen + 256 * ntiles + 64;
+}
 
-// out[t] = exclusive scan of v[t], out[n] = total (one workgroup); with
-// cap_out, v is turned into its segment capacity first.
-__global__ __launch_bounds__(kB) void k_scan_tiles(const int* v_in, int n, int* out, int* cap_out) {
+// out[t] = exclusive scan of v[t], out[n] = total (one workgroup).  With
+// cap_out, v[t] is first turned into the segment capacity of tile t (base
+// capacity plus an even share of what the budget leaves; the tile's grow
+// flag is consumed).
+__global__ __launch_bounds__(kB) void k_scan_tiles(const int* v_in, int n, int* out, int* cap_out,
+                                                   int* grow = nullptr, int64_t budget = 0) {
   __shared__ int sh[kB];
+  __shared__ long long red[kB];
+  if (cap_out) {
+    long long part = 0;
+    for (int t = threadIdx.x; t < n; t += kB) part += tile_base_capacity(v_in[t], grow && grow[t]);
+    red[threadIdx.x] = part;  // block sum of the base capacities
+    __syncthreads();
+    for (int o = kB / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+  }
+  const long long left = cap_out ? budget - red[0] : 0;
+  const long long extra = left > 0 ? left / n : 0;
+  __syncthreads();
   long long carry = 0;
   for (int b0 = 0; b0 < n; b0 += kB) {
     const int t = b0 + threadIdx.x;
@@ -200,7 +232,8 @@ __global__ __launch_bounds__(kB) void k_scan_tiles(const int* v_in, int n, int* 
     if (t < n) {
       v = v_in[t];
       if (cap_out) {
-        v = (int)tile_capacity(v);
+        v = (int)(tile_base_capacity(v, grow && grow[t]) + extra);
+        if (grow) grow[t] = 0;
         cap_out[t] = v;
       }
     }
@@ -272,6 +305,53 @@ __global__ void k_scatter(TileGeom tg, RecPtr U, int64_t n, RecPtr A, const int*
     const int t = tile_of(tg, U.cell[q], D);
     const int slot = reserve_slot(t, fillc);
     copy_rec<D>(U, q, A, (int64_t)seg_start[t] + slot);
+  }
+}
+
+// One-pass re-layout: counts per destination tile straight from the tile
+// segments of A (holes skipped) and its overflow list ...
+template <int D>
+__global__ __launch_bounds__(kB) void k_hist_seg(TileGeom tg, RecPtr A, const int* seg_start,
+                                                 const int* seg_cap, const int* cnt,
+                                                 int64_t ovf_base, const int* ovf_cnt,
+                                                 int* hcount) {
+  const int t = blockIdx.x;
+  if (t < tg.ntiles) {
+    const int n = min(cnt[t], seg_cap[t]);
+    for (int q = threadIdx.x; q < n; q += kB) {
+      const int cell = A.cell[(int64_t)seg_start[t] + q];
+      if (cell != kHole) (void)reserve_slot(tile_of(tg, cell, D), hcount);
+    }
+  } else {
+    const int n = *ovf_cnt;
+    for (int q = (t - tg.ntiles) * kB + threadIdx.x; q < n; q += (gridDim.x - tg.ntiles) * kB)
+      (void)reserve_slot(tile_of(tg, A.cell[ovf_base + q], D), hcount);
+  }
+}
+
+// ... and moves every live record of A into its tile's new segment of U.
+template <int D>
+__global__ __launch_bounds__(kB) void k_scatter_seg(TileGeom tg, RecPtr A, const int* seg_start,
+                                                    const int* seg_cap, const int* cnt,
+                                                    int64_t ovf_base, const int* ovf_cnt, RecPtr U,
+                                                    const int* new_start, int* fillc) {
+  const int t = blockIdx.x;
+  if (t < tg.ntiles) {
+    const int n = min(cnt[t], seg_cap[t]);
+    for (int q = threadIdx.x; q < n; q += kB) {
+      const int64_t src = (int64_t)seg_start[t] + q;
+      const int cell = A.cell[src];
+      if (cell == kHole) continue;
+      const int d = tile_of(tg, cell, D);
+      copy_rec<D>(A, src, U, (int64_t)new_start[d] + reserve_slot(d, fillc));
+    }
+  } else {
+    const int n = *ovf_cnt;
+    for (int q = (t - tg.ntiles) * kB + threadIdx.x; q < n; q += (gridDim.x - tg.ntiles) * kB) {
+      const int64_t src = ovf_base + q;
+      const int d = tile_of(tg, A.cell[src], D);
+      copy_rec<D>(A, src, U, (int64_t)new_start[d] + reserve_slot(d, fillc));
+    }
   }
 }
 
@@ -361,16 +441,20 @@ void tile_store_free(TileStore* ts) {
   }
   (void)hipFree(ts->seg_start);
   (void)hipFree(ts->seg_cap);
+  (void)hipFree(ts->seg_start2);
+  (void)hipFree(ts->seg_cap2);
   (void)hipFree(ts->ovf_count);
   (void)hipFree(ts->scratch);
   (void)hipFree(ts->d_total);
+  (void)hipFree(ts->grow);
   if (ts->h_ovf) (void)hipHostFree(ts->h_ovf);
-  if (ts->ovf_ev) (void)hipEventDestroy(ts->ovf_ev);
+  for (int b = 0; b < 2; b++)
+    if (ts->ovf_ev[b]) (void)hipEventDestroy(ts->ovf_ev[b]);
   *ts = TileStore();
 }
 
 // Counting sort of the n records of U (stored in buffer 1-cur) into buffer
-// cur with capacities tile_capacity(cnt) per tile.
+// cur with the re-layout capacities of k_scan_tiles.
 template <int D>
 static int sort_into_cur(TileStore* ts, const TileGeom& tg, int64_t n, hipStream_t s,
                          std::string* msg) {
@@ -378,7 +462,8 @@ static int sort_into_cur(TileStore* ts, const TileGeom& tg, int64_t n, hipStream
   int* h = ts->count[cur];
   fill<int>(h, tg.ntiles, 0, s);
   if (n > 0) k_hist<D><<<grid_for(n), kB, 0, s>>>(tg, ts->buf[oth], n, h);
-  k_scan_tiles<<<1, kB, 0, s>>>(h, tg.ntiles, ts->seg_start, ts->seg_cap);
+  k_scan_tiles<<<1, kB, 0, s>>>(h, tg.ntiles, ts->seg_start, ts->seg_cap, ts->grow,
+                                seg_budget(ts->ovf_cap, tg.ntiles));
   int total = 0;
   TS_CK(hipMemcpyAsync(&total, ts->seg_start + tg.ntiles, sizeof(int), hipMemcpyDeviceToHost, s));
   TS_CK(hipStreamSynchronize(s));
@@ -393,6 +478,7 @@ static int sort_into_cur(TileStore* ts, const TileGeom& tg, int64_t n, hipStream
   fill<int>(ts->ovf_count, 2, 0, s);
   TS_CK(hipGetLastError());
   ts->np = n;
+  ts->ovf_valid[0] = ts->ovf_valid[1] = false;
   ts->steps_since_sort = 0;
   return 0;
 }
@@ -414,7 +500,7 @@ int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const do
       return ALADYN_EINVAL;
     }
   }
-  const int64_t need = tile_capacity(np) + 64LL * tg.ntiles;  // segments (bounds the sum)
+  const int64_t need = seg_budget(np, tg.ntiles);  // segments
   const int64_t alloc = need + (np > 0 ? np : 1);            // + overflow region
   if (alloc >= (1LL << 31)) {
     *msg = "FP32 tile path: particle count exceeds the 31-bit slot range";
@@ -432,11 +518,15 @@ int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const do
     }
     TS_CK(hipMalloc(&ts->seg_start, sizeof(int) * (tg.ntiles + 1)));
     TS_CK(hipMalloc(&ts->seg_cap, sizeof(int) * (tg.ntiles + 1)));
+    TS_CK(hipMalloc(&ts->seg_start2, sizeof(int) * (tg.ntiles + 1)));
+    TS_CK(hipMalloc(&ts->seg_cap2, sizeof(int) * (tg.ntiles + 1)));
     TS_CK(hipMalloc(&ts->ovf_count, sizeof(int) * 2));
     TS_CK(hipMalloc(&ts->scratch, sizeof(int) * (tg.ntiles + 2)));
     TS_CK(hipMalloc(&ts->d_total, 64));
+    TS_CK(hipMalloc(&ts->grow, sizeof(int) * tg.ntiles));
     TS_CK(hipHostMalloc(&ts->h_ovf, 2 * sizeof(int), hipHostMallocDefault));
-    TS_CK(hipEventCreateWithFlags(&ts->ovf_ev, hipEventDisableTiming));
+    for (int b = 0; b < 2; b++)
+      TS_CK(hipEventCreateWithFlags(&ts->ovf_ev[b], hipEventDisableTiming));
     ts->h_ovf[0] = ts->h_ovf[1] = 0;
   }
   ts->ovf_cap = np > 0 ? np : 1;
@@ -457,6 +547,7 @@ int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const do
       k_convert_in<3><<<grid_for(np), kB, 0, s>>>(g, tg.bx, tg.by, dpart, np, np, ts->buf[1]);
     TS_CK(hipGetLastError());
   }
+  fill<int>(ts->grow, tg.ntiles, 0, s);
   int r = D == 2 ? sort_into_cur<2>(ts, tg, np, s, msg) : sort_into_cur<3>(ts, tg, np, s, msg);
   TS_CK(hipStreamSynchronize(s));
   (void)hipFree(dpart);
@@ -485,14 +576,37 @@ static int gather_live(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t
   return 0;
 }
 
+// One-pass re-layout of buffer cur into buffer 1-cur (which becomes cur):
+// count per destination tile from the segments, fresh capacities, scatter.
+// Fully asynchronous: the capacities fit the segment area by construction
+// (live particles never exceed the count given to put).
+template <int D>
+static int relayout_seg(TileStore* ts, const TileGeom& tg, hipStream_t s, std::string* msg) {
+  const int cur = ts->cur, oth = 1 - cur;
+  int* h = ts->count[oth];  // per-tile counts of the new layout
+  fill<int>(h, tg.ntiles, 0, s);
+  k_hist_seg<D><<<tg.ntiles + 64, kB, 0, s>>>(tg, ts->buf[cur], ts->seg_start, ts->seg_cap,
+                                              ts->count[cur], ts->ovf_base, ts->ovf_count + cur, h);
+  k_scan_tiles<<<1, kB, 0, s>>>(h, tg.ntiles, ts->seg_start2, ts->seg_cap2, ts->grow,
+                                seg_budget(ts->ovf_cap, tg.ntiles));
+  fill<int>(ts->scratch, tg.ntiles, 0, s);
+  k_scatter_seg<D><<<tg.ntiles + 64, kB, 0, s>>>(tg, ts->buf[cur], ts->seg_start, ts->seg_cap,
+                                                 ts->count[cur], ts->ovf_base, ts->ovf_count + cur,
+                                                 ts->buf[oth], ts->seg_start2, ts->scratch);
+  fill<int>(ts->ovf_count, 2, 0, s);
+  TS_CK(hipGetLastError());
+  std::swap(ts->seg_start, ts->seg_start2);
+  std::swap(ts->seg_cap, ts->seg_cap2);
+  ts->cur = oth;
+  ts->ovf_valid[0] = ts->ovf_valid[1] = false;
+  ts->steps_since_sort = 0;
+  return 0;
+}
+
 int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
                         std::string* msg) {
   if (!ts->loaded) return 0;
-  int64_t n = 0;
-  const int D = g.ndim;
-  int r = D == 2 ? gather_live<2>(ts, tg, s, &n, msg) : gather_live<3>(ts, tg, s, &n, msg);
-  if (r) return r;
-  return D == 2 ? sort_into_cur<2>(ts, tg, n, s, msg) : sort_into_cur<3>(ts, tg, n, s, msg);
+  return g.ndim == 2 ? relayout_seg<2>(ts, tg, s, msg) : relayout_seg<3>(ts, tg, s, msg);
 }
 
 int tile_store_count(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t* np,
@@ -581,27 +695,39 @@ int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const f
   if (ev_begin) TS_CK(hipEventRecord(ev_begin, s));
   TS_CK(launch_tile_kernel(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt], ts->seg_start, ts->seg_cap,
                            ts->count[cur], ts->count[nxt], ts->ovf_count + cur,
-                           ts->ovf_count + nxt, patch, s));
+                           ts->ovf_count + nxt, patch, ts->grow, s));
   if (ev_end) TS_CK(hipEventRecord(ev_end, s));
   k_fold<<<grid_for(g.ncell, 8192), kB, 0, s>>>(tg, g, patch, jc, j_is_zero ? 0 : 1);
   if (D == 2)
     k_ovf_particles<2><<<512, kB, 0, s>>>(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt], ts->seg_start,
                                           ts->seg_cap, ts->ovf_count + cur, ts->count[nxt],
-                                          ts->ovf_count + nxt, jc);
+                                          ts->ovf_count + nxt, jc, ts->grow);
   else
     k_ovf_particles<3><<<512, kB, 0, s>>>(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt], ts->seg_start,
                                           ts->seg_cap, ts->ovf_count + cur, ts->count[nxt],
-                                          ts->ovf_count + nxt, jc);
+                                          ts->ovf_count + nxt, jc, ts->grow);
   TS_CK(hipGetLastError());
   ts->cur = nxt;
   ts->steps_since_sort++;
-  // lazily watch the overflow count; re-layout on cadence or on growth
+  // Re-layout on cadence, or as soon as the overflow list is long enough to
+  // cost more than a re-layout amortises (its particles deposit with global
+  // atomics).  The count of this step is copied into one pinned slot; the
+  // host waits only for the previous step's slot, so one step stays queued
+  // on the GPU while the host decides.
+  const int slot = ts->ovf_slot, prev = 1 - slot;
+  TS_CK(hipMemcpyAsync(ts->h_ovf + slot, ts->ovf_count + nxt, sizeof(int), hipMemcpyDeviceToHost, s));
+  TS_CK(hipEventRecord(ts->ovf_ev[slot], s));
+  ts->ovf_valid[slot] = true;
+  ts->ovf_slot = prev;
   bool relayout = ts->steps_since_sort >= tg.sort_every;
-  if (hipEventQuery(ts->ovf_ev) == hipSuccess && ts->h_ovf[0] > (int)(ts->ovf_cap / 512 + 1024))
-    relayout = true;
-  TS_CK(hipMemcpyAsync(ts->h_ovf, ts->ovf_count + nxt, sizeof(int), hipMemcpyDeviceToHost, s));
-  TS_CK(hipEventRecord(ts->ovf_ev, s));
-  if (relayout) return tile_store_relayout(ts, tg, g, s, msg);
+  if (!relayout && ts->ovf_valid[prev]) {
+    TS_CK(hipEventSynchronize(ts->ovf_ev[prev]));
+    if (ts->h_ovf[prev] > (int)(ts->ovf_cap / 8192 + 2048)) relayout = true;
+  }
+  if (relayout) {
+    ts->ovf_valid[0] = ts->ovf_valid[1] = false;  // counts of the old layout
+    return tile_store_relayout(ts, tg, g, s, msg);
+  }
   return 0;
 }
 

@@ -58,6 +58,8 @@ struct TileStore {
   int cur = 0;
   int* seg_start = nullptr;  // ntiles + 1
   int* seg_cap = nullptr;    // ntiles
+  int* seg_start2 = nullptr;  // the next layout's, swapped in at a re-layout
+  int* seg_cap2 = nullptr;
   int* count[2] = {nullptr, nullptr};
   int* ovf_count = nullptr;  // [2]
   int* scratch = nullptr;    // ntiles + 1 ints for scans / fills
@@ -67,8 +69,13 @@ struct TileStore {
   bool loaded = false;
   int64_t np = 0;  // particles at the last put / re-layout
   int steps_since_sort = 0;
-  int* h_ovf = nullptr;  // pinned: overflow count read back lazily
-  hipEvent_t ovf_ev = nullptr;
+  int* grow = nullptr;   // ntiles flags: the tile spilled since the last re-layout
+  // overflow count of recent steps, read back through two pinned slots: the
+  // host waits only for the step before the one just queued
+  int* h_ovf = nullptr;  // pinned [2]
+  hipEvent_t ovf_ev[2] = {nullptr, nullptr};
+  bool ovf_valid[2] = {false, false};
+  int ovf_slot = 0;
 };
 
 int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg);

@@ -21,12 +21,12 @@ pytestmark = pytest.mark.gpu
 CASES = {"2d": small_2d, "3d": small_3d}
 
 
-def _run(cfg, part, ebf, precision, nsteps):
+def _run(cfg, part, ebf, precision, nsteps, **kw):
     orc = Oracle(cfg)
     p = part.copy(order="F")
     e = ebf.copy(order="F")
     jc = orc.zeros_current()
-    gpu = LPStep(cfg, precision, track_ids=True)
+    gpu = LPStep(cfg, precision, track_ids=True, **kw)
     gpu.put_species(0, part)
     gpu.put_fields(ebf)
     for it in range(nsteps):
@@ -73,6 +73,20 @@ def test_f32_one_step(dim):
 def test_f32_ten_steps(dim):
     cfg, part, ebf = CASES[dim]()
     _check(cfg, _run(cfg, part, ebf, PREC_F32, 10), 1e-4, check_current=False)
+
+
+@pytest.mark.parametrize("dim", ["2d", "3d"])
+def test_f32_relayout_every_step(dim):
+    """The tile store re-laid out after every step (one-pass count + scatter
+    into the other buffer) gives the same ten steps."""
+    cfg, part, ebf = CASES[dim]()
+    _check(cfg, _run(cfg, part, ebf, PREC_F32, 10, sort_every=1), 1e-4, check_current=False)
+
+
+@pytest.mark.parametrize("dim,tile", [("3d", (8, 8, 4)), ("3d", (16, 8, 4)), ("2d", (32, 16))])
+def test_f32_other_tile_shapes(dim, tile):
+    cfg, part, ebf = CASES[dim]()
+    _check(cfg, _run(cfg, part, ebf, PREC_F32, 3, tile=tile), 1e-5)
 
 
 @pytest.mark.parametrize("precision,tol", [(PREC_F64, 1e-12), (PREC_F32, 1e-5)])
