@@ -59,6 +59,9 @@ SIGNATURES = {
     "aladyn_particle_stage": [_P, C.c_int32, C.c_int32],
     "aladyn_curr_mpi_collect": [_P],
     "aladyn_advance_lpf_fields": [_P],
+    "aladyn_lp_window_xshift": [_P, C.c_int32, C.c_int32, C.c_double, C.POINTER(C.c_int32)],
+    "aladyn_inject_species": [_P, C.c_int32, _D, _I64, _I64],
+    "aladyn_box": [_P, _D],
     "aladyn_energy_momenta": [_P, C.c_int32, _D, _D],
     "aladyn_field_energy": [_P, _D],
     "aladyn_grid_charge": [_P, C.c_int32, _D],

@@ -3,6 +3,7 @@
 // orchestration of lpf2_evolve (pic_evolve_in_time.f90:37-109).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -94,6 +95,9 @@ struct aladyn_ctx {
   double* rho1 = nullptr;
   void* jraw = nullptr;
   bool have_resid = false;
+  // lp_window_xshift saved state (window.f90:278-280)
+  double win_xlapse = 0.0;
+  int win_wi1 = 0;
   // timing: event pairs around the particle kernel and the field solve,
   // recorded on the context stream (ring of kRing steps)
   TimerRing tpart, tfield;
@@ -281,15 +285,12 @@ static int check_species(aladyn_ctx* c, int32_t ic) {
   return 0;
 }
 
-extern "C" int aladyn_put_species(aladyn_ctx* c, int32_t ic, const double* part, int64_t ld,
-                                  int64_t np) {
-  if (int r = check_species(c, ic)) return r;
-  if (np < 0 || (np > 0 && (!part || ld < np)))
-    return fail(c, ALADYN_EINVAL, "put_species: need part != NULL and ld >= np");
+// Positions must lie in the particle box (x > gmin, x <= gmax): every kernel
+// index is derived from them, so this is the bounds guarantee.  Momenta
+// must be finite.
+static int check_particles(aladyn_ctx* c, const double* part, int64_t ld, int64_t np) {
   const DevGrid& g = c->g;
-  const int D = g.curr_ndim, ncol = c->ncol;
-  // positions must be in the particle box (x > gmin, x <= gmax): every
-  // kernel index is derived from them, so this is the bounds guarantee
+  const int D = g.curr_ndim;
   const double mn[3] = {g.xmn, g.ymn, g.zmn}, mx[3] = {g.xmx, g.ymx, g.zmx};
   for (int d = 0; d < D; d++)
     for (int64_t n = 0; n < np; n++) {
@@ -302,6 +303,17 @@ extern "C" int aladyn_put_species(aladyn_ctx* c, int32_t ic, const double* part,
     for (int64_t n = 0; n < np; n++)
       if (!std::isfinite(part[d * ld + n]))
         return fail(c, ALADYN_EINVAL, "particle %lld momentum is not finite", (long long)n);
+  return 0;
+}
+
+extern "C" int aladyn_put_species(aladyn_ctx* c, int32_t ic, const double* part, int64_t ld,
+                                  int64_t np) {
+  if (int r = check_species(c, ic)) return r;
+  if (np < 0 || (np > 0 && (!part || ld < np)))
+    return fail(c, ALADYN_EINVAL, "put_species: need part != NULL and ld >= np");
+  const DevGrid& g = c->g;
+  const int ncol = c->ncol;
+  if (int r = check_particles(c, part, ld, np)) return r;
   if (f64(c)) {
     SpeciesF64& s = c->sp64[ic];
     if (np > s.cap) {
@@ -342,6 +354,119 @@ extern "C" int aladyn_species_count(aladyn_ctx* c, int32_t ic, int64_t* np) {
   std::string msg;
   int r = tile_store_count(&c->ts[ic], c->tg, c->stream, np, &msg);
   if (r) return fail(c, r, "%s", msg.c_str());
+  return ALADYN_OK;
+}
+
+// FP64 species: room for np particles, keeping the first s.np rows.
+static int reserve_f64(aladyn_ctx* c, SpeciesF64& s, int64_t np) {
+  if (np <= s.cap) return 0;
+  const int ncol = c->ncol;
+  const int64_t cap = std::max(np, 2 * s.cap);
+  double *part = nullptr, *tmp = nullptr;
+  HIPCK(c, hipMalloc(&part, (size_t)cap * ncol * sizeof(double)));
+  if (hipMalloc(&tmp, (size_t)cap * ncol * sizeof(double)) != hipSuccess) {
+    (void)hipFree(part);
+    return fail(c, ALADYN_ENOMEM, "out of device memory growing species storage");
+  }
+  for (int col = 0; col < ncol && s.np > 0; col++)
+    HIPCK(c, hipMemcpyAsync(part + col * cap, s.part + col * s.cap, (size_t)s.np * sizeof(double),
+                            hipMemcpyDeviceToDevice, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(s.part);
+  (void)hipFree(s.tmp);
+  s.part = part;
+  s.tmp = tmp;
+  s.cap = cap;
+  const int64_t nb = nblk_compact(cap);
+  if (nb > c->blk_cap) {
+    (void)hipFree(c->d_blk);
+    HIPCK(c, hipMalloc(&c->d_blk, (size_t)(nb + 1) * sizeof(int)));
+    c->blk_cap = nb;
+  }
+  return 0;
+}
+
+extern "C" int aladyn_lp_window_xshift(aladyn_ctx* c, int32_t witr, int32_t init_iter,
+                                       double w_speed, int32_t* nshx_out) {
+  if (!c) return ALADYN_EINVAL;
+  if (!nshx_out) return fail(c, ALADYN_EINVAL, "nshx is NULL");
+  *nshx_out = 0;
+  if (init_iter == 0) {  // window.f90:284-288
+    c->win_xlapse = 0.0;
+    c->win_wi1 = 0;
+    return ALADYN_OK;
+  }
+  if (!(w_speed >= 0.0) || witr < 0)
+    return fail(c, ALADYN_EINVAL, "window: w_speed and witr must be >= 0");
+  DevGrid& g = c->g;
+  const double xlapse = c->win_xlapse + w_speed * g.dt * witr;
+  const long wi2 = std::lround(g.dx_inv * xlapse);
+  const long nshx = wi2 - c->win_wi1;
+  if (nshx >= g.ix2)  // the reference sets ier = 2 (window.f90:314-319)
+    return fail(c, ALADYN_ERANGE, "window shift of %ld cells exceeds the box", nshx);
+  c->win_xlapse = xlapse;
+  c->win_wi1 = (int)wi2;
+  *nshx_out = (int32_t)nshx;
+  if (nshx == 0) return ALADYN_OK;
+  g.xmn += g.dx * nshx;
+  g.xmx += g.dx * nshx;
+  if (c->have_fields) {
+    if (f64(c))
+      fields_left_xshift<double>(g, c->ebf64, (int)nshx, c->stream);
+    else
+      fields_left_xshift<float>(g, c->ebf32, (int)nshx, c->stream);
+    HIPCK(c, hipGetLastError());
+  }
+  // particles behind the new left face leave (cell_part_dist(.true.))
+  for (int ic = 0; ic < c->cfg.nsp; ic++) {
+    if (f64(c)) {
+      SpeciesF64& s = c->sp64[ic];
+      if (s.np == 0) continue;
+      compact_f64(g, s.part, s.tmp, s.cap, s.np, c->d_blk, c->d_count, c->stream);
+      HIPCK(c, hipGetLastError());
+      int64_t n = 0;
+      HIPCK(c, hipMemcpyAsync(&n, c->d_count, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+      HIPCK(c, hipStreamSynchronize(c->stream));
+      std::swap(s.part, s.tmp);
+      s.np = n;
+    } else {
+      std::string msg;
+      if (int r = tile_store_window(&c->ts[ic], c->tg, g, (int)nshx, c->stream, &msg))
+        return fail(c, r, "%s", msg.c_str());
+    }
+  }
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_inject_species(aladyn_ctx* c, int32_t ic, const double* part, int64_t ld,
+                                     int64_t np) {
+  if (int r = check_species(c, ic)) return r;
+  if (np < 0 || (np > 0 && (!part || ld < np)))
+    return fail(c, ALADYN_EINVAL, "inject_species: need part != NULL and ld >= np");
+  if (np == 0) return ALADYN_OK;
+  if (int r = check_particles(c, part, ld, np)) return r;
+  if (f64(c)) {
+    SpeciesF64& s = c->sp64[ic];
+    if (int r = reserve_f64(c, s, s.np + np)) return r;
+    for (int col = 0; col < c->ncol; col++)
+      HIPCK(c, hipMemcpyAsync(s.part + col * s.cap + s.np, part + col * ld,
+                              (size_t)np * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));  // the host array is only borrowed
+    s.np += np;
+    return ALADYN_OK;
+  }
+  std::string msg;
+  if (int r = tile_store_inject(&c->ts[ic], c->tg, c->g, part, ld, np, c->stream,
+                                c->cfg.reserved[0] != 0, &msg))
+    return fail(c, r, "%s", msg.c_str());
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_box(aladyn_ctx* c, double out[6]) {
+  if (!c || !out) return ALADYN_EINVAL;
+  const DevGrid& g = c->g;
+  const double v[6] = {g.xmn, g.xmx, g.ymn, g.ymx, g.zmn, g.zmx};
+  for (int i = 0; i < 6; i++) out[i] = v[i];
   return ALADYN_OK;
 }
 

@@ -395,7 +395,38 @@ void fill(T* a, int64_t n, T v, hipStream_t s) {
   k_fill<T><<<nblocks(n), kFieldBlock, 0, s>>>(a, n, v);
 }
 
+// fields_left_xshift (mpi_field_interface.f90:299-359), one rank: for every
+// component and interior row (jy1..jy2, kz1..kz2), f(ix) = f(ix + shx) for
+// ix = ix1 .. ix2 - shx and f = 0 on the last shx columns ix2-shx+1 .. ix2.
+// One workgroup per row walks it left to right in blocks: a block of
+// sources is read before its destinations (shx columns to the left) are
+// written, and later blocks only read columns further right.
+template <typename T>
+__global__ __launch_bounds__(kFieldBlock) void k_left_xshift(DevGrid g, T* ebf, int shx) {
+  const int ny = g.jy2 - g.jy1 + 1, nz = g.kz2 - g.kz1 + 1;
+  const int row = blockIdx.x;
+  const int iy = g.jy1 + row % ny, iz = g.kz1 + (row / ny) % nz, ic = row / (ny * nz) + 1;
+  T* f = comp(ebf, g, ic) + fofs(g, 1, iy, iz) - 1;  // f[i] = ebf(i, iy, iz, ic)
+  const int i1 = g.ix1, i2 = g.ix2 - shx;
+  for (int b = i1; b <= g.ix2; b += kFieldBlock) {
+    const int ix = b + (int)threadIdx.x;
+    T v = T(0);
+    if (ix <= i2) v = f[ix + shx];
+    __syncthreads();
+    if (ix <= g.ix2) f[ix] = v;
+    __syncthreads();
+  }
+}
+
+template <typename T>
+void fields_left_xshift(const DevGrid& g, T* ebf, int shx, hipStream_t s) {
+  if (shx <= 0) return;
+  const int rows = (g.jy2 - g.jy1 + 1) * (g.kz2 - g.kz1 + 1) * g.nfield;
+  k_left_xshift<T><<<rows, kFieldBlock, 0, s>>>(g, ebf, shx);
+}
+
 #define ALADYN_INST_FIELDS(T)                                              \
+  template void fields_left_xshift<T>(const DevGrid&, T*, int, hipStream_t); \
   template void field_xyzbd<T>(const DevGrid&, T*, int, hipStream_t);      \
   template void curr_collect<T>(const DevGrid&, T*, hipStream_t);          \
   template void advance_fields<T>(const DevGrid&, T*, const T*, hipStream_t); \

@@ -11,6 +11,7 @@ template <typename T> void field_xyzbd(const DevGrid& g, T* ebf, int nc, hipStre
 template <typename T> void curr_collect(const DevGrid& g, T* jc, hipStream_t s);
 template <typename T> void advance_fields(const DevGrid& g, T* ebf, const T* jc, hipStream_t s);
 template <typename T> void fill(T* a, int64_t n, T v, hipStream_t s);
+template <typename T> void fields_left_xshift(const DevGrid& g, T* ebf, int shx, hipStream_t s);
 
 // ------------------------------------------------------- particles_f64.hip
 // Fused gather + charge multiply + [back-shift] + push + box test + Esirkepov

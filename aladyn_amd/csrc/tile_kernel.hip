@@ -195,6 +195,7 @@ struct TileShape {
   static constexpr int I_WORDS = 2 * NCL + 4;
   static constexpr int BYTES = 4 * (F_WORDS + J_WORDS + Q_WORDS + I_WORDS) + 2 * kChunk;
   static_assert(NT % 64 == 0 && NT >= NCL, "threads: whole wavefronts, at least one per cell");
+  static_assert(kChunk % NT == 0, "a chunk is a whole number of records per thread");
 };
 
 // Drains n <= 64 queued node-changing particles, one per lane, through the
@@ -252,15 +253,27 @@ __global__ __launch_bounds__(NT) void k_tile(
   const int t = blockIdx.x;
   const int tx = t % ta.nt0, ty = (t / ta.nt0) % ta.nt1, tz = t / (ta.nt0 * ta.nt1);
   const int T0x = 3 + tx * TX, T0y = 3 + ty * TY, T0z = D == 3 ? 3 + tz * TZ : 1;
-  // field patch: nodes T0-1 .. T0+T
-  for (int q = threadIdx.x; q < NF * GVOL; q += NT) {
-    const int c = q / GVOL, r = q - c * GVOL;
-    const int li = r % GX, lj = (r / GX) % GY, lk = r / (GX * GY);
-    const int i = T0x - 1 + li, j = T0y - 1 + lj, k = D == 3 ? T0z - 1 + lk : 1;
-    float v = 0.f;
-    if (i <= ta.n1p && j <= ta.n2p && k <= ta.n3p)
-      v = ebf[(int64_t)c * ta.ncell + (i - 1) + (int64_t)ta.n1p * ((j - 1) + (int64_t)ta.n2p * (k - 1))];
-    Fl[q] = v;
+  // field patch: nodes T0-1 .. T0+T; every load of the thread is issued
+  // before the first LDS store, so the HBM latency is paid once per tile
+  {
+    constexpr int FPT = (NF * GVOL + NT - 1) / NT;
+    float fv[FPT];
+#pragma unroll
+    for (int u = 0; u < FPT; u++) {
+      const int q = threadIdx.x + u * NT;
+      const int c = q / GVOL, r = q - c * GVOL;
+      const int li = r % GX, lj = (r / GX) % GY, lk = r / (GX * GY);
+      const int i = T0x - 1 + li, j = T0y - 1 + lj, k = D == 3 ? T0z - 1 + lk : 1;
+      fv[u] = 0.f;
+      if (q < NF * GVOL && i <= ta.n1p && j <= ta.n2p && k <= ta.n3p)
+        fv[u] = ebf[(int64_t)c * ta.ncell + (i - 1) +
+                    (int64_t)ta.n1p * ((j - 1) + (int64_t)ta.n2p * (k - 1))];
+    }
+#pragma unroll
+    for (int u = 0; u < FPT; u++) {
+      const int q = threadIdx.x + u * NT;
+      if (q < NF * GVOL) Fl[q] = fv[u];
+    }
   }
   for (int q = threadIdx.x; q < D * PVOL; q += NT) Jl[q] = 0.0;
 
@@ -288,15 +301,29 @@ __global__ __launch_bounds__(NT) void k_tile(
     __syncthreads();  // patches staged / previous chunk consumed
     if (me < NCL) offs[me] = 0;
     if (me == 0) misc[2] = 0;
-    __syncthreads();
-    for (int q = me; q < m; q += NT) {
-      const int cell = A.cell[base + c0 + q];
+    // the chunk's packed cells, loaded once (all loads in flight together)
+    // and kept in registers for the histogram and the permutation
+    constexpr int CPT = kChunk / NT;
+    int lcv[CPT];
+#pragma unroll
+    for (int u = 0; u < CPT; u++) {
+      const int q = me + u * NT;
+      lcv[u] = q < m ? A.cell[base + c0 + q] : kHole;
+    }
+    // local cell index, kHole for a hole, -2 for a record of another tile
+#pragma unroll
+    for (int u = 0; u < CPT; u++) {
+      const int cell = lcv[u];
       if (cell == kHole) continue;
       const int li = (cell & mi) - T0x, lj = ((cell >> ta.bx) & mj) - T0y;
       const int lk = D == 3 ? (cell >> shk) - T0z : 0;
-      if (li >= 0 && li < TX && lj >= 0 && lj < TY && lk >= 0 && lk < TZ)
-        atomicAdd(&offs[li + TX * (lj + TY * lk)], 1);
+      lcv[u] = (li >= 0 && li < TX && lj >= 0 && lj < TY && lk >= 0 && lk < TZ)
+                   ? li + TX * (lj + TY * lk) : -2;
     }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < CPT; u++)
+      if (lcv[u] >= 0) atomicAdd(&offs[lcv[u]], 1);
     __syncthreads();
     {  // exclusive scan of offs[0..NCL) (Hillis-Steele through fillc)
       const int v = me < NCL ? offs[me] : 0;
@@ -321,15 +348,12 @@ __global__ __launch_bounds__(NT) void k_tile(
       if (me < NCL) fillc[me] = 0;
       __syncthreads();
     }
-    for (int q = me; q < m; q += NT) {
-      const int cell = A.cell[base + c0 + q];
-      if (cell == kHole) continue;
-      const int li = (cell & mi) - T0x, lj = ((cell >> ta.bx) & mj) - T0y;
-      const int lk = D == 3 ? (cell >> shk) - T0z : 0;
-      if (li >= 0 && li < TX && lj >= 0 && lj < TY && lk >= 0 && lk < TZ) {
-        const int lc = li + TX * (lj + TY * lk);
+#pragma unroll
+    for (int u = 0; u < CPT; u++) {
+      const int q = me + u * NT, lc = lcv[u];
+      if (lc >= 0) {
         perm[offs[lc] + atomicAdd(&fillc[lc], 1)] = (unsigned short)q;
-      } else {
+      } else if (lc == -2) {
         // not in this tile (the store never does this): hand it to the
         // global-memory kernel that runs after this one
         copy_rec<D>(A, base + c0 + q, A, sc.ovf_base + atomicAdd(ovf_cur, 1));
@@ -464,14 +488,17 @@ __global__ __launch_bounds__(NT) void k_tile(
   for (int q = threadIdx.x; q < D * PVOL; q += NT) P[q] = (float)Jl[q];
 }
 
-// threads per workgroup: 512 (8 wavefronts, one thread per cell of an 8^3
-// tile); the register sums of the current need more than the 128 VGPRs a
-// 1024-thread workgroup allows
-static constexpr int kTileThreads = 512;
+// threads per workgroup: one per node-cell of the tile (8 wavefronts for an
+// 8^3 tile); the register sums of the current need more than the 128 VGPRs
+// a 1024-thread workgroup would allow
+template <int D, int TX, int TY, int TZ>
+static constexpr int tile_threads_for() {
+  return TX * TY * TZ;
+}
 
 template <int D, int TX, int TY, int TZ>
 static int lds_bytes_for() {
-  return TileShape<D, TX, TY, TZ, kTileThreads>::BYTES;
+  return TileShape<D, TX, TY, TZ, tile_threads_for<D, TX, TY, TZ>()>::BYTES;
 }
 
 template <int D, int TX, int TY, int TZ>
@@ -479,9 +506,9 @@ static hipError_t launch(const TileGeom& tg, const TileArgs& ta, const StepConst
                          const float* ebf, const RecPtr& A, const RecPtr& B, const int* ss,
                          const int* sc_, const int* cc, int* cn, int* oc, int* on, float* patch,
                          int* grow, hipStream_t s) {
-  k_tile<D, TX, TY, TZ, kTileThreads>
-      <<<tg.ntiles, kTileThreads, lds_bytes_for<D, TX, TY, TZ>(), s>>>(ta, sc, ebf, A, B, ss, sc_,
-                                                                      cc, cn, oc, on, patch, grow);
+  constexpr int NT = tile_threads_for<D, TX, TY, TZ>();
+  k_tile<D, TX, TY, TZ, NT><<<tg.ntiles, NT, lds_bytes_for<D, TX, TY, TZ>(), s>>>(
+      ta, sc, ebf, A, B, ss, sc_, cc, cn, oc, on, patch, grow);
   return hipGetLastError();
 }
 
@@ -502,7 +529,7 @@ int tile_lds_bytes(int D, int tx, int ty, int tz) {
   return lds_bytes_for<2, 32, 16, 1>();
 }
 
-int tile_threads() { return kTileThreads; }
+int tile_threads(int tx, int ty, int tz) { return tx * ty * tz; }
 
 hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepConst& sc,
                               const float* ebf, const RecPtr& A, const RecPtr& B,

@@ -14,7 +14,7 @@ namespace aladyn {
 // tile_kernel.hip
 bool tile_shape_supported(int D, int tx, int ty, int tz);
 int tile_lds_bytes(int D, int tx, int ty, int tz);
-int tile_threads();
+int tile_threads(int tx, int ty, int tz);
 hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepConst& sc,
                               const float* ebf, const RecPtr& A, const RecPtr& B,
                               const int* seg_start, const int* seg_cap, const int* cnt_cur,
@@ -58,7 +58,7 @@ int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg) {
   tg->by = bits_for(g.n2p);
   const int bz = D == 3 ? bits_for(g.n3p) : 0;
   if (tg->bx + tg->by + bz > 31) return 2;
-  tg->block = tile_threads();
+  tg->block = tile_threads(tg->T[0], tg->T[1], tg->T[2]);
   tg->sort_every = cfg.sort_every > 0 ? cfg.sort_every : 16;
   tg->lds_bytes = tile_lds_bytes(D, tg->T[0], tg->T[1], tg->T[2]);
   if (tg->lds_bytes > 160 * 1024) return 3;
@@ -310,22 +310,37 @@ __global__ void k_scatter(TileGeom tg, RecPtr U, int64_t n, RecPtr A, const int*
 
 // One-pass re-layout: counts per destination tile straight from the tile
 // segments of A (holes skipped) and its overflow list ...
+// A window shift of shx cells moves every record shx nodes to the left
+// (an exact change of the packed node index) and drops the records now at or
+// behind the box's left face, x <= xmin: node < 3, or node 3 with s <= 0
+// (the test of reset_all_part_dist, mpi_part_interface.f90:494-498).
+__device__ inline int shifted_cell(const TileGeom& tg, const RecPtr& A, int64_t src, int cell,
+                                   int shx) {
+  if (shx == 0 || cell == kHole) return cell;
+  const int i = (cell & ((1 << tg.bx) - 1)) - shx;
+  if (i < 3 || (i == 3 && !(A.s[0][src] > 0.f))) return kHole;
+  return cell - shx;
+}
+
 template <int D>
 __global__ __launch_bounds__(kB) void k_hist_seg(TileGeom tg, RecPtr A, const int* seg_start,
                                                  const int* seg_cap, const int* cnt,
                                                  int64_t ovf_base, const int* ovf_cnt,
-                                                 int* hcount) {
+                                                 int* hcount, int shx) {
   const int t = blockIdx.x;
   if (t < tg.ntiles) {
     const int n = min(cnt[t], seg_cap[t]);
     for (int q = threadIdx.x; q < n; q += kB) {
-      const int cell = A.cell[(int64_t)seg_start[t] + q];
+      const int64_t src = (int64_t)seg_start[t] + q;
+      const int cell = shifted_cell(tg, A, src, A.cell[src], shx);
       if (cell != kHole) (void)reserve_slot(tile_of(tg, cell, D), hcount);
     }
   } else {
     const int n = *ovf_cnt;
-    for (int q = (t - tg.ntiles) * kB + threadIdx.x; q < n; q += (gridDim.x - tg.ntiles) * kB)
-      (void)reserve_slot(tile_of(tg, A.cell[ovf_base + q], D), hcount);
+    for (int q = (t - tg.ntiles) * kB + threadIdx.x; q < n; q += (gridDim.x - tg.ntiles) * kB) {
+      const int cell = shifted_cell(tg, A, ovf_base + q, A.cell[ovf_base + q], shx);
+      if (cell != kHole) (void)reserve_slot(tile_of(tg, cell, D), hcount);
+    }
   }
 }
 
@@ -334,31 +349,37 @@ template <int D>
 __global__ __launch_bounds__(kB) void k_scatter_seg(TileGeom tg, RecPtr A, const int* seg_start,
                                                     const int* seg_cap, const int* cnt,
                                                     int64_t ovf_base, const int* ovf_cnt, RecPtr U,
-                                                    const int* new_start, int* fillc) {
+                                                    const int* new_start, int* fillc, int shx) {
   const int t = blockIdx.x;
   if (t < tg.ntiles) {
     const int n = min(cnt[t], seg_cap[t]);
     for (int q = threadIdx.x; q < n; q += kB) {
       const int64_t src = (int64_t)seg_start[t] + q;
-      const int cell = A.cell[src];
+      const int cell = shifted_cell(tg, A, src, A.cell[src], shx);
       if (cell == kHole) continue;
       const int d = tile_of(tg, cell, D);
-      copy_rec<D>(A, src, U, (int64_t)new_start[d] + reserve_slot(d, fillc));
+      const int64_t dst = (int64_t)new_start[d] + reserve_slot(d, fillc);
+      copy_rec<D>(A, src, U, dst);
+      U.cell[dst] = cell;
     }
   } else {
     const int n = *ovf_cnt;
     for (int q = (t - tg.ntiles) * kB + threadIdx.x; q < n; q += (gridDim.x - tg.ntiles) * kB) {
       const int64_t src = ovf_base + q;
-      const int d = tile_of(tg, A.cell[src], D);
-      copy_rec<D>(A, src, U, (int64_t)new_start[d] + reserve_slot(d, fillc));
+      const int cell = shifted_cell(tg, A, src, A.cell[src], shx);
+      if (cell == kHole) continue;
+      const int d = tile_of(tg, cell, D);
+      const int64_t dst = (int64_t)new_start[d] + reserve_slot(d, fillc);
+      copy_rec<D>(A, src, U, dst);
+      U.cell[dst] = cell;
     }
   }
 }
 
-// reference layout part(ld, 2D+1) -> records (ids = input index)
+// reference layout part(ld, 2D+1) -> records (ids = id0 + input index)
 template <int D>
 __global__ void k_convert_in(DevGrid g, int bx, int by, const double* part, int64_t ld,
-                             int64_t np, RecPtr U) {
+                             int64_t np, RecPtr U, int64_t id0 = 0) {
   const double mn[3] = {g.xmn, g.ymn, g.zmn};
   const double inv[3] = {g.dx_inv, g.dy_inv, g.dz_inv};
   for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < np;
@@ -379,7 +400,7 @@ __global__ void k_convert_in(DevGrid g, int bx, int by, const double* part, int6
     U.cell[n] = N[0] | (N[1] << bx) | (D == 3 ? (N[2] << (bx + by)) : 0);
     const unsigned long long b = (unsigned long long)__double_as_longlong(part[2 * D * ld + n]);
     U.w[n] = __uint_as_float((unsigned)(b >> 32));
-    if (U.id) U.id[n] = (int)n;
+    if (U.id) U.id[n] = (int)(id0 + n);
   }
 }
 
@@ -534,6 +555,7 @@ int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const do
   ts->charge = charge;
   ts->part_ind = pind;
   ts->cur = 0;
+  ts->next_id = np;
   double* dpart = nullptr;
   const int ncol = 2 * D + 1;
   if (np > 0) {
@@ -581,18 +603,20 @@ static int gather_live(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t
 // Fully asynchronous: the capacities fit the segment area by construction
 // (live particles never exceed the count given to put).
 template <int D>
-static int relayout_seg(TileStore* ts, const TileGeom& tg, hipStream_t s, std::string* msg) {
+static int relayout_seg(TileStore* ts, const TileGeom& tg, hipStream_t s, std::string* msg,
+                        int shx = 0) {
   const int cur = ts->cur, oth = 1 - cur;
   int* h = ts->count[oth];  // per-tile counts of the new layout
   fill<int>(h, tg.ntiles, 0, s);
   k_hist_seg<D><<<tg.ntiles + 64, kB, 0, s>>>(tg, ts->buf[cur], ts->seg_start, ts->seg_cap,
-                                              ts->count[cur], ts->ovf_base, ts->ovf_count + cur, h);
+                                              ts->count[cur], ts->ovf_base, ts->ovf_count + cur, h,
+                                              shx);
   k_scan_tiles<<<1, kB, 0, s>>>(h, tg.ntiles, ts->seg_start2, ts->seg_cap2, ts->grow,
                                 seg_budget(ts->ovf_cap, tg.ntiles));
   fill<int>(ts->scratch, tg.ntiles, 0, s);
   k_scatter_seg<D><<<tg.ntiles + 64, kB, 0, s>>>(tg, ts->buf[cur], ts->seg_start, ts->seg_cap,
                                                  ts->count[cur], ts->ovf_base, ts->ovf_count + cur,
-                                                 ts->buf[oth], ts->seg_start2, ts->scratch);
+                                                 ts->buf[oth], ts->seg_start2, ts->scratch, shx);
   fill<int>(ts->ovf_count, 2, 0, s);
   TS_CK(hipGetLastError());
   std::swap(ts->seg_start, ts->seg_start2);
@@ -607,6 +631,121 @@ int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hip
                         std::string* msg) {
   if (!ts->loaded) return 0;
   return g.ndim == 2 ? relayout_seg<2>(ts, tg, s, msg) : relayout_seg<3>(ts, tg, s, msg);
+}
+
+// Moves the box window_cells cells to the right (window.f90:276-347): every
+// record's node index drops by window_cells, records left behind the new
+// left face are dropped, and the store is re-laid out in the same pass.
+int tile_store_window(TileStore* ts, const TileGeom& tg, const DevGrid& g, int window_cells,
+                      hipStream_t s, std::string* msg) {
+  if (!ts->loaded || window_cells == 0) return 0;
+  return g.ndim == 2 ? relayout_seg<2>(ts, tg, s, msg, window_cells)
+                     : relayout_seg<3>(ts, tg, s, msg, window_cells);
+}
+
+// Re-allocates the two buffers for new_cap particles, keeping the live ones.
+template <int D>
+static int grow_store(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t new_cap,
+                      std::string* msg) {
+  int64_t n = 0;
+  if (int r = gather_live<D>(ts, tg, s, &n, msg)) return r;  // dense, in buffer 1-cur
+  const int64_t alloc = seg_budget(new_cap, tg.ntiles) + new_cap;
+  if (alloc >= (1LL << 31)) {
+    *msg = "FP32 tile path: particle count exceeds the 31-bit slot range";
+    return ALADYN_EINVAL;
+  }
+  const size_t per = (size_t)(2 + 2 * D + (ts->track_ids ? 1 : 0)) * 4;
+  void* fresh[2] = {nullptr, nullptr};
+  for (int b = 0; b < 2; b++) {
+    if (hipMalloc(&fresh[b], per * (size_t)alloc) != hipSuccess) {
+      (void)hipFree(fresh[0]);
+      *msg = "tile store: out of device memory growing the particle buffers";
+      return ALADYN_ENOMEM;
+    }
+  }
+  const RecPtr old = ts->buf[1 - ts->cur];
+  void* oldmem[2] = {ts->mem[0], ts->mem[1]};
+  ts->mem[0] = fresh[0];
+  ts->mem[1] = fresh[1];
+  ts->alloc = alloc;
+  set_ptrs(ts, 0, D);
+  set_ptrs(ts, 1, D);
+  const RecPtr& nu = ts->buf[1];
+  const size_t nb = (size_t)n * 4;
+  if (n > 0) {
+    TS_CK(hipMemcpyAsync(nu.cell, old.cell, nb, hipMemcpyDeviceToDevice, s));
+    for (int d = 0; d < D; d++) {
+      TS_CK(hipMemcpyAsync(nu.s[d], old.s[d], nb, hipMemcpyDeviceToDevice, s));
+      TS_CK(hipMemcpyAsync(nu.p[d], old.p[d], nb, hipMemcpyDeviceToDevice, s));
+    }
+    TS_CK(hipMemcpyAsync(nu.w, old.w, nb, hipMemcpyDeviceToDevice, s));
+    if (nu.id) TS_CK(hipMemcpyAsync(nu.id, old.id, nb, hipMemcpyDeviceToDevice, s));
+  }
+  TS_CK(hipStreamSynchronize(s));
+  (void)hipFree(oldmem[0]);
+  (void)hipFree(oldmem[1]);
+  ts->ovf_cap = new_cap;
+  ts->ovf_base = alloc - new_cap;
+  ts->cur = 0;
+  return sort_into_cur<D>(ts, tg, n, s, msg);
+}
+
+// Appends n particles (reference layout, inside the box) to the species:
+// they are converted into the overflow list of the current buffer and the
+// store is re-laid out, so the next step moves them in their tiles
+// (add_particles / particles_inject, window.f90:39-270).
+template <int D>
+static int inject_impl(TileStore* ts, const TileGeom& tg, const DevGrid& g, const double* part,
+                       int64_t ld, int64_t n, hipStream_t s, std::string* msg) {
+  for (int64_t q = 0; q < n; q++) {
+    unsigned long long b;
+    std::memcpy(&b, part + 2 * D * ld + q, 8);
+    const int c = (int)(short)(b & 0xffff), pi = (int)(short)((b >> 16) & 0xffff);
+    if (c != ts->charge || pi != ts->part_ind) {
+      *msg = "FP32 tile path: injected particles need the species' charge and part_ind";
+      return ALADYN_EINVAL;
+    }
+  }
+  int64_t live = 0;
+  if (int r = tile_store_count(ts, tg, s, &live, msg)) return r;
+  if (live + n > ts->ovf_cap)
+    if (int r = grow_store<D>(ts, tg, s, std::max(2 * ts->ovf_cap, live + n), msg)) return r;
+  const int cur = ts->cur;
+  int ovf = 0;
+  TS_CK(hipMemcpyAsync(&ovf, ts->ovf_count + cur, sizeof(int), hipMemcpyDeviceToHost, s));
+  TS_CK(hipStreamSynchronize(s));
+  // overflow records are live ones, so ovf + n <= live + n <= ovf_cap
+  const int ncol = 2 * D + 1;
+  double* dpart = nullptr;
+  TS_CK(hipMalloc(&dpart, (size_t)n * ncol * sizeof(double)));
+  for (int c = 0; c < ncol; c++)
+    TS_CK(hipMemcpyAsync(dpart + c * n, part + c * ld, (size_t)n * sizeof(double),
+                         hipMemcpyHostToDevice, s));
+  RecPtr dst = ts->buf[cur];
+  const int64_t o = ts->ovf_base + ovf;
+  dst.cell += o;
+  for (int d = 0; d < 3; d++) {
+    if (dst.s[d]) dst.s[d] += o;
+    if (dst.p[d]) dst.p[d] += o;
+  }
+  dst.w += o;
+  if (dst.id) dst.id += o;
+  k_convert_in<D><<<grid_for(n), kB, 0, s>>>(g, tg.bx, tg.by, dpart, n, n, dst, ts->next_id);
+  TS_CK(hipGetLastError());
+  ts->next_id += n;
+  const int total = ovf + (int)n;
+  TS_CK(hipMemcpyAsync(ts->ovf_count + cur, &total, sizeof(int), hipMemcpyHostToDevice, s));
+  TS_CK(hipStreamSynchronize(s));
+  (void)hipFree(dpart);
+  return relayout_seg<D>(ts, tg, s, msg);
+}
+
+int tile_store_inject(TileStore* ts, const TileGeom& tg, const DevGrid& g, const double* part,
+                      int64_t ld, int64_t n, hipStream_t s, bool track_ids, std::string* msg) {
+  if (n <= 0) return 0;
+  if (!ts->loaded) return tile_store_put(ts, tg, g, part, ld, n, s, track_ids, msg);
+  return g.ndim == 2 ? inject_impl<2>(ts, tg, g, part, ld, n, s, msg)
+                     : inject_impl<3>(ts, tg, g, part, ld, n, s, msg);
 }
 
 int tile_store_count(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t* np,

@@ -67,7 +67,8 @@ struct TileStore {
   int charge = 0, part_ind = 0;
   bool track_ids = false;
   bool loaded = false;
-  int64_t np = 0;  // particles at the last put / re-layout
+  int64_t np = 0;       // particles at the last put / re-layout
+  int64_t next_id = 0;  // id of the next injected particle (ids tracked)
   int steps_since_sort = 0;
   int* grow = nullptr;   // ntiles flags: the tile spilled since the last re-layout
   // overflow count of recent steps, read back through two pinned slots: the
@@ -85,6 +86,10 @@ int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const do
                    int64_t ld, int64_t np, hipStream_t s, bool track_ids, std::string* msg);
 int tile_store_count(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t* np,
                      std::string* msg);
+int tile_store_window(TileStore* ts, const TileGeom& tg, const DevGrid& g, int window_cells,
+                      hipStream_t s, std::string* msg);
+int tile_store_inject(TileStore* ts, const TileGeom& tg, const DevGrid& g, const double* part,
+                      int64_t ld, int64_t n, hipStream_t s, bool track_ids, std::string* msg);
 int tile_store_get(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* part, int64_t ld,
                    hipStream_t s, std::string* msg);
 // One species-loop body: gather + push + box test + deposit + re-binning.

@@ -46,6 +46,7 @@ module aladyn_gpu
   public :: aladyn_put_fields, aladyn_get_fields, aladyn_get_current
   public :: aladyn_lpf2_evolve, aladyn_pfields_prepare, aladyn_curr_clean
   public :: aladyn_particle_stage, aladyn_curr_mpi_collect, aladyn_advance_lpf_fields
+  public :: aladyn_lp_window_xshift, aladyn_inject_species, aladyn_box
   public :: aladyn_energy_momenta, aladyn_field_energy, aladyn_grid_charge
   public :: aladyn_continuity_residual, aladyn_set_diagnostics
   public :: aladyn_synchronize, aladyn_kernel_timing
@@ -145,6 +146,30 @@ module aladyn_gpu
         bind(C, name="aladyn_advance_lpf_fields")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function aladyn_lp_window_xshift(ctx, witr, init_iter, w_speed, nshx) &
+        bind(C, name="aladyn_lp_window_xshift")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: witr, init_iter
+      real(c_double), value :: w_speed
+      integer(c_int32_t), intent(out) :: nshx
+    end function
+
+    integer(c_int) function aladyn_inject_species(ctx, ic, part, ld, np) &
+        bind(C, name="aladyn_inject_species")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ic
+      real(c_double), intent(in) :: part(*)
+      integer(c_int64_t), value :: ld, np
+    end function
+
+    integer(c_int) function aladyn_box(ctx, out) bind(C, name="aladyn_box")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(6)
     end function
 
     integer(c_int) function aladyn_energy_momenta(ctx, ic, ek, ekmax) &

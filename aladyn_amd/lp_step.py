@@ -172,6 +172,24 @@ class LPStep:
         self._ck(self.lib.aladyn_continuity_residual(self._h, _dptr(out)))
         return float(out[0]), float(out[1])
 
+    def lp_window_xshift(self, witr: int, init_iter: int, w_speed: float) -> int:
+        """lp_window_xshift (window.f90:276-347): moves the box; returns nshx."""
+        n = C.c_int32()
+        self._ck(self.lib.aladyn_lp_window_xshift(self._h, int(witr), int(init_iter),
+                                                  float(w_speed), C.byref(n)))
+        return int(n.value)
+
+    def inject_species(self, ic: int, part: np.ndarray):
+        """Appends particles (add_particles, window.f90:39-111)."""
+        p = np.asfortranarray(part, dtype=np.float64)
+        self._ck(self.lib.aladyn_inject_species(self._h, ic, _dptr(p), p.shape[0], p.shape[0]))
+
+    def box(self) -> np.ndarray:
+        """The current particle box (xmin, xmax, ymin, ymax, zmin, zmax)."""
+        out = np.zeros(6)
+        self._ck(self.lib.aladyn_box(self._h, _dptr(out)))
+        return out
+
     def kernel_timing(self, reset: bool = False):
         pm = np.zeros(1)
         fm = np.zeros(1)

@@ -15,7 +15,15 @@ timed steps; `cpu_baseline` times the reference's own Fortran gather + push +
 deposit (oracle/_ref, built from the reference sources) on one host core over
 a bounded particle sample.
 
+`roofline.traffic` is the particle kernel's measured HBM bytes per launch
+(rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a plain
+device copy of known size), read from the committed profile summary
+profiles/k_tile_3d_lwfa_256x128x128.md when it covers this workload; null
+otherwise.  `--calibrate-copy` runs only that copy (1 GiB read + 1 GiB
+written) for the calibration pass.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu]
+       python bench.py --calibrate-copy
 """
 from __future__ import annotations
 
@@ -31,6 +39,32 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "particle-steps/s (gather+Boris+Esirkepov), FP32 vs FP64 on 1 GPU; % HBM peak"
 HBM_PEAK_GBS = 8000.0
+PROFILE = ROOT / "profiles" / "k_tile_3d_lwfa_256x128x128.md"
+
+
+def measured_traffic(nx, ny, nz):
+    """HBM bytes per particle-kernel launch from the committed profile summary
+    (a line `traffic_bytes_per_launch: <value>` for grid `<nx>x<ny>x<nz>`)."""
+    if not PROFILE.exists() or (nx, ny, nz) != (256, 128, 128):
+        return None
+    for line in PROFILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def calibrate_copy():
+    """One device-to-device copy of 1 GiB (the reference kernel for the HBM
+    byte counters): 1 GiB read, 1 GiB written."""
+    import torch
+
+    n = (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": n * 4}), flush=True)
 
 
 def log(*a):
@@ -99,7 +133,11 @@ def main():
     ap.add_argument("--nx", type=int, default=256)
     ap.add_argument("--ny", type=int, default=128)
     ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the step would shard by spatial domain; see DESIGN.md)", file=sys.stderr)
@@ -161,7 +199,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(cfg.nx, cfg.ny, cfg.nz),
             "kernel": "k_tile_particles<3>",
             "kernel_ms": r32["part_ms"],
             "algorithmic_bytes_per_launch": kern_bytes,

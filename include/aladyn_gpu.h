@@ -116,6 +116,30 @@ int aladyn_curr_mpi_collect(aladyn_ctx *ctx);
 /* advance_lpf_fields(ebf, jc, 0) (curr_and_fields_util.f90:176-260). */
 int aladyn_advance_lpf_fields(aladyn_ctx *ctx);
 
+/* lp_window_xshift(witr, init_iter) (window.f90:276-347) with w_speed, on
+ * the device copy.  init_iter == 0 resets the saved window state (xlapse,
+ * wi1) and does nothing else (window.f90:284-288).  Otherwise
+ * xlapse += w_speed*dt*witr and the box moves by
+ * *nshx = nint(xlapse/dx) - wi1 cells: the field interior shifts left by nshx
+ * with zeros entering on the right (fields_left_xshift,
+ * mpi_field_interface.f90:299-359), the box limits move by dx*nshx, and
+ * particles left behind the new left face are removed
+ * (cell_part_dist(.true.), mpi_part_interface.f90:525-596).  The caller then
+ * adds the plasma entering on the right with aladyn_inject_species, as
+ * particles_inject does from the caller's own target tables
+ * (window.f90:113-270).  ALADYN_ERANGE if nshx reaches the box length (the
+ * reference's ier = 2). */
+int aladyn_lp_window_xshift(aladyn_ctx *ctx, int32_t witr, int32_t init_iter,
+                            double w_speed, int32_t *nshx);
+/* Appends np particles part(ld, 2*curr_ndim+1) to species ic, in the current
+ * box frame (add_particles, window.f90:39-111); positions must lie in the
+ * box.  They are moved from the next step on. */
+int aladyn_inject_species(aladyn_ctx *ctx, int32_t ic, const double *part,
+                          int64_t ld, int64_t np);
+/* The current particle box: out = {xmin, xmax, ymin, ymax, zmin, zmax}
+ * (loc_*grid%gmin/gmax; x moves with the window). */
+int aladyn_box(aladyn_ctx *ctx, double out[6]);
+
 /* energy_momenta for species ic (diag_part_and_fields.f90:172-219), FP64
  * device reduction: ek[0..6] = ek(1:7), *ekmax = max(gamma-1). */
 int aladyn_energy_momenta(aladyn_ctx *ctx, int32_t ic, double ek[7],

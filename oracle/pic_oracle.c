@@ -1075,3 +1075,45 @@ void orc_field_energy(const orc_grid *g, const double *ebf, double out[6]) {
     out[ik - 1] = dgvol * s;
   }
 }
+
+int orc_window_xshift(orc_grid *g, orc_window *w, int witr, int init_iter,
+                      double w_speed, double *ebf, int64_t *np, double *part,
+                      int64_t ld) {
+  /* lp_window_xshift (window.f90:276-347), one rank, no fluid/envelope */
+  if (init_iter == 0) {
+    w->xlapse = 0.0;
+    w->wi1 = 0;
+    return 0;
+  }
+  w->xlapse = w->xlapse + w_speed * g->dt * witr;
+  const int wi2 = (int)lround(g->dx_inv * w->xlapse); /* nint */
+  const int nshx = wi2 - w->wi1;
+  w->wi1 = wi2;
+  g->xmn = g->xmn + g->dx * nshx;
+  g->xmx = g->xmx + g->dx * nshx;
+  const int i1 = g->ix1, i2 = g->ix2 - nshx;
+  if (i2 <= 0) return -1;
+  if (nshx > 0) {
+    /* fields_left_xshift(ebf, i1, i2, 1, nfield, nshx) */
+    for (int ic = 1; ic <= g->nfield; ic++)
+      for (int iz = g->kz1; iz <= g->kz2; iz++)
+        for (int iy = g->jy1; iy <= g->jy2; iy++) {
+          for (int ix = i1; ix <= i2; ix++)
+            F4(ebf, ix, iy, iz, ic) = F4(ebf, ix + nshx, iy, iz, ic);
+          for (int ix = i2 + 1; ix <= i2 + nshx; ix++) F4(ebf, ix, iy, iz, ic) = 0.0;
+        }
+  }
+  /* cell_part_dist(.true.): x only, (xmn, xmx], stable */
+  const int ndv = 2 * g->curr_ndim + 1;
+  int64_t p = 0;
+  for (int64_t n = 0; n < *np; n++) {
+    const double xp = PC(part, n, 1);
+    if (xp > g->xmn && xp <= g->xmx) {
+      if (p != n)
+        for (int c = 1; c <= ndv; c++) PC(part, p, c) = PC(part, n, c);
+      p++;
+    }
+  }
+  *np = p;
+  return nshx;
+}

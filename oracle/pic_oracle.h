@@ -128,6 +128,26 @@ void orc_energy_momenta(const orc_grid *g, int64_t np, const double *part,
  * out[c] = dgvol * sum over ix1..ix2, jy1..nyp, kz1..nzp of ebf(c)^2. */
 void orc_field_energy(const orc_grid *g, const double *ebf, double out[6]);
 
+/* Saved state of lp_window_xshift (window.f90:276-288): xlapse, wi1. */
+typedef struct orc_window {
+  double xlapse;
+  int wi1;
+} orc_window;
+
+/* lp_window_xshift(witr, init_iter) on one rank (window.f90:276-347) with
+ * w_speed: init_iter == 0 resets the state and returns 0.  Otherwise the box
+ * moves by nshx = nint(xlapse/dx) - wi1 cells: the field interior
+ * ix1..ix2-nshx takes the values nshx cells to its right and the last nshx
+ * columns are zeroed (fields_left_xshift, mpi_field_interface.f90:299-359,
+ * i1 = ix1, i2 = ix2 - nshx, rows jy1..jy2, planes kz1..kz2, all nfield
+ * components); xmn, xmx move by dx*nshx; particles with x <= xmn or x > xmx
+ * are removed keeping the order (cell_part_dist(.true.) ->
+ * reset_all_part_dist on x, mpi_part_interface.f90:467-523).  Returns nshx,
+ * or -1 if nshx >= ix2 (the reference's ier = 2); *np is updated. */
+int orc_window_xshift(orc_grid *g, orc_window *w, int witr, int init_iter,
+                      double w_speed, double *ebf, int64_t *np, double *part,
+                      int64_t ld);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,12 @@ def _f(a):
     return a
 
 
+class OrcWindow(C.Structure):
+    """Saved state of lp_window_xshift (window.f90:276-288)."""
+
+    _fields_ = [("xlapse", C.c_double), ("wi1", C.c_int)]
+
+
 class Oracle:
     """The C restatement of the LP step on the reference layouts."""
 
@@ -82,12 +88,16 @@ class Oracle:
             "orc_grid_charge": [_G, _I64, _D, _I64, _D],
             "orc_energy_momenta": [_G, _I64, _D, _I64, _D, _D],
             "orc_field_energy": [_G, _D, _D],
+            "orc_window_xshift": [_G, C.POINTER(OrcWindow), C.c_int, C.c_int, C.c_double, _D,
+                                  C.POINTER(C.c_int64), _D, _I64],
         }
         for name, at in sig.items():
             fn = getattr(L, name)
             fn.argtypes = at
             fn.restype = _I64 if name in ("orc_part_dist", "orc_particle_stage", "orc_lpf2_evolve",
                                           "orc_esirkepov_skipped") else None
+            if name == "orc_window_xshift":
+                fn.restype = C.c_int
         self.L = L
         self.cfg = cfg
         g = OrcGrid()
@@ -176,6 +186,14 @@ class Oracle:
                                    _p(ebfp), _p(_f(ebf)), _p(_f(jc)), 1 if initial_time else 0)
         return np.asfortranarray(part[:n])
 
+    def window_xshift(self, win, witr, init_iter, w_speed, ebf, part):
+        """lp_window_xshift: returns (nshx, surviving particles as a new array);
+        moves this oracle's box (xmn, xmx) with the window."""
+        n = C.c_int64(part.shape[0])
+        nshx = self.L.orc_window_xshift(C.byref(self.g), C.byref(win), witr, init_iter, w_speed,
+                                        _p(_f(ebf)), C.byref(n), _p(_f(part)), self._ld(part))
+        return int(nshx), np.asfortranarray(part[:n.value])
+
     def grid_charge(self, part):
         rho = np.zeros((self.cfg.n1p, self.cfg.n2p, self.cfg.n3p), dtype=np.float64, order="F")
         self.L.orc_grid_charge(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part), _p(rho))
@@ -225,6 +243,14 @@ class RefHarness:
 
     def esirkepov(self, part, ebfp, jc):
         self.L.ref_esirkepov(part.shape[0], _p(_f(part)), max(part.shape[0], 1), _p(ebfp), _p(_f(jc)))
+
+    def window_xshift(self, win, witr, init_iter, w_speed, ebf, part):
+        """lp_window_xshift: returns (nshx, surviving particles as a new array);
+        moves this oracle's box (xmn, xmx) with the window."""
+        n = C.c_int64(part.shape[0])
+        nshx = self.L.orc_window_xshift(C.byref(self.g), C.byref(win), witr, init_iter, w_speed,
+                                        _p(_f(ebf)), C.byref(n), _p(_f(part)), self._ld(part))
+        return int(nshx), np.asfortranarray(part[:n.value])
 
     def grid_charge(self, part):
         rho = np.zeros((self.cfg.n1p, self.cfg.n2p, self.cfg.n3p), order="F")
