@@ -29,32 +29,34 @@ struct StepConst {
   int64_t ovf_base;
 };
 
-template <int D>
-__device__ inline void load_rec(const RecPtr& r, int64_t n, Rec& q, int bx, int by) {
-  const int cell = r.cell[n];
-  q.N[0] = cell & ((1 << bx) - 1);
-  q.N[1] = (cell >> bx) & ((1 << by) - 1);
-  q.N[2] = D == 3 ? (cell >> (bx + by)) : 1;
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    q.s[d] = r.s[d][n];
-    q.p[d] = r.p[d][n];
-  }
-  q.w = r.w[n];
-  q.id = r.id ? r.id[n] : 0;
+// accessors of one field of a record
+__device__ inline int rec_cell(const RecPtr& r, int64_t n) {
+  return __float_as_int(rec_at(r, n)[0].x);
 }
+__device__ inline void set_rec_cell(const RecPtr& r, int64_t n, int cell) {
+  reinterpret_cast<int*>(rec_at(r, n))[0] = cell;
+}
+__device__ inline float rec_s(const RecPtr& r, int64_t n, int d) {
+  return reinterpret_cast<const float*>(rec_at(r, n))[1 + d];
+}
+__device__ inline float rec_p(const RecPtr& r, int64_t n, int d) {
+  return reinterpret_cast<const float*>(rec_at(r, n))[4 + d];
+}
+__device__ inline float rec_w(const RecPtr& r, int64_t n) { return rec_at(r, n)[1].w; }
 
-// The same record with the packed cell left undecoded, so that a load can be
-// issued one iteration ahead of its use (the registers hold raw values).
+// The record with the packed cell left undecoded (q.N[0] holds the bits), so
+// that a load can be issued one iteration ahead of its use.
 template <int D>
 __device__ inline void load_raw(const RecPtr& r, int64_t n, Rec& q) {
-  q.N[0] = r.cell[n];
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    q.s[d] = r.s[d][n];
-    q.p[d] = r.p[d][n];
-  }
-  q.w = r.w[n];
+  const float4 a = rec_at(r, n)[0], b = rec_at(r, n)[1];
+  q.N[0] = __float_as_int(a.x);
+  q.s[0] = a.y;
+  q.s[1] = a.z;
+  q.s[2] = a.w;
+  q.p[0] = b.x;
+  q.p[1] = b.y;
+  q.p[2] = b.z;
+  q.w = b.w;
   q.id = r.id ? r.id[n] : 0;
 }
 
@@ -67,26 +69,25 @@ __device__ inline void decode_cell(Rec& q, int bx, int by) {
 }
 
 template <int D>
+__device__ inline void load_rec(const RecPtr& r, int64_t n, Rec& q, int bx, int by) {
+  load_raw<D>(r, n, q);
+  decode_cell<D>(q, bx, by);
+}
+
+template <int D>
 __device__ inline void store_rec(const RecPtr& r, int64_t n, const Rec& q, int bx, int by) {
-  r.cell[n] = q.N[0] | (q.N[1] << bx) | (D == 3 ? (q.N[2] << (bx + by)) : 0);
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    r.s[d][n] = q.s[d];
-    r.p[d][n] = q.p[d];
-  }
-  r.w[n] = q.w;
+  const int cell = q.N[0] | (q.N[1] << bx) | (D == 3 ? (q.N[2] << (bx + by)) : 0);
+  float4* o = rec_at(r, n);
+  o[0] = make_float4(__int_as_float(cell), q.s[0], q.s[1], D == 3 ? q.s[2] : 0.f);
+  o[1] = make_float4(q.p[0], q.p[1], D == 3 ? q.p[2] : 0.f, q.w);
   if (r.id) r.id[n] = q.id;
 }
 
 template <int D>
 __device__ inline void copy_rec(const RecPtr& a, int64_t i, const RecPtr& b, int64_t o) {
-  b.cell[o] = a.cell[i];
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    b.s[d][o] = a.s[d][i];
-    b.p[d][o] = a.p[d][i];
-  }
-  b.w[o] = a.w[i];
+  const float4 x = rec_at(a, i)[0], y = rec_at(a, i)[1];
+  rec_at(b, o)[0] = x;
+  rec_at(b, o)[1] = y;
   if (a.id && b.id) b.id[o] = a.id[i];
 }
 

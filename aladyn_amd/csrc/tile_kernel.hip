@@ -308,7 +308,7 @@ __global__ __launch_bounds__(NT) void k_tile(
 #pragma unroll
     for (int u = 0; u < CPT; u++) {
       const int q = me + u * NT;
-      lcv[u] = q < m ? A.cell[base + c0 + q] : kHole;
+      lcv[u] = q < m ? rec_cell(A, base + c0 + q) : kHole;
     }
     // local cell index, kHole for a hole, -2 for a record of another tile
 #pragma unroll
@@ -472,7 +472,7 @@ __global__ __launch_bounds__(NT) void k_tile(
       const int nstay = misc[1], nval = offs[NCL];
       for (int q = nstay + me; q < nval; q += NT) {
         const int slot = chunk_base + q;
-        if (slot < cap_t) B.cell[base + slot] = kHole;
+        if (slot < cap_t) set_rec_cell(B, base + slot, kHole);
       }
     }
   }

@@ -40,7 +40,9 @@ const char* tile_geom_error(int code) {
 int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg) {
   std::memset(tg, 0, sizeof *tg);
   const int D = g.ndim;
-  const int def[3] = {D == 3 ? 8 : 16, D == 3 ? 8 : 16, D == 3 ? 8 : 1};
+  // 8x8x4 in 3-D: 256 threads and 68 KB of LDS, so two tiles share a CU and
+  // one's staging latency hides behind the other's particles
+  const int def[3] = {D == 3 ? 8 : 16, D == 3 ? 8 : 16, D == 3 ? 4 : 1};
   for (int d = 0; d < 3; d++) tg->T[d] = cfg.tile[d] > 0 ? cfg.tile[d] : def[d];
   if (D == 2) tg->T[2] = 1;
   if (!tile_shape_supported(D, tg->T[0], tg->T[1], tg->T[2])) return 1;
@@ -257,7 +259,7 @@ __global__ __launch_bounds__(kB) void k_count_live(RecPtr A, const int* seg_star
   }
   const int n = min(cnt[t], seg_cap[t]);
   int c = 0;
-  for (int q = threadIdx.x; q < n; q += kB) c += A.cell[(int64_t)seg_start[t] + q] != kHole;
+  for (int q = threadIdx.x; q < n; q += kB) c += rec_cell(A, (int64_t)seg_start[t] + q) != kHole;
   int total;
   block_scan_excl(c, sh, total);
   if (threadIdx.x == 0) live[t] = total;
@@ -276,7 +278,7 @@ __global__ __launch_bounds__(kB) void k_gather_live(RecPtr A, RecPtr U, const in
     int dst = off[t];
     for (int q0 = 0; q0 < n; q0 += kB) {
       const int q = q0 + threadIdx.x;
-      const bool ok = q < n && A.cell[(int64_t)seg_start[t] + q] != kHole;
+      const bool ok = q < n && rec_cell(A, (int64_t)seg_start[t] + q) != kHole;
       int total;
       const int ex = block_scan_excl(ok ? 1 : 0, sh, total);
       if (ok) copy_rec<D>(A, (int64_t)seg_start[t] + q, U, (int64_t)dst + ex);
@@ -294,7 +296,7 @@ template <int D>
 __global__ void k_hist(TileGeom tg, RecPtr U, int64_t n, int* hcount) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x)
-    (void)reserve_slot(tile_of(tg, U.cell[q], D), hcount);
+    (void)reserve_slot(tile_of(tg, rec_cell(U, q), D), hcount);
 }
 
 template <int D>
@@ -302,7 +304,7 @@ __global__ void k_scatter(TileGeom tg, RecPtr U, int64_t n, RecPtr A, const int*
                           int* fillc) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
-    const int t = tile_of(tg, U.cell[q], D);
+    const int t = tile_of(tg, rec_cell(U, q), D);
     const int slot = reserve_slot(t, fillc);
     copy_rec<D>(U, q, A, (int64_t)seg_start[t] + slot);
   }
@@ -318,7 +320,7 @@ __device__ inline int shifted_cell(const TileGeom& tg, const RecPtr& A, int64_t 
                                    int shx) {
   if (shx == 0 || cell == kHole) return cell;
   const int i = (cell & ((1 << tg.bx) - 1)) - shx;
-  if (i < 3 || (i == 3 && !(A.s[0][src] > 0.f))) return kHole;
+  if (i < 3 || (i == 3 && !(rec_s(A, src, 0) > 0.f))) return kHole;
   return cell - shx;
 }
 
@@ -332,13 +334,13 @@ __global__ __launch_bounds__(kB) void k_hist_seg(TileGeom tg, RecPtr A, const in
     const int n = min(cnt[t], seg_cap[t]);
     for (int q = threadIdx.x; q < n; q += kB) {
       const int64_t src = (int64_t)seg_start[t] + q;
-      const int cell = shifted_cell(tg, A, src, A.cell[src], shx);
+      const int cell = shifted_cell(tg, A, src, rec_cell(A, src), shx);
       if (cell != kHole) (void)reserve_slot(tile_of(tg, cell, D), hcount);
     }
   } else {
     const int n = *ovf_cnt;
     for (int q = (t - tg.ntiles) * kB + threadIdx.x; q < n; q += (gridDim.x - tg.ntiles) * kB) {
-      const int cell = shifted_cell(tg, A, ovf_base + q, A.cell[ovf_base + q], shx);
+      const int cell = shifted_cell(tg, A, ovf_base + q, rec_cell(A, ovf_base + q), shx);
       if (cell != kHole) (void)reserve_slot(tile_of(tg, cell, D), hcount);
     }
   }
@@ -355,23 +357,23 @@ __global__ __launch_bounds__(kB) void k_scatter_seg(TileGeom tg, RecPtr A, const
     const int n = min(cnt[t], seg_cap[t]);
     for (int q = threadIdx.x; q < n; q += kB) {
       const int64_t src = (int64_t)seg_start[t] + q;
-      const int cell = shifted_cell(tg, A, src, A.cell[src], shx);
+      const int cell = shifted_cell(tg, A, src, rec_cell(A, src), shx);
       if (cell == kHole) continue;
       const int d = tile_of(tg, cell, D);
       const int64_t dst = (int64_t)new_start[d] + reserve_slot(d, fillc);
       copy_rec<D>(A, src, U, dst);
-      U.cell[dst] = cell;
+      set_rec_cell(U, dst, cell);
     }
   } else {
     const int n = *ovf_cnt;
     for (int q = (t - tg.ntiles) * kB + threadIdx.x; q < n; q += (gridDim.x - tg.ntiles) * kB) {
       const int64_t src = ovf_base + q;
-      const int cell = shifted_cell(tg, A, src, A.cell[src], shx);
+      const int cell = shifted_cell(tg, A, src, rec_cell(A, src), shx);
       if (cell == kHole) continue;
       const int d = tile_of(tg, cell, D);
       const int64_t dst = (int64_t)new_start[d] + reserve_slot(d, fillc);
       copy_rec<D>(A, src, U, dst);
-      U.cell[dst] = cell;
+      set_rec_cell(U, dst, cell);
     }
   }
 }
@@ -385,6 +387,7 @@ __global__ void k_convert_in(DevGrid g, int bx, int by, const double* part, int6
   for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < np;
        n += (int64_t)gridDim.x * blockDim.x) {
     int N[3] = {1, 1, 1};
+    float sv[3] = {0.f, 0.f, 0.f}, pv[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int d = 0; d < D; d++) {
       double s;
@@ -394,12 +397,14 @@ __global__ void k_convert_in(DevGrid g, int bx, int by, const double* part, int6
         N[d] += 1;
         sf -= 1.f;
       }
-      U.s[d][n] = sf;
-      U.p[d][n] = (float)part[(D + d) * ld + n];
+      sv[d] = sf;
+      pv[d] = (float)part[(D + d) * ld + n];
     }
-    U.cell[n] = N[0] | (N[1] << bx) | (D == 3 ? (N[2] << (bx + by)) : 0);
+    const int cell = N[0] | (N[1] << bx) | (D == 3 ? (N[2] << (bx + by)) : 0);
     const unsigned long long b = (unsigned long long)__double_as_longlong(part[2 * D * ld + n]);
-    U.w[n] = __uint_as_float((unsigned)(b >> 32));
+    float4* o = rec_at(U, n);
+    o[0] = make_float4(__int_as_float(cell), sv[0], sv[1], sv[2]);
+    o[1] = make_float4(pv[0], pv[1], pv[2], __uint_as_float((unsigned)(b >> 32)));
     if (U.id) U.id[n] = (int)(id0 + n);
   }
 }
@@ -412,36 +417,27 @@ __global__ void k_convert_out(DevGrid g, int bx, int by, RecPtr U, int64_t n, do
   const double dd[3] = {g.dx, g.dy, g.dz};
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
-    const int cell = U.cell[q];
+    const int cell = rec_cell(U, q);
     const int N[3] = {cell & ((1 << bx) - 1), (cell >> bx) & ((1 << by) - 1),
                       D == 3 ? (cell >> (bx + by)) : 1};
 #pragma unroll
     for (int d = 0; d < D; d++) {
-      out[d * ld + q] = mn[d] + ((double)(N[d] - 3) + (double)U.s[d][q]) * dd[d];
-      out[(D + d) * ld + q] = (double)U.p[d][q];
+      out[d * ld + q] = mn[d] + ((double)(N[d] - 3) + (double)rec_s(U, q, d)) * dd[d];
+      out[(D + d) * ld + q] = (double)rec_p(U, q, d);
     }
-    out[2 * D * ld + q] = (double)U.w[q];
+    out[2 * D * ld + q] = (double)rec_w(U, q);
   }
 }
 
-static void set_ptrs(TileStore* ts, int b, int D) {
-  // layout of one buffer: cell | s[D] | p[D] | w | id
+// bytes of one record slot: the 32-byte record plus the tracked id
+static size_t rec_bytes(bool track_ids) { return 32 + (track_ids ? 4 : 0); }
+
+static void set_ptrs(TileStore* ts, int b, int /*D*/) {
+  // layout of one buffer: alloc records of 32 bytes, then alloc ids
   char* m = (char*)ts->mem[b];
-  const int64_t n = ts->alloc;
   RecPtr& r = ts->buf[b];
-  r.cell = (int*)m;
-  m += n * 4;
-  for (int d = 0; d < 3; d++) {
-    r.s[d] = d < D ? (float*)m : nullptr;
-    if (d < D) m += n * 4;
-  }
-  for (int d = 0; d < 3; d++) {
-    r.p[d] = d < D ? (float*)m : nullptr;
-    if (d < D) m += n * 4;
-  }
-  r.w = (float*)m;
-  m += n * 4;
-  r.id = ts->track_ids ? (int*)m : nullptr;
+  r.r = (float4*)m;
+  r.id = ts->track_ids ? (int*)(m + 32 * ts->alloc) : nullptr;
 }
 
 #define TS_CK(call)                                                       \
@@ -531,7 +527,7 @@ int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const do
     tile_store_free(ts);
     ts->track_ids = track_ids;
     ts->alloc = alloc;
-    const size_t per = (size_t)(2 + 2 * D + (track_ids ? 1 : 0)) * 4;
+    const size_t per = rec_bytes(track_ids);
     for (int b = 0; b < 2; b++) {
       TS_CK(hipMalloc(&ts->mem[b], per * (size_t)alloc));
       set_ptrs(ts, b, D);
@@ -654,7 +650,7 @@ static int grow_store(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t 
     *msg = "FP32 tile path: particle count exceeds the 31-bit slot range";
     return ALADYN_EINVAL;
   }
-  const size_t per = (size_t)(2 + 2 * D + (ts->track_ids ? 1 : 0)) * 4;
+  const size_t per = rec_bytes(ts->track_ids);
   void* fresh[2] = {nullptr, nullptr};
   for (int b = 0; b < 2; b++) {
     if (hipMalloc(&fresh[b], per * (size_t)alloc) != hipSuccess) {
@@ -671,15 +667,9 @@ static int grow_store(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t 
   set_ptrs(ts, 0, D);
   set_ptrs(ts, 1, D);
   const RecPtr& nu = ts->buf[1];
-  const size_t nb = (size_t)n * 4;
   if (n > 0) {
-    TS_CK(hipMemcpyAsync(nu.cell, old.cell, nb, hipMemcpyDeviceToDevice, s));
-    for (int d = 0; d < D; d++) {
-      TS_CK(hipMemcpyAsync(nu.s[d], old.s[d], nb, hipMemcpyDeviceToDevice, s));
-      TS_CK(hipMemcpyAsync(nu.p[d], old.p[d], nb, hipMemcpyDeviceToDevice, s));
-    }
-    TS_CK(hipMemcpyAsync(nu.w, old.w, nb, hipMemcpyDeviceToDevice, s));
-    if (nu.id) TS_CK(hipMemcpyAsync(nu.id, old.id, nb, hipMemcpyDeviceToDevice, s));
+    TS_CK(hipMemcpyAsync(nu.r, old.r, (size_t)n * 32, hipMemcpyDeviceToDevice, s));
+    if (nu.id) TS_CK(hipMemcpyAsync(nu.id, old.id, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
   }
   TS_CK(hipStreamSynchronize(s));
   (void)hipFree(oldmem[0]);
@@ -723,12 +713,7 @@ static int inject_impl(TileStore* ts, const TileGeom& tg, const DevGrid& g, cons
                          hipMemcpyHostToDevice, s));
   RecPtr dst = ts->buf[cur];
   const int64_t o = ts->ovf_base + ovf;
-  dst.cell += o;
-  for (int d = 0; d < 3; d++) {
-    if (dst.s[d]) dst.s[d] += o;
-    if (dst.p[d]) dst.p[d] += o;
-  }
-  dst.w += o;
+  dst.r += 2 * o;
   if (dst.id) dst.id += o;
   k_convert_in<D><<<grid_for(n), kB, 0, s>>>(g, tg.bx, tg.by, dpart, n, n, dst, ts->next_id);
   TS_CK(hipGetLastError());
@@ -876,13 +861,13 @@ __global__ void k_grid_charge_f32(TileGeom tg, RecPtr U, int64_t n, float qch, d
                                   DevGrid g) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
-    const int cell = U.cell[q];
+    const int cell = rec_cell(U, q);
     const int N[3] = {cell & ((1 << tg.bx) - 1), (cell >> tg.bx) & ((1 << tg.by) - 1),
                       D == 3 ? (cell >> (tg.bx + tg.by)) : 1};
     double a[3][3] = {{0, 0, 0}, {0, 0, 0}, {1, 0, 0}};
 #pragma unroll
-    for (int d = 0; d < D; d++) qweights<double>((double)U.s[d][q], a[d]);
-    const double wq = (double)(qch * U.w[q]);
+    for (int d = 0; d < D; d++) qweights<double>((double)rec_s(U, q, d), a[d]);
+    const double wq = (double)(qch * rec_w(U, q));
     const int nk = D == 3 ? 3 : 1;
     for (int k1 = 0; k1 < nk; k1++)
       for (int j1 = 0; j1 < 3; j1++)
@@ -920,23 +905,23 @@ __global__ __launch_bounds__(kB) void k_energy_f32(TileGeom tg, DevGrid g, RecPt
        q += (int64_t)gridDim.x * blockDim.x) {
     double p[3] = {0, 0, 0};
 #pragma unroll
-    for (int d = 0; d < D; d++) p[d] = (double)U.p[d][q];
-    const double w = (double)U.w[q];
+    for (int d = 0; d < D; d++) p[d] = (double)rec_p(U, q, d);
+    const double w = (double)rec_w(U, q);
     const double gam1 = sqrt(1. + p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) - 1.;
     if constexpr (D == 2) {
       ek[0] += w * p[0];
       ek[1] += w * p[1];
     } else {
-      const int cell = U.cell[q];
+      const int cell = rec_cell(U, q);
       const int Nj = (cell >> tg.bx) & ((1 << tg.by) - 1), Nk = cell >> (tg.bx + tg.by);
-      const double y = g.ymn + ((double)(Nj - 3) + (double)U.s[1][q]) * g.dy;
-      const double z = g.zmn + ((double)(Nk - 3) + (double)U.s[2][q]) * g.dz;
+      const double y = g.ymn + ((double)(Nj - 3) + (double)rec_s(U, q, 1)) * g.dy;
+      const double z = g.zmn + ((double)(Nk - 3) + (double)rec_s(U, q, 2)) * g.dz;
       ek[0] += p[0];
       ek[1] += p[1];
       ek[2] += p[2];
       ek[3] += w * (y * p[2] - z * p[1]);
     }
-    ek[5] += (double)(qch * U.w[q]);
+    ek[5] += (double)(qch * rec_w(U, q));
     ek[6] += w * gam1;
     ek[7] = fmax(ek[7], gam1);
   }

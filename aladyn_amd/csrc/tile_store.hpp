@@ -41,13 +41,15 @@ struct TileGeom {
   int lds_bytes;
 };
 
+// Particle records, 32 bytes each (two 16-byte words, so a record moves
+// with two vector loads or stores): [cell, s0, s1, s2 | p0, p1, p2, w], the
+// packed nearest node as int bits; unused slots (z in 2-D) are 0.
 struct RecPtr {
-  int* cell;
-  float* s[3];
-  float* p[3];
-  float* w;
-  int* id;  // optional original index (tests), nullptr when not tracked
+  float4* r;  // 2 per record
+  int* id;    // optional original index (tests), nullptr when not tracked
 };
+
+__host__ __device__ inline float4* rec_at(const RecPtr& a, int64_t n) { return a.r + 2 * n; }
 
 struct TileStore {
   int64_t alloc = 0;     // records per buffer

@@ -83,7 +83,7 @@ def test_f32_relayout_every_step(dim):
     _check(cfg, _run(cfg, part, ebf, PREC_F32, 10, sort_every=1), 1e-4, check_current=False)
 
 
-@pytest.mark.parametrize("dim,tile", [("3d", (8, 8, 4)), ("3d", (16, 8, 4)), ("2d", (32, 16))])
+@pytest.mark.parametrize("dim,tile", [("3d", (8, 8, 8)), ("3d", (16, 8, 4)), ("2d", (32, 16))])
 def test_f32_other_tile_shapes(dim, tile):
     cfg, part, ebf = CASES[dim]()
     _check(cfg, _run(cfg, part, ebf, PREC_F32, 3, tile=tile), 1e-5)
