@@ -57,36 +57,23 @@ struct TimerRing {
   }
 };
 
-struct SpeciesF64 {
-  double* part = nullptr;  // part(cap, 2D+1) on the device
-  double* tmp = nullptr;   // compaction target, same shape
-  int64_t cap = 0, np = 0;
-};
-
 struct aladyn_ctx {
   aladyn_config cfg{};
   DevGrid g{};
   hipStream_t stream = nullptr;
   std::string err;
   int dev = 0;
-  int ncol = 0;
   bool have_fields = false;
-  // FP64 reference-layout state
+  // grids of the configured precision (the other pair stays null)
   double* ebf64 = nullptr;
   double* jc64 = nullptr;
-  SpeciesF64 sp64[ALADYN_MAX_SPECIES];
-  int* d_blk = nullptr;
-  int64_t blk_cap = 0;
-  // FP32 tile-path state
   float* ebf32 = nullptr;
   float* jc32 = nullptr;
+  // tile-sorted particle stores, one per species
   TileStore ts[ALADYN_MAX_SPECIES];
   TileGeom tg{};
-  float* patch = nullptr;  // per-tile J partial sums
+  void* patch = nullptr;  // per-tile J partial sums (FP32 or FP64)
   bool j_is_zero = true;
-  // small device scalars: [0] ndead (int), [8..] int64 count
-  int* d_ndead = nullptr;
-  int64_t* d_count = nullptr;
   double* d_red = nullptr;      // reduction outputs (16 doubles)
   double* h_red = nullptr;      // pinned mirror
   // diagnostics (continuity)
@@ -215,7 +202,6 @@ extern "C" int aladyn_create(const aladyn_config* cfg, aladyn_ctx** out) {
   if (a.dt >= a.dx || a.dt >= a.dy || (a.ndim == 3 && a.dt >= a.dz))
     return bad("dt must be smaller than every cell size (one-cell-per-step limit)");
   setup_grid(c);
-  c->ncol = 2 * c->g.curr_ndim + 1;
   *out = c;
   HIPCK(c, hipGetDevice(&c->dev));
   HIPCK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -233,13 +219,10 @@ extern "C" int aladyn_create(const aladyn_config* cfg, aladyn_ctx** out) {
     HIPCK(c, hipMalloc(&c->jc32, nj * sizeof(float)));
     HIPCK(c, hipMemsetAsync(c->ebf32, 0, nf * sizeof(float), c->stream));
     HIPCK(c, hipMemsetAsync(c->jc32, 0, nj * sizeof(float), c->stream));
-    int r = tile_geom_init(g, a, &c->tg);
-    if (r) return fail(c, ALADYN_EINVAL, "%s", tile_geom_error(r));
-    const size_t npatch = (size_t)c->tg.ntiles * c->tg.pvol * g.curr_ndim;
-    HIPCK(c, hipMalloc(&c->patch, npatch * sizeof(float)));
   }
-  HIPCK(c, hipMalloc(&c->d_ndead, 64));
-  HIPCK(c, hipMalloc(&c->d_count, 64));
+  if (int r = tile_geom_init(g, a, &c->tg)) return fail(c, ALADYN_EINVAL, "%s", tile_geom_error(r));
+  const size_t npatch = (size_t)c->tg.ntiles * c->tg.pvol * g.curr_ndim;
+  HIPCK(c, hipMalloc(&c->patch, npatch * c->tg.rbytes));
   HIPCK(c, hipMalloc(&c->d_red, 32 * sizeof(double)));
   HIPCK(c, hipHostMalloc(&c->h_red, 32 * sizeof(double), hipHostMallocDefault));
   HIPCK(c, hipStreamSynchronize(c->stream));
@@ -254,14 +237,7 @@ extern "C" int aladyn_destroy(aladyn_ctx* c) {
   (void)hipFree(c->ebf32);
   (void)hipFree(c->jc32);
   (void)hipFree(c->patch);
-  for (auto& s : c->sp64) {
-    (void)hipFree(s.part);
-    (void)hipFree(s.tmp);
-  }
   for (auto& t : c->ts) tile_store_free(&t);
-  (void)hipFree(c->d_blk);
-  (void)hipFree(c->d_ndead);
-  (void)hipFree(c->d_count);
   (void)hipFree(c->d_red);
   if (c->h_red) (void)hipHostFree(c->h_red);
   (void)hipFree(c->rho0);
@@ -312,31 +288,7 @@ extern "C" int aladyn_put_species(aladyn_ctx* c, int32_t ic, const double* part,
   if (np < 0 || (np > 0 && (!part || ld < np)))
     return fail(c, ALADYN_EINVAL, "put_species: need part != NULL and ld >= np");
   const DevGrid& g = c->g;
-  const int ncol = c->ncol;
   if (int r = check_particles(c, part, ld, np)) return r;
-  if (f64(c)) {
-    SpeciesF64& s = c->sp64[ic];
-    if (np > s.cap) {
-      (void)hipFree(s.part);
-      (void)hipFree(s.tmp);
-      s.part = s.tmp = nullptr;
-      s.cap = np > 0 ? np : 1;
-      HIPCK(c, hipMalloc(&s.part, (size_t)s.cap * ncol * sizeof(double)));
-      HIPCK(c, hipMalloc(&s.tmp, (size_t)s.cap * ncol * sizeof(double)));
-    }
-    for (int col = 0; col < ncol && np > 0; col++)
-      HIPCK(c, hipMemcpyAsync(s.part + col * s.cap, part + col * ld, (size_t)np * sizeof(double),
-                              hipMemcpyHostToDevice, c->stream));
-    s.np = np;
-    const int64_t nb = nblk_compact(s.cap);
-    if (nb > c->blk_cap) {
-      (void)hipFree(c->d_blk);
-      HIPCK(c, hipMalloc(&c->d_blk, (size_t)(nb + 1) * sizeof(int)));
-      c->blk_cap = nb;
-    }
-    HIPCK(c, hipStreamSynchronize(c->stream));
-    return ALADYN_OK;
-  }
   std::string msg;
   int r = tile_store_put(&c->ts[ic], c->tg, g, part, ld, np, c->stream, c->cfg.reserved[0] != 0,
                          &msg);
@@ -347,43 +299,10 @@ extern "C" int aladyn_put_species(aladyn_ctx* c, int32_t ic, const double* part,
 extern "C" int aladyn_species_count(aladyn_ctx* c, int32_t ic, int64_t* np) {
   if (int r = check_species(c, ic)) return r;
   if (!np) return fail(c, ALADYN_EINVAL, "np is NULL");
-  if (f64(c)) {
-    *np = c->sp64[ic].np;
-    return ALADYN_OK;
-  }
   std::string msg;
   int r = tile_store_count(&c->ts[ic], c->tg, c->stream, np, &msg);
   if (r) return fail(c, r, "%s", msg.c_str());
   return ALADYN_OK;
-}
-
-// FP64 species: room for np particles, keeping the first s.np rows.
-static int reserve_f64(aladyn_ctx* c, SpeciesF64& s, int64_t np) {
-  if (np <= s.cap) return 0;
-  const int ncol = c->ncol;
-  const int64_t cap = std::max(np, 2 * s.cap);
-  double *part = nullptr, *tmp = nullptr;
-  HIPCK(c, hipMalloc(&part, (size_t)cap * ncol * sizeof(double)));
-  if (hipMalloc(&tmp, (size_t)cap * ncol * sizeof(double)) != hipSuccess) {
-    (void)hipFree(part);
-    return fail(c, ALADYN_ENOMEM, "out of device memory growing species storage");
-  }
-  for (int col = 0; col < ncol && s.np > 0; col++)
-    HIPCK(c, hipMemcpyAsync(part + col * cap, s.part + col * s.cap, (size_t)s.np * sizeof(double),
-                            hipMemcpyDeviceToDevice, c->stream));
-  HIPCK(c, hipStreamSynchronize(c->stream));
-  (void)hipFree(s.part);
-  (void)hipFree(s.tmp);
-  s.part = part;
-  s.tmp = tmp;
-  s.cap = cap;
-  const int64_t nb = nblk_compact(cap);
-  if (nb > c->blk_cap) {
-    (void)hipFree(c->d_blk);
-    HIPCK(c, hipMalloc(&c->d_blk, (size_t)(nb + 1) * sizeof(int)));
-    c->blk_cap = nb;
-  }
-  return 0;
 }
 
 extern "C" int aladyn_lp_window_xshift(aladyn_ctx* c, int32_t witr, int32_t init_iter,
@@ -419,21 +338,9 @@ extern "C" int aladyn_lp_window_xshift(aladyn_ctx* c, int32_t witr, int32_t init
   }
   // particles behind the new left face leave (cell_part_dist(.true.))
   for (int ic = 0; ic < c->cfg.nsp; ic++) {
-    if (f64(c)) {
-      SpeciesF64& s = c->sp64[ic];
-      if (s.np == 0) continue;
-      compact_f64(g, s.part, s.tmp, s.cap, s.np, c->d_blk, c->d_count, c->stream);
-      HIPCK(c, hipGetLastError());
-      int64_t n = 0;
-      HIPCK(c, hipMemcpyAsync(&n, c->d_count, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
-      HIPCK(c, hipStreamSynchronize(c->stream));
-      std::swap(s.part, s.tmp);
-      s.np = n;
-    } else {
-      std::string msg;
-      if (int r = tile_store_window(&c->ts[ic], c->tg, g, (int)nshx, c->stream, &msg))
-        return fail(c, r, "%s", msg.c_str());
-    }
+    std::string msg;
+    if (int r = tile_store_window(&c->ts[ic], c->tg, g, (int)nshx, c->stream, &msg))
+      return fail(c, r, "%s", msg.c_str());
   }
   return ALADYN_OK;
 }
@@ -445,16 +352,6 @@ extern "C" int aladyn_inject_species(aladyn_ctx* c, int32_t ic, const double* pa
     return fail(c, ALADYN_EINVAL, "inject_species: need part != NULL and ld >= np");
   if (np == 0) return ALADYN_OK;
   if (int r = check_particles(c, part, ld, np)) return r;
-  if (f64(c)) {
-    SpeciesF64& s = c->sp64[ic];
-    if (int r = reserve_f64(c, s, s.np + np)) return r;
-    for (int col = 0; col < c->ncol; col++)
-      HIPCK(c, hipMemcpyAsync(s.part + col * s.cap + s.np, part + col * ld,
-                              (size_t)np * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCK(c, hipStreamSynchronize(c->stream));  // the host array is only borrowed
-    s.np += np;
-    return ALADYN_OK;
-  }
   std::string msg;
   if (int r = tile_store_inject(&c->ts[ic], c->tg, c->g, part, ld, np, c->stream,
                                 c->cfg.reserved[0] != 0, &msg))
@@ -479,14 +376,6 @@ extern "C" int aladyn_get_species(aladyn_ctx* c, int32_t ic, double* part, int64
   if (n > 0 && (!part || ld < n))
     return fail(c, ALADYN_EINVAL, "get_species: ld (%lld) < particle count (%lld)",
                 (long long)ld, (long long)n);
-  if (f64(c)) {
-    SpeciesF64& s = c->sp64[ic];
-    for (int col = 0; col < c->ncol && n > 0; col++)
-      HIPCK(c, hipMemcpyAsync(part + col * ld, s.part + col * s.cap, (size_t)n * sizeof(double),
-                              hipMemcpyDeviceToHost, c->stream));
-    HIPCK(c, hipStreamSynchronize(c->stream));
-    return ALADYN_OK;
-  }
   std::string msg;
   int r = tile_store_get(&c->ts[ic], c->tg, c->g, part, ld, c->stream, &msg);
   if (r) return fail(c, r, "%s", msg.c_str());
@@ -581,12 +470,7 @@ static int ensure_diag_buffers(aladyn_ctx* c) {
 
 // rho(ncell) += grid charge of species ic (device FP64)
 static int grid_charge_dev(aladyn_ctx* c, int ic, double* rho) {
-  if (f64(c)) {
-    const SpeciesF64& s = c->sp64[ic];
-    grid_charge_f64(c->g, s.part, s.cap, s.np, rho, c->stream);
-  } else {
-    tile_store_grid_charge(&c->ts[ic], c->tg, c->g, rho, c->stream);
-  }
+  tile_store_grid_charge(&c->ts[ic], c->tg, c->g, rho, c->stream);
   HIPCK(c, hipGetLastError());
   return 0;
 }
@@ -607,34 +491,17 @@ extern "C" int aladyn_particle_stage(aladyn_ctx* c, int32_t ic, int32_t initial_
                             (size_t)g.ncell * g.curr_ndim * esz, hipMemcpyDeviceToDevice,
                             c->stream));
   }
-  if (f64(c)) {
-    SpeciesF64& s = c->sp64[ic];
-    if (s.np > 0) {
-      HIPCK(c, hipMemsetAsync(c->d_ndead, 0, sizeof(int), c->stream));
-      HIPCK(c, hipEventRecord(c->tpart.begin(), c->stream));
-      particles_f64(g, s.part, s.cap, s.np, c->ebf64, c->jc64, alp, initial_time ? 1 : 0,
-                    c->d_ndead, c->stream);
-      HIPCK(c, hipGetLastError());
-      HIPCK(c, hipEventRecord(c->tpart.end(), c->stream));
-      c->tpart.commit();
-      int ndead = 0;
-      HIPCK(c, hipMemcpyAsync(&ndead, c->d_ndead, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      HIPCK(c, hipStreamSynchronize(c->stream));
-      if (ndead > 0) {
-        compact_f64(g, s.part, s.tmp, s.cap, s.np, c->d_blk, c->d_count, c->stream);
-        HIPCK(c, hipGetLastError());
-        std::swap(s.part, s.tmp);
-        s.np -= ndead;
-      }
-    }
-  } else {
-    std::string msg;
-    int r = tile_store_step(&c->ts[ic], c->tg, g, c->ebf32, c->jc32, c->patch, (float)alp,
-                            initial_time ? 1 : 0, c->j_is_zero, c->stream, c->tpart.begin(),
-                            c->tpart.end(), &msg);
-    if (r) return fail(c, r, "%s", msg.c_str());
-    if (c->ts[ic].loaded) c->tpart.commit();
-  }
+  std::string msg;
+  const int init = initial_time ? 1 : 0;
+  const int r =
+      f64(c) ? tile_store_step<double>(&c->ts[ic], c->tg, g, c->ebf64, c->jc64, (double*)c->patch,
+                                       alp, init, c->j_is_zero, c->stream, c->tpart.begin(),
+                                       c->tpart.end(), &msg)
+             : tile_store_step<float>(&c->ts[ic], c->tg, g, c->ebf32, c->jc32, (float*)c->patch,
+                                      (float)alp, init, c->j_is_zero, c->stream, c->tpart.begin(),
+                                      c->tpart.end(), &msg);
+  if (r) return fail(c, r, "%s", msg.c_str());
+  if (c->ts[ic].loaded) c->tpart.commit();
   c->j_is_zero = false;
   if (c->diag) {
     fill<double>(c->rho1, g.ncell, 0.0, c->stream);
@@ -684,12 +551,7 @@ extern "C" int aladyn_lpf2_evolve(aladyn_ctx* c, int32_t iter, int32_t initial_t
 extern "C" int aladyn_energy_momenta(aladyn_ctx* c, int32_t ic, double ek[7], double* ekmax) {
   if (int r = check_species(c, ic)) return r;
   HIPCK(c, hipMemsetAsync(c->d_red, 0, 8 * sizeof(double), c->stream));
-  if (f64(c)) {
-    const SpeciesF64& s = c->sp64[ic];
-    energy_momenta_f64(c->g, s.part, s.cap, s.np, c->d_red, c->stream);
-  } else {
-    tile_store_energy(&c->ts[ic], c->tg, c->g, c->d_red, c->stream);
-  }
+  tile_store_energy(&c->ts[ic], c->tg, c->g, c->d_red, c->stream);
   HIPCK(c, hipGetLastError());
   HIPCK(c, hipMemcpyAsync(c->h_red, c->d_red, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCK(c, hipStreamSynchronize(c->stream));

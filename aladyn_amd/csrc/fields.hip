@@ -2,7 +2,7 @@
 // (field_xyzbd), current ghost fold + normalisation (jc_xyzbd, djc), Mur-type
 // absorbing boundaries (ef_bds/bf_bds, imbd = 0) and the Yee curl updates
 // (rote, rotb + current source).  Templated on the storage type (double for
-// the reference-layout path, float for the tile path); all arrays are in the
+// the FP64 path, float for the FP32 path); all arrays are in the
 // reference's Fortran layout ebf(n1p,n2p,n3p,nc), x fastest, so consecutive
 // lanes touch consecutive addresses.
 #include "lp_kernels.hpp"

@@ -1,5 +1,5 @@
 // lp_kernels.hpp -- host-side launchers of the LP-step kernels and the
-// device functions shared by the two particle paths.
+// device functions shared by the tile kernel and the overflow kernel.
 #pragma once
 
 #include "lp_common.hpp"
@@ -12,25 +12,6 @@ template <typename T> void curr_collect(const DevGrid& g, T* jc, hipStream_t s);
 template <typename T> void advance_fields(const DevGrid& g, T* ebf, const T* jc, hipStream_t s);
 template <typename T> void fill(T* a, int64_t n, T v, hipStream_t s);
 template <typename T> void fields_left_xshift(const DevGrid& g, T* ebf, int shx, hipStream_t s);
-
-// ------------------------------------------------------- particles_f64.hip
-// Fused gather + charge multiply + [back-shift] + push + box test + Esirkepov
-// deposit on the reference layout part(ld, 2D+1); jc += raw current with
-// FP64 atomics; *ndead += particles that left the box.
-void particles_f64(const DevGrid& g, double* part, int64_t ld, int64_t np, const double* ebf,
-                   double* jc, double alp, int initial, int* ndead, hipStream_t s);
-// Stable removal of particles outside the box (reset_all_part_dist order).
-// Writes survivors of `part` into `out` (same ld); returns the count through
-// *d_count (device) and block scratch `blk` (>= nblk_compact(np) ints).
-int64_t nblk_compact(int64_t np);
-void compact_f64(const DevGrid& g, const double* part, double* out, int64_t ld, int64_t np,
-                 int* blk, int64_t* d_count, hipStream_t s);
-// set_grid_charge in FP64 from the reference layout, rho += q*w*S.
-void grid_charge_f64(const DevGrid& g, const double* part, int64_t ld, int64_t np, double* rho,
-                     hipStream_t s);
-// energy_momenta partial sums: out[0..6] (+=) and out[7] = max(gamma-1).
-void energy_momenta_f64(const DevGrid& g, const double* part, int64_t ld, int64_t np,
-                        double* out, hipStream_t s);
 
 // ---------------------------------------------------------- reductions.hip
 template <typename T>

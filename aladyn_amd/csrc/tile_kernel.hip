@@ -1,8 +1,8 @@
-// tile_kernel.hip -- the FP32 particle kernel of the tile path.
+// tile_kernel.hip -- the particle kernel of the tile path (FP32 and FP64).
 //
-// One workgroup of 1024 threads per tile of TX x TY x TZ node-cells
-// (the tile shape is a template parameter, so LDS offsets inside a stencil
-// are immediates).  Per workgroup:
+// One workgroup per tile of TX x TY x TZ node-cells, one thread per
+// node-cell (the tile shape is a template parameter, so LDS offsets inside a
+// stencil are immediates; R is float or double).  Per workgroup:
 //   1. stage the field patch (nodes T0-1 .. T0+T per axis, all components)
 //      from HBM into LDS and zero an FP64 LDS current patch (nodes
 //      T0-2 .. T0+T+1, the reach of the 5-slot Esirkepov window of
@@ -11,13 +11,14 @@
 //   3. split the sorted chunk into equal consecutive slices, one per thread,
 //      and walk them in lock step: gather from LDS, charge multiply,
 //      [back-shift], exact-gamma push, carry, box test; a particle that keeps
-//      its nearest node deposits its 2x3x3-per-component Esirkepov current
-//      with LDS FP64 atomics, one that changes node is queued per wave and
-//      deposited through the union window when 64 are waiting; survivors are
-//      written to the other buffer: stayers into slots reserved per chunk,
-//      movers into their new tile with wave-aggregated atomics, overflow into
-//      the global list;
-//   4. write the LDS current patch to HBM (FP32) for the fold.
+//      its nearest node adds its 2x3x3-per-component Esirkepov current to
+//      register sums of its cell (flushed into the LDS patch when the
+//      thread's slice moves to the next cell), one that changes node is
+//      queued per wave and deposited through the union window when 64 are
+//      waiting; survivors are written to the other buffer: stayers into
+//      slots reserved per chunk, movers into their new tile with
+//      wave-aggregated atomics, overflow into the global list;
+//   4. write the LDS current patch to HBM for the fold.
 // LDS accumulation is FP64 because LDS FP32 atomic adds are far slower than
 // FP64 ones on gfx950; there are no global float atomics.
 #include "tile_device.hpp"
@@ -36,25 +37,26 @@ struct TileArgs {
 // min(N_old, N_new) - 1; the 3 fluxes between them are cumulative sums of
 // W0 - W1 (the fourth vanishes by the partition of unity).  jb points at the
 // LDS patch node of the union base; offsets are compile-time.
-template <int D, int JX, int JY, int PVOL>
-__device__ inline void deposit_general(double* jb, const float s0[3], const float s1[3],
-                                       const int o0[3], const int o1[3], float wq) {
-  float W0[3][4], W1[3][4];
+template <int D, int JX, int JY, int PVOL, typename R>
+__device__ inline void deposit_general(double* jb, const R s0[3], const R s1[3],
+                                       const int o0[3], const int o1[3], R wq) {
+  const R zero = R(0), half = R(0.5);
+  R W0[3][4], W1[3][4];
 #pragma unroll
   for (int d = 0; d < D; d++) {
-    float a0[3], a1[3];
-    qweights<float>(s0[d], a0);
-    qweights<float>(s1[d], a1);
+    R a0[3], a1[3];
+    qweights<R>(s0[d], a0);
+    qweights<R>(s1[d], a1);
 #pragma unroll
     for (int m = 0; m < 4; m++) {
-      W0[d][m] = o0[d] == 0 ? (m < 3 ? a0[m] : 0.f) : (m > 0 ? a0[m - 1] : 0.f);
-      W1[d][m] = o1[d] == 0 ? (m < 3 ? a1[m] : 0.f) : (m > 0 ? a1[m - 1] : 0.f);
+      W0[d][m] = o0[d] == 0 ? (m < 3 ? a0[m] : zero) : (m > 0 ? a0[m - 1] : zero);
+      W1[d][m] = o1[d] == 0 ? (m < 3 ? a1[m] : zero) : (m > 0 ? a1[m - 1] : zero);
     }
   }
-  float F[3][3];
+  R F[3][3];
 #pragma unroll
   for (int d = 0; d < D; d++) {
-    float c = 0.f;
+    R c = zero;
 #pragma unroll
     for (int f = 0; f < 3; f++) {
       c = c + W0[d][f] - W1[d][f];
@@ -73,13 +75,13 @@ __device__ inline void deposit_general(double* jb, const float s0[3], const floa
     // J_x: flux_x (x) [W0z (W0y + W1y/2) + W1z (W1y + W0y/2)], etc.
 #pragma unroll
     for (int m = 0; m < 4; m++) {
-      const float y0 = W0[1][m] + 0.5f * W1[1][m], y1 = W1[1][m] + 0.5f * W0[1][m];
-      const float x0 = W0[0][m] + 0.5f * W1[0][m], x1 = W1[0][m] + 0.5f * W0[0][m];
+      const R y0 = W0[1][m] + half * W1[1][m], y1 = W1[1][m] + half * W0[1][m];
+      const R x0 = W0[0][m] + half * W1[0][m], x1 = W1[0][m] + half * W0[0][m];
 #pragma unroll
       for (int l = 0; l < 4; l++) {
-        const float tyz = y0 * W0[2][l] + y1 * W1[2][l];
-        const float txz = x0 * W0[2][l] + x1 * W1[2][l];
-        const float txy = x0 * W0[1][l] + x1 * W1[1][l];
+        const R tyz = y0 * W0[2][l] + y1 * W1[2][l];
+        const R txz = x0 * W0[2][l] + x1 * W1[2][l];
+        const R txy = x0 * W0[1][l] + x1 * W1[1][l];
 #pragma unroll
         for (int f = 0; f < 3; f++) {
           atomicAdd(jb + (f + JX * (m + JY * l)), (double)(F[0][f] * tyz));
@@ -97,55 +99,56 @@ __device__ inline void deposit_general(double* jb, const float s0[3], const floa
 // W0[0]-W1[0] = (s1-s0)(1-s0-s1)/2 and W1[2]-W0[2] = (s1-s0)(1+s0+s1)/2 are
 // the only nonzero fluxes, so the current has 2x3x3 values per component
 // (2x3 in 2-D), offsets from node N-1.
-template <int D>
+template <int D, typename R>
 struct CellAcc {
-  float jx[2][3][3], jy[3][2][3], jz[3][3][2];
+  R jx[2][3][3], jy[3][2][3], jz[3][3][2];
 
   __device__ inline void zero() {
 #pragma unroll
     for (int a = 0; a < 18; a++) {
-      (&jx[0][0][0])[a] = 0.f;
-      (&jy[0][0][0])[a] = 0.f;
-      (&jz[0][0][0])[a] = 0.f;
+      (&jx[0][0][0])[a] = R(0);
+      (&jy[0][0][0])[a] = R(0);
+      (&jz[0][0][0])[a] = R(0);
     }
   }
 
-  __device__ inline void add(const float s0[3], const float s1[3], float wq) {
-    float a0[3][3], a1[3][3], fl[3][2];
+  __device__ inline void add(const R s0[3], const R s1[3], R wq) {
+    const R half = R(0.5), one = R(1);
+    R a0[3][3], a1[3][3], fl[3][2];
 #pragma unroll
     for (int d = 0; d < D; d++) {
-      qweights<float>(s0[d], a0[d]);
-      qweights<float>(s1[d], a1[d]);
-      const float h = 0.5f * wq * (s1[d] - s0[d]);
-      fl[d][0] = h * ((1.f - s0[d]) - s1[d]);
-      fl[d][1] = h * ((1.f + s0[d]) + s1[d]);
+      qweights<R>(s0[d], a0[d]);
+      qweights<R>(s1[d], a1[d]);
+      const R h = half * wq * (s1[d] - s0[d]);
+      fl[d][0] = h * ((one - s0[d]) - s1[d]);
+      fl[d][1] = h * ((one + s0[d]) + s1[d]);
     }
     if constexpr (D == 2) {
 #pragma unroll
       for (int mm = 0; mm < 3; mm++) {
-        const float ty2 = a0[1][mm] + a1[1][mm];
-        const float tx2 = a0[0][mm] + a1[0][mm];
+        const R ty2 = a0[1][mm] + a1[1][mm];
+        const R tx2 = a0[0][mm] + a1[0][mm];
 #pragma unroll
         for (int f = 0; f < 2; f++) {
-          jx[f][mm][0] = fmaf(fl[0][f], ty2, jx[f][mm][0]);
-          jy[mm][f][0] = fmaf(fl[1][f], tx2, jy[mm][f][0]);
+          jx[f][mm][0] = fma_of<R>(fl[0][f], ty2, jx[f][mm][0]);
+          jy[mm][f][0] = fma_of<R>(fl[1][f], tx2, jy[mm][f][0]);
         }
       }
     } else {
 #pragma unroll
       for (int mm = 0; mm < 3; mm++) {
-        const float y0 = a0[1][mm] + 0.5f * a1[1][mm], y1 = a1[1][mm] + 0.5f * a0[1][mm];
-        const float x0 = a0[0][mm] + 0.5f * a1[0][mm], x1 = a1[0][mm] + 0.5f * a0[0][mm];
+        const R y0 = a0[1][mm] + half * a1[1][mm], y1 = a1[1][mm] + half * a0[1][mm];
+        const R x0 = a0[0][mm] + half * a1[0][mm], x1 = a1[0][mm] + half * a0[0][mm];
 #pragma unroll
         for (int l = 0; l < 3; l++) {
-          const float tyz = y0 * a0[2][l] + y1 * a1[2][l];
-          const float txz = x0 * a0[2][l] + x1 * a1[2][l];
-          const float txy = x0 * a0[1][l] + x1 * a1[1][l];
+          const R tyz = y0 * a0[2][l] + y1 * a1[2][l];
+          const R txz = x0 * a0[2][l] + x1 * a1[2][l];
+          const R txy = x0 * a0[1][l] + x1 * a1[1][l];
 #pragma unroll
           for (int f = 0; f < 2; f++) {
-            jx[f][mm][l] = fmaf(fl[0][f], tyz, jx[f][mm][l]);
-            jy[mm][f][l] = fmaf(fl[1][f], txz, jy[mm][f][l]);
-            jz[mm][l][f] = fmaf(fl[2][f], txy, jz[mm][l][f]);
+            jx[f][mm][l] = fma_of<R>(fl[0][f], tyz, jx[f][mm][l]);
+            jy[mm][f][l] = fma_of<R>(fl[1][f], txz, jy[mm][f][l]);
+            jz[mm][l][f] = fma_of<R>(fl[2][f], txy, jz[mm][l][f]);
           }
         }
       }
@@ -178,43 +181,55 @@ struct CellAcc {
   }
 };
 
-// LDS layout of the tile kernel (all sizes compile-time):
-//   Fl  float  [NF][GVOL]   field patch, nodes T0-1 .. T0+T per axis
+// LDS layout of the tile kernel (all sizes compile-time; R = float or double):
+//   Fl  R      [NF][GVOL]   field patch, nodes T0-1 .. T0+T per axis
 //   Jl  double [D][PVOL]    current patch, nodes T0-2 .. T0+T+1 per axis
-//   Q   int    [NW][8][QCAP] per-wave queue of node-changing particles
+//   Q   per wave: R s[6][QCAP], float wq[QCAP], int meta[QCAP]: the queue of
+//       node-changing particles
 //   offs, fillc int [NCL+1], [NCL]; misc int [3]; perm u16 [kChunk]
-template <int D, int TX, int TY, int TZ, int NT>
+template <int D, int TX, int TY, int TZ, int NT, typename R>
 struct TileShape {
   static constexpr int NCL = TX * TY * TZ, NF = D == 3 ? 6 : 3, NW = NT / 64;
   static constexpr int GX = TX + 2, GY = TY + 2, GZ = D == 3 ? TZ + 2 : 1, GVOL = GX * GY * GZ;
   static constexpr int JX = TX + 4, JY = TY + 4, JZ = D == 3 ? TZ + 4 : 1, PVOL = JX * JY * JZ;
-  static constexpr int QCAP = 128, QW = 8;
-  static constexpr int F_WORDS = (NF * GVOL + 1) & ~1;
-  static constexpr int J_WORDS = 2 * D * PVOL;
-  static constexpr int Q_WORDS = NW * QW * QCAP;
-  static constexpr int I_WORDS = 2 * NCL + 4;
-  static constexpr int BYTES = 4 * (F_WORDS + J_WORDS + Q_WORDS + I_WORDS) + 2 * kChunk;
+  static constexpr int QCAP = 128;
+  static constexpr int F_BYTES = (NF * GVOL * (int)sizeof(R) + 15) & ~15;
+  static constexpr int J_BYTES = 8 * D * PVOL;
+  static constexpr int QW_BYTES = QCAP * (6 * (int)sizeof(R) + 8);  // one wave's queue
+  static constexpr int Q_BYTES = NW * QW_BYTES;
+  static constexpr int I_BYTES = 4 * (2 * NCL + 4);
+  static constexpr int BYTES = F_BYTES + J_BYTES + Q_BYTES + I_BYTES + 2 * kChunk;
   static_assert(NT % 64 == 0 && NT >= NCL, "threads: whole wavefronts, at least one per cell");
   static_assert(kChunk % NT == 0, "a chunk is a whole number of records per thread");
 };
 
+// One wave's queue in LDS.
+template <typename R, int QCAP>
+struct WaveQueue {
+  R* s;        // [6][QCAP]: s0[3], s1[3]
+  float* wq;   // [QCAP]
+  int* meta;   // [QCAP]: local cell | (sign+1) << (10 + 2d) per axis
+  __device__ WaveQueue(char* base)
+      : s((R*)base), wq((float*)(base + 6 * QCAP * sizeof(R))),
+        meta((int*)(base + 6 * QCAP * sizeof(R) + 4 * QCAP)) {}
+};
+
 // Drains n <= 64 queued node-changing particles, one per lane, through the
-// union-window deposit.  Queue entry words: s0[3], s1[3], w*q, meta (local
-// cell | (sign+1) << (10 + 2d) per axis).
-template <int D, int TX, int TY, int TZ, int NT>
-__device__ inline void drain_queue(const int* qw, int head, int n, double* Jl) {
-  using S = TileShape<D, TX, TY, TZ, NT>;
+// union-window deposit.
+template <int D, int TX, int TY, int TZ, int NT, typename R>
+__device__ inline void drain_queue(const WaveQueue<R, 128>& qu, int head, int n, double* Jl) {
+  using S = TileShape<D, TX, TY, TZ, NT, R>;
   const int lane = (int)__lane_id();
   if (lane < n) {
     const int e = (head + lane) & (S::QCAP - 1);
-    float s0[3] = {0.f, 0.f, 0.f}, s1[3] = {0.f, 0.f, 0.f};
+    R s0[3] = {R(0), R(0), R(0)}, s1[3] = {R(0), R(0), R(0)};
 #pragma unroll
     for (int d = 0; d < D; d++) {
-      s0[d] = __int_as_float(qw[d * S::QCAP + e]);
-      s1[d] = __int_as_float(qw[(3 + d) * S::QCAP + e]);
+      s0[d] = qu.s[d * S::QCAP + e];
+      s1[d] = qu.s[(3 + d) * S::QCAP + e];
     }
-    const float wq = __int_as_float(qw[6 * S::QCAP + e]);
-    const int meta = qw[7 * S::QCAP + e];
+    const R wq = (R)qu.wq[e];
+    const int meta = qu.meta[e];
     const int lc = meta & 1023;
     int o0[3] = {0, 0, 0}, o1[3] = {0, 0, 0}, sh[3] = {0, 0, 0};
 #pragma unroll
@@ -228,27 +243,28 @@ __device__ inline void drain_queue(const int* qw, int head, int n, double* Jl) {
     // union base N-1+min(0,sig) sits at patch offset 1+min(0,sig) from node N-2
     double* ub = Jl + (cx + 1 + sh[0]) +
                  S::JX * ((cy + 1 + sh[1]) + S::JY * (D == 3 ? cz + 1 + sh[2] : 0));
-    deposit_general<D, S::JX, S::JY, S::PVOL>(ub, s0, s1, o0, o1, wq);
+    deposit_general<D, S::JX, S::JY, S::PVOL, R>(ub, s0, s1, o0, o1, wq);
   }
 }
 
-template <int D, int TX, int TY, int TZ, int NT>
+template <int D, int TX, int TY, int TZ, int NT, typename R>
 __global__ __launch_bounds__(NT) void k_tile(
-    TileArgs ta, StepConst sc, const float* __restrict__ ebf, RecPtr A, RecPtr B,
+    TileArgs ta, StepConst<R> sc, const R* __restrict__ ebf, RecPtr A, RecPtr B,
     const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
     const int* __restrict__ cnt_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_cur,
-    int* __restrict__ ovf_next, float* __restrict__ patch, int* __restrict__ grow) {
-  using S = TileShape<D, TX, TY, TZ, NT>;
+    int* __restrict__ ovf_next, R* __restrict__ patch, int* __restrict__ grow) {
+  using S = TileShape<D, TX, TY, TZ, NT, R>;
   constexpr int NCL = S::NCL, NF = S::NF, GX = S::GX, GY = S::GY, GVOL = S::GVOL;
   constexpr int JX = S::JX, JY = S::JY, PVOL = S::PVOL, QCAP = S::QCAP;
-  extern __shared__ float lds[];
-  float* Fl = lds;
-  double* Jl = (double*)(lds + S::F_WORDS);
-  int* Q = (int*)(lds + S::F_WORDS + S::J_WORDS);
-  int* offs = Q + S::Q_WORDS;  // NCL + 1
+  extern __shared__ double lds_d[];
+  char* lds = (char*)lds_d;
+  R* Fl = (R*)lds;
+  double* Jl = (double*)(lds + S::F_BYTES);
+  char* Q = lds + S::F_BYTES + S::J_BYTES;
+  int* offs = (int*)(Q + S::Q_BYTES);  // NCL + 1
   int* fillc = offs + NCL + 1;  // NCL
   int* misc = fillc + NCL;      // [0] chunk slot base, [1] stayers, [2] max per cell
-  unsigned short* perm = (unsigned short*)(misc + 2);
+  unsigned short* perm = (unsigned short*)(misc + 4);
 
   const int t = blockIdx.x;
   const int tx = t % ta.nt0, ty = (t / ta.nt0) % ta.nt1, tz = t / (ta.nt0 * ta.nt1);
@@ -257,14 +273,14 @@ __global__ __launch_bounds__(NT) void k_tile(
   // before the first LDS store, so the HBM latency is paid once per tile
   {
     constexpr int FPT = (NF * GVOL + NT - 1) / NT;
-    float fv[FPT];
+    R fv[FPT];
 #pragma unroll
     for (int u = 0; u < FPT; u++) {
       const int q = threadIdx.x + u * NT;
       const int c = q / GVOL, r = q - c * GVOL;
       const int li = r % GX, lj = (r / GX) % GY, lk = r / (GX * GY);
       const int i = T0x - 1 + li, j = T0y - 1 + lj, k = D == 3 ? T0z - 1 + lk : 1;
-      fv[u] = 0.f;
+      fv[u] = R(0);
       if (q < NF * GVOL && i <= ta.n1p && j <= ta.n2p && k <= ta.n3p)
         fv[u] = ebf[(int64_t)c * ta.ncell + (i - 1) +
                     (int64_t)ta.n1p * ((j - 1) + (int64_t)ta.n2p * (k - 1))];
@@ -280,7 +296,7 @@ __global__ __launch_bounds__(NT) void k_tile(
   const int me = threadIdx.x;
   const int lane = (int)__lane_id();
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
-  int* qw = Q + (me >> 6) * (S::QW * QCAP);
+  const WaveQueue<R, 128> qu(Q + (me >> 6) * S::QW_BYTES);
   int qhead = 0, qcount = 0;  // wave-uniform ring state
 
   const int cap_t = seg_cap[t];
@@ -292,7 +308,7 @@ __global__ __launch_bounds__(NT) void k_tile(
     const int li = lc % TX, lj = (lc / TX) % TY, lk = D == 3 ? lc / (TX * TY) : 0;
     return Jl + (li + 1) + JX * ((lj + 1) + JY * (D == 3 ? lk + 1 : 0));
   };
-  CellAcc<D> acc;
+  CellAcc<D, R> acc;
   acc.zero();
   int acc_lc = -1;
 
@@ -308,7 +324,7 @@ __global__ __launch_bounds__(NT) void k_tile(
 #pragma unroll
     for (int u = 0; u < CPT; u++) {
       const int q = me + u * NT;
-      lcv[u] = q < m ? rec_cell(A, base + c0 + q) : kHole;
+      lcv[u] = q < m ? rec_cell<R>(A, base + c0 + q) : kHole;
     }
     // local cell index, kHole for a hole, -2 for a record of another tile
 #pragma unroll
@@ -356,7 +372,7 @@ __global__ __launch_bounds__(NT) void k_tile(
       } else if (lc == -2) {
         // not in this tile (the store never does this): hand it to the
         // global-memory kernel that runs after this one
-        copy_rec<D>(A, base + c0 + q, A, sc.ovf_base + atomicAdd(ovf_cur, 1));
+        copy_rec<R>(A, base + c0 + q, A, sc.ovf_base + atomicAdd(ovf_cur, 1));
       }
     }
     __syncthreads();
@@ -378,33 +394,33 @@ __global__ __launch_bounds__(NT) void k_tile(
     const int sl = (wv / sigma) * (64 * sigma) + (wv % sigma) + sigma * lane;
     const int q0 = sl * W, q1 = min(q0 + W, nval);
     const int64_t cb = base + c0;
-    Rec nxt = {};
-    if (q0 < q1) load_raw<D>(A, cb + perm[q0], nxt);
+    Rec<R> nxt = {};
+    if (q0 < q1) load_raw<D, R>(A, cb + perm[q0], nxt);
     for (int r = 0; r < W; r++) {
       const int qi = q0 + r;
-      Rec q = nxt;
-      if (qi + 1 < q1) load_raw<D>(A, cb + perm[qi + 1], nxt);
+      Rec<R> q = nxt;
+      if (qi + 1 < q1) load_raw<D, R>(A, cb + perm[qi + 1], nxt);
       bool keep = false, cross = false;
-      float so[3] = {0.f, 0.f, 0.f}, wq = 0.f;
+      R so[3] = {R(0), R(0), R(0)}, wq = R(0);
       int lc = 0, N0[3] = {0, 0, 0};
       if (qi < q1) {
-        decode_cell<D>(q, ta.bx, ta.by);
+        decode_cell<D, R>(q, ta.bx, ta.by);
         const int li = q.N[0] - T0x, lj = q.N[1] - T0y, lk = D == 3 ? q.N[2] - T0z : 0;
         lc = li + TX * (lj + TY * lk);
         // LDS field patch base at node N-1 of the particle's cell
-        const float* fb = Fl + li + GX * (lj + GY * lk);
-        const auto F = [&](int c, int i1, int j1, int k1) -> float {
+        const R* fb = Fl + li + GX * (lj + GY * lk);
+        const auto F = [&](int c, int i1, int j1, int k1) -> R {
           return fb[c * GVOL + i1 + GX * (j1 + GY * k1)];
         };
-        float ef[6];
-        gather<D>(q.s, F, ef);
+        R ef[6];
+        gather<D, R>(q.s, F, ef);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           so[d] = q.s[d];
           N0[d] = q.N[d];
         }
-        const float dtg = push<D>(q.p, ef, sc.qch, sc.alp, sc.dt, sc.initial);
-        keep = move_and_test<D>(q, dtg, sc.inv, sc.nn);
+        const R dtg = push<D, R>(q.p, ef, sc.qch, sc.alp, sc.dt, sc.initial);
+        keep = move_and_test<D, R>(q, dtg, sc.inv, sc.nn);
         wq = sc.qch * q.w;
         cross = q.N[0] != N0[0] || q.N[1] != N0[1];
         if (D == 3) cross = cross || q.N[2] != N0[2];
@@ -426,12 +442,12 @@ __global__ __launch_bounds__(NT) void k_tile(
         int meta = lc;
 #pragma unroll
         for (int d = 0; d < D; d++) {
-          qw[d * QCAP + e] = __float_as_int(so[d]);
-          qw[(3 + d) * QCAP + e] = __float_as_int(q.s[d]);
+          qu.s[d * QCAP + e] = so[d];
+          qu.s[(3 + d) * QCAP + e] = q.s[d];
           meta |= (q.N[d] - N0[d] + 1) << (10 + 2 * d);
         }
-        qw[6 * QCAP + e] = __float_as_int(wq);
-        qw[7 * QCAP + e] = meta;
+        qu.wq[e] = (float)wq;
+        qu.meta[e] = meta;
       }
       qcount += __popcll(cm);
       if (keep) {
@@ -453,13 +469,13 @@ __global__ __launch_bounds__(NT) void k_tile(
           dst = sc.ovf_base + atomicAdd(ovf_next, 1);
           grow[dtile] = 1;
         }
-        store_rec<D>(B, dst, q, ta.bx, ta.by);
+        store_rec<D, R>(B, dst, q, ta.bx, ta.by);
       }
       if (qcount >= 64) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        drain_queue<D, TX, TY, TZ, NT>(qw, qhead, 64, Jl);
+        drain_queue<D, TX, TY, TZ, NT, R>(qu, qhead, 64, Jl);
         qhead = (qhead + 64) & (QCAP - 1);
         qcount -= 64;
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -472,7 +488,7 @@ __global__ __launch_bounds__(NT) void k_tile(
       const int nstay = misc[1], nval = offs[NCL];
       for (int q = nstay + me; q < nval; q += NT) {
         const int slot = chunk_base + q;
-        if (slot < cap_t) set_rec_cell(B, base + slot, kHole);
+        if (slot < cap_t) set_rec_cell<R>(B, base + slot, kHole);
       }
     }
   }
@@ -480,75 +496,95 @@ __global__ __launch_bounds__(NT) void k_tile(
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    drain_queue<D, TX, TY, TZ, NT>(qw, qhead, qcount, Jl);
+    drain_queue<D, TX, TY, TZ, NT, R>(qu, qhead, qcount, Jl);
   }
   if (acc_lc >= 0) acc.template flush<JX, JY, PVOL>(patch_node(acc_lc));
   __syncthreads();
-  float* P = patch + (int64_t)t * D * PVOL;
-  for (int q = threadIdx.x; q < D * PVOL; q += NT) P[q] = (float)Jl[q];
+  R* P = patch + (int64_t)t * D * PVOL;
+  for (int q = threadIdx.x; q < D * PVOL; q += NT) P[q] = (R)Jl[q];
 }
 
-// threads per workgroup: one per node-cell of the tile (8 wavefronts for an
-// 8^3 tile); the register sums of the current need more than the 128 VGPRs
-// a 1024-thread workgroup would allow
+// threads per workgroup: one per node-cell of the tile (4 wavefronts for an
+// 8x8x4 tile); the register sums of the current need more than the 128
+// VGPRs a 1024-thread workgroup would allow
 template <int D, int TX, int TY, int TZ>
 static constexpr int tile_threads_for() {
   return TX * TY * TZ;
 }
 
-template <int D, int TX, int TY, int TZ>
+template <int D, int TX, int TY, int TZ, typename R>
 static int lds_bytes_for() {
-  return TileShape<D, TX, TY, TZ, tile_threads_for<D, TX, TY, TZ>()>::BYTES;
+  return TileShape<D, TX, TY, TZ, tile_threads_for<D, TX, TY, TZ>(), R>::BYTES;
 }
 
-template <int D, int TX, int TY, int TZ>
-static hipError_t launch(const TileGeom& tg, const TileArgs& ta, const StepConst& sc,
-                         const float* ebf, const RecPtr& A, const RecPtr& B, const int* ss,
-                         const int* sc_, const int* cc, int* cn, int* oc, int* on, float* patch,
+template <int D, int TX, int TY, int TZ, typename R>
+static hipError_t launch(const TileGeom& tg, const TileArgs& ta, const StepConst<R>& sc,
+                         const R* ebf, const RecPtr& A, const RecPtr& B, const int* ss,
+                         const int* sc_, const int* cc, int* cn, int* oc, int* on, R* patch,
                          int* grow, hipStream_t s) {
   constexpr int NT = tile_threads_for<D, TX, TY, TZ>();
-  k_tile<D, TX, TY, TZ, NT><<<tg.ntiles, NT, lds_bytes_for<D, TX, TY, TZ>(), s>>>(
+  k_tile<D, TX, TY, TZ, NT, R><<<tg.ntiles, NT, lds_bytes_for<D, TX, TY, TZ, R>(), s>>>(
       ta, sc, ebf, A, B, ss, sc_, cc, cn, oc, on, patch, grow);
   return hipGetLastError();
 }
 
-// shapes with a compiled kernel; tile_geom_init accepts only these
+// shapes with a compiled kernel; tile_geom_init accepts only these (and only
+// those whose LDS fits, see tile_lds_bytes)
 bool tile_shape_supported(int D, int tx, int ty, int tz) {
   if (D == 3) return (tx == 8 && ty == 8 && tz == 8) || (tx == 8 && ty == 8 && tz == 4) ||
                      (tx == 16 && ty == 8 && tz == 4);
   return (tx == 16 && ty == 16) || (tx == 32 && ty == 16);
 }
 
-int tile_lds_bytes(int D, int tx, int ty, int tz) {
+template <typename R>
+static int lds_bytes_r(int D, int tx, int ty, int tz) {
   if (D == 3) {
-    if (tx == 8 && ty == 8 && tz == 8) return lds_bytes_for<3, 8, 8, 8>();
-    if (tx == 8 && ty == 8 && tz == 4) return lds_bytes_for<3, 8, 8, 4>();
-    return lds_bytes_for<3, 16, 8, 4>();
+    if (tx == 8 && ty == 8 && tz == 8) return lds_bytes_for<3, 8, 8, 8, R>();
+    if (tx == 8 && ty == 8 && tz == 4) return lds_bytes_for<3, 8, 8, 4, R>();
+    return lds_bytes_for<3, 16, 8, 4, R>();
   }
-  if (tx == 16) return lds_bytes_for<2, 16, 16, 1>();
-  return lds_bytes_for<2, 32, 16, 1>();
+  if (tx == 16) return lds_bytes_for<2, 16, 16, 1, R>();
+  return lds_bytes_for<2, 32, 16, 1, R>();
+}
+
+int tile_lds_bytes(int D, int tx, int ty, int tz, int real_bytes) {
+  return real_bytes == 8 ? lds_bytes_r<double>(D, tx, ty, tz) : lds_bytes_r<float>(D, tx, ty, tz);
 }
 
 int tile_threads(int tx, int ty, int tz) { return tx * ty * tz; }
 
-hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepConst& sc,
-                              const float* ebf, const RecPtr& A, const RecPtr& B,
+template <typename R>
+hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepConst<R>& sc,
+                              const R* ebf, const RecPtr& A, const RecPtr& B,
                               const int* seg_start, const int* seg_cap, const int* cnt_cur,
-                              int* cnt_next, int* ovf_cur, int* ovf_next, float* patch,
-                              int* grow, hipStream_t s) {
+                              int* cnt_next, int* ovf_cur, int* ovf_next, R* patch, int* grow,
+                              hipStream_t s) {
   TileArgs ta{tg.nt[0], tg.nt[1], tg.bx, tg.by, g.n1p, g.n2p, g.n3p, g.ncell};
   const int* T = tg.T;
-#define ALADYN_TILE_CASE(DD, X, Y, Z)                                                        \
-  if (g.ndim == DD && T[0] == X && T[1] == Y && (DD == 2 || T[2] == Z))                      \
-    return launch<DD, X, Y, Z>(tg, ta, sc, ebf, A, B, seg_start, seg_cap, cnt_cur, cnt_next, \
-                               ovf_cur, ovf_next, patch, grow, s);
+#define ALADYN_TILE_CASE(DD, X, Y, Z)                                                           \
+  if (g.ndim == DD && T[0] == X && T[1] == Y && (DD == 2 || T[2] == Z))                         \
+    return launch<DD, X, Y, Z, R>(tg, ta, sc, ebf, A, B, seg_start, seg_cap, cnt_cur, cnt_next, \
+                                  ovf_cur, ovf_next, patch, grow, s);
   ALADYN_TILE_CASE(3, 8, 8, 8)
   ALADYN_TILE_CASE(3, 8, 8, 4)
-  ALADYN_TILE_CASE(3, 16, 8, 4)
+  if constexpr (sizeof(R) == 4) {  // the FP64 patches of this shape exceed the LDS
+    ALADYN_TILE_CASE(3, 16, 8, 4)
+  }
   ALADYN_TILE_CASE(2, 16, 16, 1)
   ALADYN_TILE_CASE(2, 32, 16, 1)
 #undef ALADYN_TILE_CASE
   return hipErrorInvalidConfiguration;
 }
+
+template hipError_t launch_tile_kernel<float>(const TileGeom&, const DevGrid&,
+                                              const StepConst<float>&, const float*,
+                                              const RecPtr&, const RecPtr&, const int*, const int*,
+                                              const int*, int*, int*, int*, float*, int*,
+                                              hipStream_t);
+template hipError_t launch_tile_kernel<double>(const TileGeom&, const DevGrid&,
+                                               const StepConst<double>&, const double*,
+                                               const RecPtr&, const RecPtr&, const int*,
+                                               const int*, const int*, int*, int*, int*, double*,
+                                               int*, hipStream_t);
 
 }  // namespace aladyn

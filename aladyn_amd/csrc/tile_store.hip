@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "tile_device.hpp"
@@ -13,13 +14,14 @@ namespace aladyn {
 
 // tile_kernel.hip
 bool tile_shape_supported(int D, int tx, int ty, int tz);
-int tile_lds_bytes(int D, int tx, int ty, int tz);
+int tile_lds_bytes(int D, int tx, int ty, int tz, int real_bytes);
 int tile_threads(int tx, int ty, int tz);
-hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepConst& sc,
-                              const float* ebf, const RecPtr& A, const RecPtr& B,
+template <typename R>
+hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepConst<R>& sc,
+                              const R* ebf, const RecPtr& A, const RecPtr& B,
                               const int* seg_start, const int* seg_cap, const int* cnt_cur,
-                              int* cnt_next, int* ovf_cur, int* ovf_next, float* patch,
-                              int* grow, hipStream_t s);
+                              int* cnt_next, int* ovf_cur, int* ovf_next, R* patch, int* grow,
+                              hipStream_t s);
 
 // ------------------------------------------------------------ geometry
 static int bits_for(int n) {  // bits to hold values 0..n
@@ -62,15 +64,17 @@ int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg) {
   if (tg->bx + tg->by + bz > 31) return 2;
   tg->block = tile_threads(tg->T[0], tg->T[1], tg->T[2]);
   tg->sort_every = cfg.sort_every > 0 ? cfg.sort_every : 16;
-  tg->lds_bytes = tile_lds_bytes(D, tg->T[0], tg->T[1], tg->T[2]);
+  tg->rbytes = cfg.precision == ALADYN_PREC_F64 ? 8 : 4;
+  tg->lds_bytes = tile_lds_bytes(D, tg->T[0], tg->T[1], tg->T[2], tg->rbytes);
   if (tg->lds_bytes > 160 * 1024) return 3;
   return 0;
 }
 
-struct GlobalAcc32 {
-  float* jc;
+template <typename R>
+struct GlobalAcc {
+  R* jc;
   const DevGrid* g;
-  __device__ void operator()(int c, int i, int j, int k, float v) const {
+  __device__ void operator()(int c, int i, int j, int k, R v) const {
     unsafeAtomicAdd(jc + (int64_t)c * g->ncell + fofs(*g, i, j, k), v);
   }
 };
@@ -79,28 +83,28 @@ struct GlobalAcc32 {
 // Particles whose destination segment was full: gather from HBM, deposit
 // with global atomics (after the fold has written jc), re-bin like the tile
 // kernel.  Runs every step; usually there is nothing to do.
-template <int D>
+template <int D, typename R>
 __global__ __launch_bounds__(256) void k_ovf_particles(
-    TileGeom tg, DevGrid g, StepConst sc, const float* __restrict__ ebf, RecPtr A, RecPtr B,
+    TileGeom tg, DevGrid g, StepConst<R> sc, const R* __restrict__ ebf, RecPtr A, RecPtr B,
     const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
     const int* __restrict__ ovf_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_next,
-    float* __restrict__ jc, int* __restrict__ grow) {
+    R* __restrict__ jc, int* __restrict__ grow) {
   const int cnt = *ovf_cur;
-  const GlobalAcc32 acc{jc, &g};
+  const GlobalAcc<R> acc{jc, &g};
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < cnt; n += gridDim.x * blockDim.x) {
-    Rec q;
-    load_rec<D>(A, sc.ovf_base + n, q, tg.bx, tg.by);
+    Rec<R> q;
+    load_rec<D, R>(A, sc.ovf_base + n, q, tg.bx, tg.by);
     const int i0 = q.N[0] - 1, j0 = q.N[1] - 1, k0 = D == 3 ? q.N[2] - 1 : 1;
-    const auto F = [&](int c, int i1, int j1, int k1) -> float {
+    const auto F = [&](int c, int i1, int j1, int k1) -> R {
       return ebf[(int64_t)c * g.ncell + fofs(g, i0 + i1, j0 + j1, D == 3 ? k0 + k1 : 1)];
     };
-    float ef[6];
-    gather<D>(q.s, F, ef);
+    R ef[6];
+    gather<D, R>(q.s, F, ef);
     const int io[3] = {q.N[0], q.N[1], q.N[2]};
-    const float so[3] = {q.s[0], q.s[1], q.s[2]};
-    const float dtg = push<D>(q.p, ef, sc.qch, sc.alp, sc.dt, sc.initial);
-    if (!move_and_test<D>(q, dtg, sc.inv, sc.nn)) continue;
-    esirkepov<D, float>(io, so, q.N, q.s, sc.qch * q.w, acc);
+    const R so[3] = {q.s[0], q.s[1], q.s[2]};
+    const R dtg = push<D, R>(q.p, ef, sc.qch, sc.alp, sc.dt, sc.initial);
+    if (!move_and_test<D, R>(q, dtg, sc.inv, sc.nn)) continue;
+    esirkepov<D, R>(io, so, q.N, q.s, (R)(sc.qch * q.w), acc);
     const int dtile = (q.N[0] - 3) / tg.T[0] +
                       tg.nt[0] * ((q.N[1] - 3) / tg.T[1] + tg.nt[1] * (D == 3 ? (q.N[2] - 3) / tg.T[2] : 0));
     const int slot = reserve_slot(dtile, cnt_next);
@@ -109,15 +113,16 @@ __global__ __launch_bounds__(256) void k_ovf_particles(
       dst = sc.ovf_base + atomicAdd(ovf_next, 1);
       grow[dtile] = 1;
     }
-    store_rec<D>(B, dst, q, tg.bx, tg.by);
+    store_rec<D, R>(B, dst, q, tg.bx, tg.by);
   }
 }
 
 // ------------------------------------------------------------ fold kernel
 // jc(node) [+]= sum over the tile patches covering node (at most two tiles
 // per axis since T >= 4), compensated (Kahan) summation.
-__global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g, const float* __restrict__ patch,
-                                              float* __restrict__ jc, int accumulate) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g, const R* __restrict__ patch,
+                                              R* __restrict__ jc, int accumulate) {
   const int D = g.ndim;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.ncell;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -146,15 +151,15 @@ __global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g, const floa
       }
     }
     for (int c = 0; c < D; c++) {
-      float sum = accumulate ? jc[(int64_t)c * g.ncell + t] : 0.f, comp = 0.f;
+      R sum = accumulate ? jc[(int64_t)c * g.ncell + t] : R(0), comp = R(0);
       for (int a = 0; a < nt[2]; a++)
         for (int b = 0; b < nt[1]; b++)
           for (int e = 0; e < nt[0]; e++) {
             const int tile = tl[0][e] + tg.nt[0] * (tl[1][b] + tg.nt[1] * tl[2][a]);
             const int l = loc[0][e] + tg.J[0] * (loc[1][b] + tg.J[1] * loc[2][a]);
-            const float v = patch[((int64_t)tile * D + c) * tg.pvol + l];
-            const float y = v - comp;
-            const float s2 = sum + y;
+            const R v = patch[((int64_t)tile * D + c) * tg.pvol + l];
+            const R y = v - comp;
+            const R s2 = sum + y;
             comp = (s2 - sum) - y;
             sum = s2;
           }
@@ -248,6 +253,7 @@ __global__ __launch_bounds__(kB) void k_scan_tiles(const int* v_in, int n, int* 
 }
 
 // live (non-hole) records per tile: live[t], and the overflow count at [ntiles]
+template <typename R>
 __global__ __launch_bounds__(kB) void k_count_live(RecPtr A, const int* seg_start, const int* seg_cap,
                                                    const int* cnt, int ntiles, const int* ovf_cnt,
                                                    int* live) {
@@ -259,14 +265,14 @@ __global__ __launch_bounds__(kB) void k_count_live(RecPtr A, const int* seg_star
   }
   const int n = min(cnt[t], seg_cap[t]);
   int c = 0;
-  for (int q = threadIdx.x; q < n; q += kB) c += rec_cell(A, (int64_t)seg_start[t] + q) != kHole;
+  for (int q = threadIdx.x; q < n; q += kB) c += rec_cell<R>(A, (int64_t)seg_start[t] + q) != kHole;
   int total;
   block_scan_excl(c, sh, total);
   if (threadIdx.x == 0) live[t] = total;
 }
 
 // live records of A (tile segments in order, then the overflow list) -> U
-template <int D>
+template <int D, typename R>
 __global__ __launch_bounds__(kB) void k_gather_live(RecPtr A, RecPtr U, const int* seg_start,
                                                     const int* seg_cap, const int* cnt,
                                                     const int* off, int ntiles, int64_t ovf_base,
@@ -278,35 +284,35 @@ __global__ __launch_bounds__(kB) void k_gather_live(RecPtr A, RecPtr U, const in
     int dst = off[t];
     for (int q0 = 0; q0 < n; q0 += kB) {
       const int q = q0 + threadIdx.x;
-      const bool ok = q < n && rec_cell(A, (int64_t)seg_start[t] + q) != kHole;
+      const bool ok = q < n && rec_cell<R>(A, (int64_t)seg_start[t] + q) != kHole;
       int total;
       const int ex = block_scan_excl(ok ? 1 : 0, sh, total);
-      if (ok) copy_rec<D>(A, (int64_t)seg_start[t] + q, U, (int64_t)dst + ex);
+      if (ok) copy_rec<R>(A, (int64_t)seg_start[t] + q, U, (int64_t)dst + ex);
       dst += total;
     }
   } else {
     const int n = *ovf_cnt;
     const int64_t o0 = off[ntiles];
     for (int q = (t - ntiles) * blockDim.x + threadIdx.x; q < n; q += (gridDim.x - ntiles) * blockDim.x)
-      copy_rec<D>(A, ovf_base + q, U, o0 + q);
+      copy_rec<R>(A, ovf_base + q, U, o0 + q);
   }
 }
 
-template <int D>
+template <int D, typename R>
 __global__ void k_hist(TileGeom tg, RecPtr U, int64_t n, int* hcount) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x)
-    (void)reserve_slot(tile_of(tg, rec_cell(U, q), D), hcount);
+    (void)reserve_slot(tile_of(tg, rec_cell<R>(U, q), D), hcount);
 }
 
-template <int D>
+template <int D, typename R>
 __global__ void k_scatter(TileGeom tg, RecPtr U, int64_t n, RecPtr A, const int* seg_start,
                           int* fillc) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
-    const int t = tile_of(tg, rec_cell(U, q), D);
+    const int t = tile_of(tg, rec_cell<R>(U, q), D);
     const int slot = reserve_slot(t, fillc);
-    copy_rec<D>(U, q, A, (int64_t)seg_start[t] + slot);
+    copy_rec<R>(U, q, A, (int64_t)seg_start[t] + slot);
   }
 }
 
@@ -316,15 +322,16 @@ __global__ void k_scatter(TileGeom tg, RecPtr U, int64_t n, RecPtr A, const int*
 // (an exact change of the packed node index) and drops the records now at or
 // behind the box's left face, x <= xmin: node < 3, or node 3 with s <= 0
 // (the test of reset_all_part_dist, mpi_part_interface.f90:494-498).
+template <typename R>
 __device__ inline int shifted_cell(const TileGeom& tg, const RecPtr& A, int64_t src, int cell,
                                    int shx) {
   if (shx == 0 || cell == kHole) return cell;
   const int i = (cell & ((1 << tg.bx) - 1)) - shx;
-  if (i < 3 || (i == 3 && !(rec_s(A, src, 0) > 0.f))) return kHole;
+  if (i < 3 || (i == 3 && !(rec_s<R>(A, src, 0) > 0.f))) return kHole;
   return cell - shx;
 }
 
-template <int D>
+template <int D, typename R>
 __global__ __launch_bounds__(kB) void k_hist_seg(TileGeom tg, RecPtr A, const int* seg_start,
                                                  const int* seg_cap, const int* cnt,
                                                  int64_t ovf_base, const int* ovf_cnt,
@@ -334,20 +341,20 @@ __global__ __launch_bounds__(kB) void k_hist_seg(TileGeom tg, RecPtr A, const in
     const int n = min(cnt[t], seg_cap[t]);
     for (int q = threadIdx.x; q < n; q += kB) {
       const int64_t src = (int64_t)seg_start[t] + q;
-      const int cell = shifted_cell(tg, A, src, rec_cell(A, src), shx);
+      const int cell = shifted_cell<R>(tg, A, src, rec_cell<R>(A, src), shx);
       if (cell != kHole) (void)reserve_slot(tile_of(tg, cell, D), hcount);
     }
   } else {
     const int n = *ovf_cnt;
     for (int q = (t - tg.ntiles) * kB + threadIdx.x; q < n; q += (gridDim.x - tg.ntiles) * kB) {
-      const int cell = shifted_cell(tg, A, ovf_base + q, rec_cell(A, ovf_base + q), shx);
+      const int cell = shifted_cell<R>(tg, A, ovf_base + q, rec_cell<R>(A, ovf_base + q), shx);
       if (cell != kHole) (void)reserve_slot(tile_of(tg, cell, D), hcount);
     }
   }
 }
 
 // ... and moves every live record of A into its tile's new segment of U.
-template <int D>
+template <int D, typename R>
 __global__ __launch_bounds__(kB) void k_scatter_seg(TileGeom tg, RecPtr A, const int* seg_start,
                                                     const int* seg_cap, const int* cnt,
                                                     int64_t ovf_base, const int* ovf_cnt, RecPtr U,
@@ -357,29 +364,29 @@ __global__ __launch_bounds__(kB) void k_scatter_seg(TileGeom tg, RecPtr A, const
     const int n = min(cnt[t], seg_cap[t]);
     for (int q = threadIdx.x; q < n; q += kB) {
       const int64_t src = (int64_t)seg_start[t] + q;
-      const int cell = shifted_cell(tg, A, src, rec_cell(A, src), shx);
+      const int cell = shifted_cell<R>(tg, A, src, rec_cell<R>(A, src), shx);
       if (cell == kHole) continue;
       const int d = tile_of(tg, cell, D);
       const int64_t dst = (int64_t)new_start[d] + reserve_slot(d, fillc);
-      copy_rec<D>(A, src, U, dst);
-      set_rec_cell(U, dst, cell);
+      copy_rec<R>(A, src, U, dst);
+      set_rec_cell<R>(U, dst, cell);
     }
   } else {
     const int n = *ovf_cnt;
     for (int q = (t - tg.ntiles) * kB + threadIdx.x; q < n; q += (gridDim.x - tg.ntiles) * kB) {
       const int64_t src = ovf_base + q;
-      const int cell = shifted_cell(tg, A, src, rec_cell(A, src), shx);
+      const int cell = shifted_cell<R>(tg, A, src, rec_cell<R>(A, src), shx);
       if (cell == kHole) continue;
       const int d = tile_of(tg, cell, D);
       const int64_t dst = (int64_t)new_start[d] + reserve_slot(d, fillc);
-      copy_rec<D>(A, src, U, dst);
-      set_rec_cell(U, dst, cell);
+      copy_rec<R>(A, src, U, dst);
+      set_rec_cell<R>(U, dst, cell);
     }
   }
 }
 
 // reference layout part(ld, 2D+1) -> records (ids = id0 + input index)
-template <int D>
+template <int D, typename R>
 __global__ void k_convert_in(DevGrid g, int bx, int by, const double* part, int64_t ld,
                              int64_t np, RecPtr U, int64_t id0 = 0) {
   const double mn[3] = {g.xmn, g.ymn, g.zmn};
@@ -387,59 +394,71 @@ __global__ void k_convert_in(DevGrid g, int bx, int by, const double* part, int6
   for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < np;
        n += (int64_t)gridDim.x * blockDim.x) {
     int N[3] = {1, 1, 1};
-    float sv[3] = {0.f, 0.f, 0.f}, pv[3] = {0.f, 0.f, 0.f};
+    R sv[3] = {R(0), R(0), R(0)}, pv[3] = {R(0), R(0), R(0)};
 #pragma unroll
     for (int d = 0; d < D; d++) {
       double s;
       ref_node(part[d * ld + n], mn[d], inv[d], N[d], s);
-      float sf = (float)s;
-      if (sf >= 0.5f) {  // keep the [-1/2, 1/2) convention after rounding
+      R sf = (R)s;
+      if (sf >= R(0.5)) {  // keep the [-1/2, 1/2) convention after rounding
         N[d] += 1;
-        sf -= 1.f;
+        sf -= R(1);
       }
       sv[d] = sf;
-      pv[d] = (float)part[(D + d) * ld + n];
+      pv[d] = (R)part[(D + d) * ld + n];
     }
     const int cell = N[0] | (N[1] << bx) | (D == 3 ? (N[2] << (bx + by)) : 0);
     const unsigned long long b = (unsigned long long)__double_as_longlong(part[2 * D * ld + n]);
-    float4* o = rec_at(U, n);
-    o[0] = make_float4(__int_as_float(cell), sv[0], sv[1], sv[2]);
-    o[1] = make_float4(pv[0], pv[1], pv[2], __uint_as_float((unsigned)(b >> 32)));
+    put_rec<D, R>(U, n, cell, sv, pv, __uint_as_float((unsigned)(b >> 32)));
     if (U.id) U.id[n] = (int)(id0 + n);
   }
 }
 
 // records -> reference layout (weights repacked on the host)
-template <int D>
+template <int D, typename R>
 __global__ void k_convert_out(DevGrid g, int bx, int by, RecPtr U, int64_t n, double* out,
                               int64_t ld) {
   const double mn[3] = {g.xmn, g.ymn, g.zmn};
   const double dd[3] = {g.dx, g.dy, g.dz};
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
-    const int cell = rec_cell(U, q);
+    const int cell = rec_cell<R>(U, q);
     const int N[3] = {cell & ((1 << bx) - 1), (cell >> bx) & ((1 << by) - 1),
                       D == 3 ? (cell >> (bx + by)) : 1};
 #pragma unroll
     for (int d = 0; d < D; d++) {
-      out[d * ld + q] = mn[d] + ((double)(N[d] - 3) + (double)rec_s(U, q, d)) * dd[d];
-      out[(D + d) * ld + q] = (double)rec_p(U, q, d);
+      out[d * ld + q] = mn[d] + ((double)(N[d] - 3) + (double)rec_s<R>(U, q, d)) * dd[d];
+      out[(D + d) * ld + q] = (double)rec_p<R>(U, q, d);
     }
-    out[2 * D * ld + q] = (double)rec_w(U, q);
+    out[2 * D * ld + q] = (double)rec_w<R>(U, q);
   }
 }
 
-// bytes of one record slot: the 32-byte record plus the tracked id
-static size_t rec_bytes(bool track_ids) { return 32 + (track_ids ? 4 : 0); }
-
-static void set_ptrs(TileStore* ts, int b, int /*D*/) {
-  // layout of one buffer: alloc records of 32 bytes, then alloc ids
-  char* m = (char*)ts->mem[b];
-  RecPtr& r = ts->buf[b];
-  r.r = (float4*)m;
-  r.id = ts->track_ids ? (int*)(m + 32 * ts->alloc) : nullptr;
+// bytes of one record (RecWord in tile_device.hpp): 32 in the FP32 path,
+// 64 in the FP64 path
+static size_t rec_size(const TileGeom& tg) { return tg.rbytes == 8 ? 64 : 32; }
+// bytes of one record slot: the record plus the tracked id
+static size_t rec_bytes(const TileGeom& tg, bool track_ids) {
+  return rec_size(tg) + (track_ids ? 4 : 0);
 }
 
+static void set_ptrs(TileStore* ts, const TileGeom& tg, int b) {
+  // layout of one buffer: alloc records, then alloc ids
+  char* m = (char*)ts->mem[b];
+  RecPtr& r = ts->buf[b];
+  r.r = m;
+  r.id = ts->track_ids ? (int*)(m + rec_size(tg) * ts->alloc) : nullptr;
+}
+
+// Calls f(dim, real) with dim a std::integral_constant<int, D> and real a
+// value of the record's real type, for the store's dimension and precision.
+template <typename F>
+static int with_dr(const TileGeom& tg, int D, F&& f) {
+  using D2 = std::integral_constant<int, 2>;
+  using D3 = std::integral_constant<int, 3>;
+  if (D == 2) return tg.rbytes == 8 ? f(D2{}, 0.0) : f(D2{}, 0.0f);
+  return tg.rbytes == 8 ? f(D3{}, 0.0) : f(D3{}, 0.0f);
+}
 #define TS_CK(call)                                                       \
   do {                                                                    \
     hipError_t e_ = (call);                                               \
@@ -472,13 +491,13 @@ void tile_store_free(TileStore* ts) {
 
 // Counting sort of the n records of U (stored in buffer 1-cur) into buffer
 // cur with the re-layout capacities of k_scan_tiles.
-template <int D>
+template <int D, typename R>
 static int sort_into_cur(TileStore* ts, const TileGeom& tg, int64_t n, hipStream_t s,
                          std::string* msg) {
   const int cur = ts->cur, oth = 1 - cur;
   int* h = ts->count[cur];
   fill<int>(h, tg.ntiles, 0, s);
-  if (n > 0) k_hist<D><<<grid_for(n), kB, 0, s>>>(tg, ts->buf[oth], n, h);
+  if (n > 0) k_hist<D, R><<<grid_for(n), kB, 0, s>>>(tg, ts->buf[oth], n, h);
   k_scan_tiles<<<1, kB, 0, s>>>(h, tg.ntiles, ts->seg_start, ts->seg_cap, ts->grow,
                                 seg_budget(ts->ovf_cap, tg.ntiles));
   int total = 0;
@@ -490,7 +509,7 @@ static int sort_into_cur(TileStore* ts, const TileGeom& tg, int64_t n, hipStream
   }
   fill<int>(ts->scratch, tg.ntiles, 0, s);
   if (n > 0)
-    k_scatter<D><<<grid_for(n), kB, 0, s>>>(tg, ts->buf[oth], n, ts->buf[cur], ts->seg_start,
+    k_scatter<D, R><<<grid_for(n), kB, 0, s>>>(tg, ts->buf[oth], n, ts->buf[cur], ts->seg_start,
                                             ts->scratch);
   fill<int>(ts->ovf_count, 2, 0, s);
   TS_CK(hipGetLastError());
@@ -500,10 +519,15 @@ static int sort_into_cur(TileStore* ts, const TileGeom& tg, int64_t n, hipStream
   return 0;
 }
 
+// The path's name in messages.
+static const char* path_name(const TileGeom& tg) {
+  return tg.rbytes == 8 ? "FP64 tile path" : "FP32 tile path";
+}
+
 int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const double* part,
                    int64_t ld, int64_t np, hipStream_t s, bool track_ids, std::string* msg) {
   const int D = g.ndim;
-  // the FP32 record keeps the FP32 weight; charge and part_ind are per species
+  // the record keeps the FP32 weight; charge and part_ind are per species
   int charge = 0, pind = 0;
   for (int64_t n = 0; n < np; n++) {
     unsigned long long b;
@@ -513,24 +537,24 @@ int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const do
       charge = c;
       pind = p;
     } else if (c != charge || p != pind) {
-      *msg = "FP32 tile path needs one charge and part_ind per species";
+      *msg = std::string(path_name(tg)) + " needs one charge and part_ind per species";
       return ALADYN_EINVAL;
     }
   }
   const int64_t need = seg_budget(np, tg.ntiles);  // segments
-  const int64_t alloc = need + (np > 0 ? np : 1);            // + overflow region
+  const int64_t alloc = need + (np > 0 ? np : 1);   // + overflow region
   if (alloc >= (1LL << 31)) {
-    *msg = "FP32 tile path: particle count exceeds the 31-bit slot range";
+    *msg = std::string(path_name(tg)) + ": particle count exceeds the 31-bit slot range";
     return ALADYN_EINVAL;
   }
   if (!ts->loaded || ts->alloc < alloc || ts->track_ids != track_ids) {
     tile_store_free(ts);
     ts->track_ids = track_ids;
     ts->alloc = alloc;
-    const size_t per = rec_bytes(track_ids);
+    const size_t per = rec_bytes(tg, track_ids);
     for (int b = 0; b < 2; b++) {
       TS_CK(hipMalloc(&ts->mem[b], per * (size_t)alloc));
-      set_ptrs(ts, b, D);
+      set_ptrs(ts, tg, b);
       TS_CK(hipMalloc(&ts->count[b], sizeof(int) * (tg.ntiles + 1)));
     }
     TS_CK(hipMalloc(&ts->seg_start, sizeof(int) * (tg.ntiles + 1)));
@@ -559,14 +583,15 @@ int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const do
     for (int c = 0; c < ncol; c++)
       TS_CK(hipMemcpyAsync(dpart + c * np, part + c * ld, (size_t)np * sizeof(double),
                            hipMemcpyHostToDevice, s));
-    if (D == 2)
-      k_convert_in<2><<<grid_for(np), kB, 0, s>>>(g, tg.bx, tg.by, dpart, np, np, ts->buf[1]);
-    else
-      k_convert_in<3><<<grid_for(np), kB, 0, s>>>(g, tg.bx, tg.by, dpart, np, np, ts->buf[1]);
-    TS_CK(hipGetLastError());
   }
   fill<int>(ts->grow, tg.ntiles, 0, s);
-  int r = D == 2 ? sort_into_cur<2>(ts, tg, np, s, msg) : sort_into_cur<3>(ts, tg, np, s, msg);
+  const int r = with_dr(tg, D, [&](auto dc, auto rv) {
+    constexpr int DD = decltype(dc)::value;
+    using R = decltype(rv);
+    if (np > 0)
+      k_convert_in<DD, R><<<grid_for(np), kB, 0, s>>>(g, tg.bx, tg.by, dpart, np, np, ts->buf[1]);
+    return sort_into_cur<DD, R>(ts, tg, np, s, msg);
+  });
   TS_CK(hipStreamSynchronize(s));
   (void)hipFree(dpart);
   if (r) return r;
@@ -575,17 +600,18 @@ int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const do
 }
 
 // gathers the live records of buffer cur into buffer 1-cur; returns count
-template <int D>
+template <int D, typename R>
 static int gather_live(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t* n,
                        std::string* msg) {
   const int cur = ts->cur, oth = 1 - cur;
   int* live = ts->count[oth];  // free between steps
-  k_count_live<<<tg.ntiles + 1, kB, 0, s>>>(ts->buf[cur], ts->seg_start, ts->seg_cap,
-                                            ts->count[cur], tg.ntiles, ts->ovf_count + cur, live);
+  k_count_live<R><<<tg.ntiles + 1, kB, 0, s>>>(ts->buf[cur], ts->seg_start, ts->seg_cap,
+                                               ts->count[cur], tg.ntiles, ts->ovf_count + cur,
+                                               live);
   k_scan_tiles<<<1, kB, 0, s>>>(live, tg.ntiles + 1, ts->scratch, nullptr);
-  k_gather_live<D><<<tg.ntiles + 64, kB, 0, s>>>(ts->buf[cur], ts->buf[oth], ts->seg_start,
-                                                 ts->seg_cap, ts->count[cur], ts->scratch,
-                                                 tg.ntiles, ts->ovf_base, ts->ovf_count + cur);
+  k_gather_live<D, R><<<tg.ntiles + 64, kB, 0, s>>>(ts->buf[cur], ts->buf[oth], ts->seg_start,
+                                                    ts->seg_cap, ts->count[cur], ts->scratch,
+                                                    tg.ntiles, ts->ovf_base, ts->ovf_count + cur);
   TS_CK(hipGetLastError());
   int tot = 0;
   TS_CK(hipMemcpyAsync(&tot, ts->scratch + tg.ntiles + 1, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -598,21 +624,22 @@ static int gather_live(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t
 // count per destination tile from the segments, fresh capacities, scatter.
 // Fully asynchronous: the capacities fit the segment area by construction
 // (live particles never exceed the count given to put).
-template <int D>
+template <int D, typename R>
 static int relayout_seg(TileStore* ts, const TileGeom& tg, hipStream_t s, std::string* msg,
                         int shx = 0) {
   const int cur = ts->cur, oth = 1 - cur;
   int* h = ts->count[oth];  // per-tile counts of the new layout
   fill<int>(h, tg.ntiles, 0, s);
-  k_hist_seg<D><<<tg.ntiles + 64, kB, 0, s>>>(tg, ts->buf[cur], ts->seg_start, ts->seg_cap,
-                                              ts->count[cur], ts->ovf_base, ts->ovf_count + cur, h,
-                                              shx);
+  k_hist_seg<D, R><<<tg.ntiles + 64, kB, 0, s>>>(tg, ts->buf[cur], ts->seg_start, ts->seg_cap,
+                                                 ts->count[cur], ts->ovf_base, ts->ovf_count + cur,
+                                                 h, shx);
   k_scan_tiles<<<1, kB, 0, s>>>(h, tg.ntiles, ts->seg_start2, ts->seg_cap2, ts->grow,
                                 seg_budget(ts->ovf_cap, tg.ntiles));
   fill<int>(ts->scratch, tg.ntiles, 0, s);
-  k_scatter_seg<D><<<tg.ntiles + 64, kB, 0, s>>>(tg, ts->buf[cur], ts->seg_start, ts->seg_cap,
-                                                 ts->count[cur], ts->ovf_base, ts->ovf_count + cur,
-                                                 ts->buf[oth], ts->seg_start2, ts->scratch, shx);
+  k_scatter_seg<D, R><<<tg.ntiles + 64, kB, 0, s>>>(tg, ts->buf[cur], ts->seg_start, ts->seg_cap,
+                                                    ts->count[cur], ts->ovf_base,
+                                                    ts->ovf_count + cur, ts->buf[oth],
+                                                    ts->seg_start2, ts->scratch, shx);
   fill<int>(ts->ovf_count, 2, 0, s);
   TS_CK(hipGetLastError());
   std::swap(ts->seg_start, ts->seg_start2);
@@ -626,7 +653,9 @@ static int relayout_seg(TileStore* ts, const TileGeom& tg, hipStream_t s, std::s
 int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
                         std::string* msg) {
   if (!ts->loaded) return 0;
-  return g.ndim == 2 ? relayout_seg<2>(ts, tg, s, msg) : relayout_seg<3>(ts, tg, s, msg);
+  return with_dr(tg, g.ndim, [&](auto dc, auto rv) {
+    return relayout_seg<decltype(dc)::value, decltype(rv)>(ts, tg, s, msg);
+  });
 }
 
 // Moves the box window_cells cells to the right (window.f90:276-347): every
@@ -635,22 +664,23 @@ int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hip
 int tile_store_window(TileStore* ts, const TileGeom& tg, const DevGrid& g, int window_cells,
                       hipStream_t s, std::string* msg) {
   if (!ts->loaded || window_cells == 0) return 0;
-  return g.ndim == 2 ? relayout_seg<2>(ts, tg, s, msg, window_cells)
-                     : relayout_seg<3>(ts, tg, s, msg, window_cells);
+  return with_dr(tg, g.ndim, [&](auto dc, auto rv) {
+    return relayout_seg<decltype(dc)::value, decltype(rv)>(ts, tg, s, msg, window_cells);
+  });
 }
 
 // Re-allocates the two buffers for new_cap particles, keeping the live ones.
-template <int D>
+template <int D, typename R>
 static int grow_store(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t new_cap,
                       std::string* msg) {
   int64_t n = 0;
-  if (int r = gather_live<D>(ts, tg, s, &n, msg)) return r;  // dense, in buffer 1-cur
+  if (int r = gather_live<D, R>(ts, tg, s, &n, msg)) return r;  // dense, in buffer 1-cur
   const int64_t alloc = seg_budget(new_cap, tg.ntiles) + new_cap;
   if (alloc >= (1LL << 31)) {
-    *msg = "FP32 tile path: particle count exceeds the 31-bit slot range";
+    *msg = std::string(path_name(tg)) + ": particle count exceeds the 31-bit slot range";
     return ALADYN_EINVAL;
   }
-  const size_t per = rec_bytes(ts->track_ids);
+  const size_t per = rec_bytes(tg, ts->track_ids);
   void* fresh[2] = {nullptr, nullptr};
   for (int b = 0; b < 2; b++) {
     if (hipMalloc(&fresh[b], per * (size_t)alloc) != hipSuccess) {
@@ -664,11 +694,11 @@ static int grow_store(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t 
   ts->mem[0] = fresh[0];
   ts->mem[1] = fresh[1];
   ts->alloc = alloc;
-  set_ptrs(ts, 0, D);
-  set_ptrs(ts, 1, D);
+  set_ptrs(ts, tg, 0);
+  set_ptrs(ts, tg, 1);
   const RecPtr& nu = ts->buf[1];
   if (n > 0) {
-    TS_CK(hipMemcpyAsync(nu.r, old.r, (size_t)n * 32, hipMemcpyDeviceToDevice, s));
+    TS_CK(hipMemcpyAsync(nu.r, old.r, (size_t)n * rec_size(tg), hipMemcpyDeviceToDevice, s));
     if (nu.id) TS_CK(hipMemcpyAsync(nu.id, old.id, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
   }
   TS_CK(hipStreamSynchronize(s));
@@ -677,14 +707,14 @@ static int grow_store(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t 
   ts->ovf_cap = new_cap;
   ts->ovf_base = alloc - new_cap;
   ts->cur = 0;
-  return sort_into_cur<D>(ts, tg, n, s, msg);
+  return sort_into_cur<D, R>(ts, tg, n, s, msg);
 }
 
 // Appends n particles (reference layout, inside the box) to the species:
 // they are converted into the overflow list of the current buffer and the
 // store is re-laid out, so the next step moves them in their tiles
 // (add_particles / particles_inject, window.f90:39-270).
-template <int D>
+template <int D, typename R>
 static int inject_impl(TileStore* ts, const TileGeom& tg, const DevGrid& g, const double* part,
                        int64_t ld, int64_t n, hipStream_t s, std::string* msg) {
   for (int64_t q = 0; q < n; q++) {
@@ -692,14 +722,15 @@ static int inject_impl(TileStore* ts, const TileGeom& tg, const DevGrid& g, cons
     std::memcpy(&b, part + 2 * D * ld + q, 8);
     const int c = (int)(short)(b & 0xffff), pi = (int)(short)((b >> 16) & 0xffff);
     if (c != ts->charge || pi != ts->part_ind) {
-      *msg = "FP32 tile path: injected particles need the species' charge and part_ind";
+      *msg = std::string(path_name(tg)) +
+             ": injected particles need the species' charge and part_ind";
       return ALADYN_EINVAL;
     }
   }
   int64_t live = 0;
   if (int r = tile_store_count(ts, tg, s, &live, msg)) return r;
   if (live + n > ts->ovf_cap)
-    if (int r = grow_store<D>(ts, tg, s, std::max(2 * ts->ovf_cap, live + n), msg)) return r;
+    if (int r = grow_store<D, R>(ts, tg, s, std::max(2 * ts->ovf_cap, live + n), msg)) return r;
   const int cur = ts->cur;
   int ovf = 0;
   TS_CK(hipMemcpyAsync(&ovf, ts->ovf_count + cur, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -713,24 +744,25 @@ static int inject_impl(TileStore* ts, const TileGeom& tg, const DevGrid& g, cons
                          hipMemcpyHostToDevice, s));
   RecPtr dst = ts->buf[cur];
   const int64_t o = ts->ovf_base + ovf;
-  dst.r += 2 * o;
+  dst.r = (char*)dst.r + rec_size(tg) * (size_t)o;
   if (dst.id) dst.id += o;
-  k_convert_in<D><<<grid_for(n), kB, 0, s>>>(g, tg.bx, tg.by, dpart, n, n, dst, ts->next_id);
+  k_convert_in<D, R><<<grid_for(n), kB, 0, s>>>(g, tg.bx, tg.by, dpart, n, n, dst, ts->next_id);
   TS_CK(hipGetLastError());
   ts->next_id += n;
   const int total = ovf + (int)n;
   TS_CK(hipMemcpyAsync(ts->ovf_count + cur, &total, sizeof(int), hipMemcpyHostToDevice, s));
   TS_CK(hipStreamSynchronize(s));
   (void)hipFree(dpart);
-  return relayout_seg<D>(ts, tg, s, msg);
+  return relayout_seg<D, R>(ts, tg, s, msg);
 }
 
 int tile_store_inject(TileStore* ts, const TileGeom& tg, const DevGrid& g, const double* part,
                       int64_t ld, int64_t n, hipStream_t s, bool track_ids, std::string* msg) {
   if (n <= 0) return 0;
   if (!ts->loaded) return tile_store_put(ts, tg, g, part, ld, n, s, track_ids, msg);
-  return g.ndim == 2 ? inject_impl<2>(ts, tg, g, part, ld, n, s, msg)
-                     : inject_impl<3>(ts, tg, g, part, ld, n, s, msg);
+  return with_dr(tg, g.ndim, [&](auto dc, auto rv) {
+    return inject_impl<decltype(dc)::value, decltype(rv)>(ts, tg, g, part, ld, n, s, msg);
+  });
 }
 
 int tile_store_count(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t* np,
@@ -739,8 +771,14 @@ int tile_store_count(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t* 
   if (!ts->loaded) return 0;
   const int cur = ts->cur, oth = 1 - cur;
   int* live = ts->count[oth];
-  k_count_live<<<tg.ntiles + 1, kB, 0, s>>>(ts->buf[cur], ts->seg_start, ts->seg_cap,
-                                            ts->count[cur], tg.ntiles, ts->ovf_count + cur, live);
+  if (tg.rbytes == 8)
+    k_count_live<double><<<tg.ntiles + 1, kB, 0, s>>>(ts->buf[cur], ts->seg_start, ts->seg_cap,
+                                                      ts->count[cur], tg.ntiles,
+                                                      ts->ovf_count + cur, live);
+  else
+    k_count_live<float><<<tg.ntiles + 1, kB, 0, s>>>(ts->buf[cur], ts->seg_start, ts->seg_cap,
+                                                     ts->count[cur], tg.ntiles,
+                                                     ts->ovf_count + cur, live);
   k_scan_tiles<<<1, kB, 0, s>>>(live, tg.ntiles + 1, ts->scratch, nullptr);
   TS_CK(hipGetLastError());
   int tot = 0;
@@ -755,7 +793,9 @@ int tile_store_get(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* 
   if (!ts->loaded) return 0;
   const int D = g.ndim, ncol = 2 * D + 1;
   int64_t n = 0;
-  int r = D == 2 ? gather_live<2>(ts, tg, s, &n, msg) : gather_live<3>(ts, tg, s, &n, msg);
+  int r = with_dr(tg, D, [&](auto dc, auto rv) {
+    return gather_live<decltype(dc)::value, decltype(rv)>(ts, tg, s, &n, msg);
+  });
   if (r) return r;
   if (n == 0) return 0;
   if (ld < n) {
@@ -765,10 +805,11 @@ int tile_store_get(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* 
   double* d = nullptr;
   TS_CK(hipMalloc(&d, (size_t)n * ncol * sizeof(double)));
   const RecPtr& U = ts->buf[1 - ts->cur];
-  if (D == 2)
-    k_convert_out<2><<<grid_for(n), kB, 0, s>>>(g, tg.bx, tg.by, U, n, d, n);
-  else
-    k_convert_out<3><<<grid_for(n), kB, 0, s>>>(g, tg.bx, tg.by, U, n, d, n);
+  with_dr(tg, D, [&](auto dc, auto rv) {
+    k_convert_out<decltype(dc)::value, decltype(rv)>
+        <<<grid_for(n), kB, 0, s>>>(g, tg.bx, tg.by, U, n, d, n);
+    return 0;
+  });
   std::vector<double> h((size_t)n * ncol);
   std::vector<int> ids;
   TS_CK(hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -797,18 +838,23 @@ int tile_store_get(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* 
   return 0;
 }
 
-int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const float* ebf,
-                    float* jc, float* patch, float alp, int initial, bool j_is_zero,
-                    hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg) {
+template <typename R>
+int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf, R* jc,
+                    R* patch, R alp, int initial, bool j_is_zero, hipStream_t s,
+                    hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg) {
   if (!ts->loaded) return 0;
+  if ((int)sizeof(R) != tg.rbytes) {
+    *msg = "tile store: step called with the other precision's arrays";
+    return ALADYN_EINVAL;
+  }
   const int D = g.ndim, cur = ts->cur, nxt = 1 - cur;
-  StepConst sc;
+  StepConst<R> sc;
   sc.alp = alp;
-  sc.qch = (float)ts->charge;
-  sc.dt = (float)g.dt;
-  sc.inv[0] = (float)g.dx_inv;
-  sc.inv[1] = (float)g.dy_inv;
-  sc.inv[2] = (float)g.dz_inv;
+  sc.qch = (R)ts->charge;
+  sc.dt = (R)g.dt;
+  sc.inv[0] = (R)g.dx_inv;
+  sc.inv[1] = (R)g.dy_inv;
+  sc.inv[2] = (R)g.dz_inv;
   sc.nn[0] = g.nx;
   sc.nn[1] = g.ny;
   sc.nn[2] = g.nz;
@@ -817,19 +863,19 @@ int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const f
   fill<int>(ts->count[nxt], tg.ntiles, 0, s);
   fill<int>(ts->ovf_count + nxt, 1, 0, s);
   if (ev_begin) TS_CK(hipEventRecord(ev_begin, s));
-  TS_CK(launch_tile_kernel(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt], ts->seg_start, ts->seg_cap,
-                           ts->count[cur], ts->count[nxt], ts->ovf_count + cur,
-                           ts->ovf_count + nxt, patch, ts->grow, s));
+  TS_CK(launch_tile_kernel<R>(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt], ts->seg_start,
+                              ts->seg_cap, ts->count[cur], ts->count[nxt], ts->ovf_count + cur,
+                              ts->ovf_count + nxt, patch, ts->grow, s));
   if (ev_end) TS_CK(hipEventRecord(ev_end, s));
-  k_fold<<<grid_for(g.ncell, 8192), kB, 0, s>>>(tg, g, patch, jc, j_is_zero ? 0 : 1);
+  k_fold<R><<<grid_for(g.ncell, 8192), kB, 0, s>>>(tg, g, patch, jc, j_is_zero ? 0 : 1);
   if (D == 2)
-    k_ovf_particles<2><<<512, kB, 0, s>>>(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt], ts->seg_start,
-                                          ts->seg_cap, ts->ovf_count + cur, ts->count[nxt],
-                                          ts->ovf_count + nxt, jc, ts->grow);
+    k_ovf_particles<2, R><<<512, kB, 0, s>>>(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt],
+                                             ts->seg_start, ts->seg_cap, ts->ovf_count + cur,
+                                             ts->count[nxt], ts->ovf_count + nxt, jc, ts->grow);
   else
-    k_ovf_particles<3><<<512, kB, 0, s>>>(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt], ts->seg_start,
-                                          ts->seg_cap, ts->ovf_count + cur, ts->count[nxt],
-                                          ts->ovf_count + nxt, jc, ts->grow);
+    k_ovf_particles<3, R><<<512, kB, 0, s>>>(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt],
+                                             ts->seg_start, ts->seg_cap, ts->ovf_count + cur,
+                                             ts->count[nxt], ts->ovf_count + nxt, jc, ts->grow);
   TS_CK(hipGetLastError());
   ts->cur = nxt;
   ts->steps_since_sort++;
@@ -839,7 +885,8 @@ int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const f
   // host waits only for the previous step's slot, so one step stays queued
   // on the GPU while the host decides.
   const int slot = ts->ovf_slot, prev = 1 - slot;
-  TS_CK(hipMemcpyAsync(ts->h_ovf + slot, ts->ovf_count + nxt, sizeof(int), hipMemcpyDeviceToHost, s));
+  TS_CK(hipMemcpyAsync(ts->h_ovf + slot, ts->ovf_count + nxt, sizeof(int), hipMemcpyDeviceToHost,
+                       s));
   TS_CK(hipEventRecord(ts->ovf_ev[slot], s));
   ts->ovf_valid[slot] = true;
   ts->ovf_slot = prev;
@@ -854,20 +901,26 @@ int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const f
   }
   return 0;
 }
+template int tile_store_step<float>(TileStore*, const TileGeom&, const DevGrid&, const float*,
+                                    float*, float*, float, int, bool, hipStream_t, hipEvent_t,
+                                    hipEvent_t, std::string*);
+template int tile_store_step<double>(TileStore*, const TileGeom&, const DevGrid&, const double*,
+                                     double*, double*, double, int, bool, hipStream_t, hipEvent_t,
+                                     hipEvent_t, std::string*);
 
 // ------------------------------------------------------------ diagnostics
-template <int D>
-__global__ void k_grid_charge_f32(TileGeom tg, RecPtr U, int64_t n, float qch, double* rho,
+template <int D, typename R>
+__global__ void k_grid_charge_rec(TileGeom tg, RecPtr U, int64_t n, double qch, double* rho,
                                   DevGrid g) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
-    const int cell = rec_cell(U, q);
+    const int cell = rec_cell<R>(U, q);
     const int N[3] = {cell & ((1 << tg.bx) - 1), (cell >> tg.bx) & ((1 << tg.by) - 1),
                       D == 3 ? (cell >> (tg.bx + tg.by)) : 1};
     double a[3][3] = {{0, 0, 0}, {0, 0, 0}, {1, 0, 0}};
 #pragma unroll
-    for (int d = 0; d < D; d++) qweights<double>((double)rec_s(U, q, d), a[d]);
-    const double wq = (double)(qch * rec_w(U, q));
+    for (int d = 0; d < D; d++) qweights<double>((double)rec_s<R>(U, q, d), a[d]);
+    const double wq = qch * (double)rec_w<R>(U, q);
     const int nk = D == 3 ? 3 : 1;
     for (int k1 = 0; k1 < nk; k1++)
       for (int j1 = 0; j1 < 3; j1++)
@@ -883,45 +936,46 @@ void tile_store_grid_charge(TileStore* ts, const TileGeom& tg, const DevGrid& g,
   std::string msg;
   int64_t n = 0;
   const int D = g.ndim;
-  if ((D == 2 ? gather_live<2>(ts, tg, s, &n, &msg) : gather_live<3>(ts, tg, s, &n, &msg)) || n == 0)
-    return;
-  const RecPtr& U = ts->buf[1 - ts->cur];
-  if (D == 2)
-    k_grid_charge_f32<2><<<grid_for(n), kB, 0, s>>>(tg, U, n, (float)ts->charge, rho, g);
-  else
-    k_grid_charge_f32<3><<<grid_for(n), kB, 0, s>>>(tg, U, n, (float)ts->charge, rho, g);
+  with_dr(tg, D, [&](auto dc, auto rv) {
+    constexpr int DD = decltype(dc)::value;
+    using R = decltype(rv);
+    if (gather_live<DD, R>(ts, tg, s, &n, &msg) || n == 0) return 0;
+    const RecPtr& U = ts->buf[1 - ts->cur];
+    k_grid_charge_rec<DD, R><<<grid_for(n), kB, 0, s>>>(tg, U, n, (double)ts->charge, rho, g);
+    return 0;
+  });
 }
 
 __device__ inline void amax_nonneg(double* addr, double v) {
   atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
 }
 
-template <int D>
-__global__ __launch_bounds__(kB) void k_energy_f32(TileGeom tg, DevGrid g, RecPtr U, int64_t n,
-                                                   float qch, double* out) {
+template <int D, typename R>
+__global__ __launch_bounds__(kB) void k_energy_rec(TileGeom tg, DevGrid g, RecPtr U, int64_t n,
+                                                   double qch, double* out) {
   __shared__ double sh[8][kB / 64];
   double ek[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
     double p[3] = {0, 0, 0};
 #pragma unroll
-    for (int d = 0; d < D; d++) p[d] = (double)rec_p(U, q, d);
-    const double w = (double)rec_w(U, q);
+    for (int d = 0; d < D; d++) p[d] = (double)rec_p<R>(U, q, d);
+    const double w = (double)rec_w<R>(U, q);
     const double gam1 = sqrt(1. + p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) - 1.;
     if constexpr (D == 2) {
       ek[0] += w * p[0];
       ek[1] += w * p[1];
     } else {
-      const int cell = rec_cell(U, q);
+      const int cell = rec_cell<R>(U, q);
       const int Nj = (cell >> tg.bx) & ((1 << tg.by) - 1), Nk = cell >> (tg.bx + tg.by);
-      const double y = g.ymn + ((double)(Nj - 3) + (double)rec_s(U, q, 1)) * g.dy;
-      const double z = g.zmn + ((double)(Nk - 3) + (double)rec_s(U, q, 2)) * g.dz;
+      const double y = g.ymn + ((double)(Nj - 3) + (double)rec_s<R>(U, q, 1)) * g.dy;
+      const double z = g.zmn + ((double)(Nk - 3) + (double)rec_s<R>(U, q, 2)) * g.dz;
       ek[0] += p[0];
       ek[1] += p[1];
       ek[2] += p[2];
       ek[3] += w * (y * p[2] - z * p[1]);
     }
-    ek[5] += (double)(qch * rec_w(U, q));
+    ek[5] += qch * w;
     ek[6] += w * gam1;
     ek[7] = fmax(ek[7], gam1);
   }
@@ -951,13 +1005,14 @@ void tile_store_energy(TileStore* ts, const TileGeom& tg, const DevGrid& g, doub
   std::string msg;
   int64_t n = 0;
   const int D = g.ndim;
-  if ((D == 2 ? gather_live<2>(ts, tg, s, &n, &msg) : gather_live<3>(ts, tg, s, &n, &msg)) || n == 0)
-    return;
-  const RecPtr& U = ts->buf[1 - ts->cur];
-  if (D == 2)
-    k_energy_f32<2><<<grid_for(n, 4096), kB, 0, s>>>(tg, g, U, n, (float)ts->charge, out);
-  else
-    k_energy_f32<3><<<grid_for(n, 4096), kB, 0, s>>>(tg, g, U, n, (float)ts->charge, out);
+  with_dr(tg, D, [&](auto dc, auto rv) {
+    constexpr int DD = decltype(dc)::value;
+    using R = decltype(rv);
+    if (gather_live<DD, R>(ts, tg, s, &n, &msg) || n == 0) return 0;
+    const RecPtr& U = ts->buf[1 - ts->cur];
+    k_energy_rec<DD, R><<<grid_for(n, 4096), kB, 0, s>>>(tg, g, U, n, (double)ts->charge, out);
+    return 0;
+  });
 }
 
 }  // namespace aladyn

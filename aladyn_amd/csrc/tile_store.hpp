@@ -1,13 +1,14 @@
-// tile_store.hpp -- the FP32 particle store of the tile path.
+// tile_store.hpp -- the particle store of the tile path (FP32 and FP64).
 //
-// Particles of one species live in SoA arrays grouped by tile: tile t owns
+// Particles of one species live in records grouped by tile: tile t owns
 // the segment [seg_start[t], seg_start[t] + seg_cap[t]) of which the first
 // count[t] slots are live.  A particle is stored as
 //   cell  int32  packed nearest-node indices (i | j << bx | k << (bx+by)),
 //                Fortran node numbering of ebf (interior starts at 3)
-//   s[d]  FP32   offset from that node, in cells, in [-1/2, 1/2)
-//   p[d]  FP32   momentum p/(mc)
+//   s[d]  R      offset from that node, in cells, in [-1/2, 1/2)
+//   p[d]  R      momentum p/(mc)
 //   w     FP32   weight (the FP32 `wgh` of the packed slot)
+// with R = float (32-byte record) or double (64-byte record)
 // so x = xmn + (i - 3 + s_x)*dx exactly as the reference's local coordinate
 // xx = 3 + (x - xmn)/dx = i + s_x (grid_part_lib.f90:114-116); a moving-window
 // shift is an integer subtraction.  Two such buffers are used in turn: the
@@ -39,17 +40,16 @@ struct TileGeom {
   int block;   // threads per workgroup
   int sort_every;
   int lds_bytes;
+  int rbytes;  // bytes of the record's real type: 4 (FP32 path) or 8 (FP64 path)
 };
 
-// Particle records, 32 bytes each (two 16-byte words, so a record moves
-// with two vector loads or stores): [cell, s0, s1, s2 | p0, p1, p2, w], the
-// packed nearest node as int bits; unused slots (z in 2-D) are 0.
+// Particle records of the tile store: one record per particle, 32 bytes in
+// the FP32 path and 64 in the FP64 path (RecWord in tile_device.hpp gives
+// the layout), so a record moves with two or four 16-byte loads or stores.
 struct RecPtr {
-  float4* r;  // 2 per record
-  int* id;    // optional original index (tests), nullptr when not tracked
+  void* r;  // records
+  int* id;  // optional original index (tests), nullptr when not tracked
 };
-
-__host__ __device__ inline float4* rec_at(const RecPtr& a, int64_t n) { return a.r + 2 * n; }
 
 struct TileStore {
   int64_t alloc = 0;     // records per buffer
@@ -95,10 +95,12 @@ int tile_store_inject(TileStore* ts, const TileGeom& tg, const DevGrid& g, const
 int tile_store_get(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* part, int64_t ld,
                    hipStream_t s, std::string* msg);
 // One species-loop body: gather + push + box test + deposit + re-binning.
-// jc(raw) = (j_is_zero ? 0 : jc) + this species' Esirkepov current.
-int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const float* ebf,
-                    float* jc, float* patch, float alp, int initial, bool j_is_zero,
-                    hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg);
+// jc(raw) = (j_is_zero ? 0 : jc) + this species' Esirkepov current.  R is
+// float or double and must match the precision the geometry was made for.
+template <typename R>
+int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf, R* jc,
+                    R* patch, R alp, int initial, bool j_is_zero, hipStream_t s,
+                    hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg);
 int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
                         std::string* msg);
 void tile_store_grid_charge(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* rho,

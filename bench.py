@@ -7,8 +7,9 @@ n/nc = 0.005, 8 particles per cell on a 2x2x2 sub-grid with seeded jitter
 A step is one full lpf2_evolve: field ghosts, gather, push, removal,
 Esirkepov deposit, current fold/normalisation and the Yee update.
 
-The FP32 tile path is the measured value; the FP64 reference-layout path runs
-the same workload for the FP32/FP64 ratio.  Inputs are resident in HBM before
+The FP32 tile path is the measured value; the FP64 tile path (the same
+kernels on 64-byte double records) runs the same workload for the FP32/FP64
+ratio.  Inputs are resident in HBM before
 timing starts.  `roofline` refers to the dominant kernel (the FP32 particle
 kernel), its duration measured with HIP events on the context stream over the
 timed steps; `cpu_baseline` times the reference's own Fortran gather + push +
@@ -200,7 +201,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(cfg.nx, cfg.ny, cfg.nz),
-            "kernel": "k_tile_particles<3>",
+            "kernel": "k_tile<3, 8x8x4 tiles, float>",
             "kernel_ms": r32["part_ms"],
             "algorithmic_bytes_per_launch": kern_bytes,
         },
@@ -208,8 +209,13 @@ def main():
         "cpu_baseline": cpu,
     }
     if r64:
+        # FP64 particle kernel: 104 B/particle (56 R + 48 W) + gather 48 + J 24 B/cell
+        n64 = 0.5 * (r64["n0"] + r64["n1"])
+        b64 = n64 * 104.0 + cells * (48.0 + 24.0)
+        a64 = b64 / (r64["part_ms"] * 1e-3) / 1e9 if r64["part_ms"] > 0 else 0.0
         out["fp64"] = {"value": r64["pps"], "ms_per_step": r64["ms_per_step"],
-                       "particle_kernel_ms": r64["part_ms"], "field_ms": r64["field_ms"]}
+                       "particle_kernel_ms": r64["part_ms"], "field_ms": r64["field_ms"],
+                       "particle_kernel_gbs": a64, "particle_kernel_frac": a64 / HBM_PEAK_GBS}
         out["fp32_over_fp64"] = r32["pps"] / r64["pps"]
     print(json.dumps(out), flush=True)
 

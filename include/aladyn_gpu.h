@@ -41,8 +41,9 @@ extern "C" {
 #define ALADYN_ESTATE 4   /* call out of order (e.g. step before put) */
 #define ALADYN_ERANGE 5   /* particle outside the box at put_species */
 
-#define ALADYN_PREC_F64 64 /* reference layout, FP64 throughout */
-#define ALADYN_PREC_F32 32 /* tile-sorted SoA, int32 cell + FP32 offset */
+#define ALADYN_PREC_F64 64 /* tile-sorted 64-byte records, FP64 grids and push */
+#define ALADYN_PREC_F32 32 /* tile-sorted 32-byte records (int32 node + FP32
+                              offset), FP32 grids, FP64 current accumulation */
 
 #define ALADYN_MAX_SPECIES 4 /* spec(4) in pstruct_data.f90:39 */
 
@@ -60,12 +61,16 @@ typedef struct aladyn_config {
   double dx, dy, dz;     /* cell sizes (dz ignored in 2-D) */
   double dt;             /* dt_loc */
   double cfl;            /* only used by der_ord 3 */
-  double ompe;           /* 4*pi*rc0*n0_ref*1e-3 (set_init_param.f90:254-255) */
+  double ompe;           /* 4*pi*rc0*1e-3 = 0.035411 (set_init_param.f90:254-255) */
   double lorentz_fact[ALADYN_MAX_SPECIES]; /* 1/m per species */
-  /* FP32 tile path tuning (0 = default / autotuned) */
+  /* tile path tuning (0 = default) */
   int32_t tile[3];       /* tile extent in nodes per axis */
-  int32_t block;         /* threads per workgroup of the particle kernel */
+  int32_t block;         /* unused: the particle kernel runs one thread per
+                            node-cell of the tile */
   int32_t sort_every;    /* steps between full re-layouts of the tile store */
+  /* reserved[0] != 0 keeps each particle's original index, so that
+   * get_species returns particles in put/inject order (costs 4 B/particle of
+   * HBM traffic per step); the rest must be 0 */
   int32_t reserved[5];
 } aladyn_config;
 

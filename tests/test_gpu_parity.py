@@ -1,10 +1,10 @@
 """GPU parity: the HIP library (through the C-ABI) against the C oracle.
 
 Bars (max-norm relative errors):
-  FP64 reference-layout path: one step <= 1e-12 (particles, fields, current),
-      ten steps <= 1e-10, continuity residual <= 1e-12 of max|d rho|.
-  FP32 tile path: one step <= 1e-5 (momenta, fields), continuity <= 1e-5.
-The FP32 store keeps particle identities when track_ids=True so get_species
+  FP64 path: one step <= 1e-12 (particles, fields, current), ten steps
+      <= 1e-10, continuity residual <= 1e-12 of max|d rho|.
+  FP32 path: one step <= 1e-5 (momenta, fields), continuity <= 1e-5.
+The tile store keeps particle identities when track_ids=True so get_species
 returns them in input order (survivors only), like the reference's stable
 compaction in reset_all_part_dist.
 """
