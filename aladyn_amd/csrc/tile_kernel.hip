@@ -10,12 +10,14 @@
 //   2. counting-sort the tile's particles by cell in LDS (chunks of kChunk);
 //   3. split the sorted chunk into equal consecutive slices, one per thread,
 //      and walk them in lock step: gather from LDS, charge multiply,
-//      [back-shift], exact-gamma push, carry, box test; a particle that keeps
-//      its nearest node adds its 2x3x3-per-component Esirkepov current to
-//      register sums of its cell (flushed into the LDS patch when the
-//      thread's slice moves to the next cell), one that changes node is
-//      queued per wave and deposited through the union window when 64 are
-//      waiting; survivors are written to the other buffer: stayers into
+//      [back-shift], exact-gamma push, carry, box test; a particle whose
+//      nearest node moved on at most one axis adds the part of its Esirkepov
+//      current on its old cell's node block to register sums of that cell
+//      (flushed into the LDS patch when the thread's slice moves to the next
+//      cell) and the layer beyond the block, if any, to the LDS patch; one
+//      whose node moved on two or three axes is queued per wave and
+//      deposited through the union window when 64 are waiting; survivors
+//      are written to the other buffer: stayers into
 //      slots reserved per chunk, movers into their new tile with
 //      wave-aggregated atomics, overflow into the global list;
 //   4. write the LDS current patch to HBM for the fold.
@@ -32,7 +34,7 @@ struct TileArgs {
 };
 
 // Union-window Esirkepov deposit (grid_part_connect.f90:979-1060, 1168-1334)
-// for a particle whose nearest node changed on at least one axis.  Per axis
+// for a particle whose nearest node changed on two or three axes.  Per axis
 // the old and new 3-node windows merge into 4 slots starting at
 // min(N_old, N_new) - 1; the 3 fluxes between them are cumulative sums of
 // W0 - W1 (the fourth vanishes by the partition of unity).  jb points at the
@@ -93,12 +95,47 @@ __device__ inline void deposit_general(double* jb, const R s0[3], const R s1[3],
   }
 }
 
-// Esirkepov current (grid_part_connect.f90:979-1060, 1168-1334) of particles
-// whose nearest node N is the same before and after the move, summed in
-// registers for one cell: per axis the 3-node windows coincide,
-// W0[0]-W1[0] = (s1-s0)(1-s0-s1)/2 and W1[2]-W0[2] = (s1-s0)(1+s0+s1)/2 are
-// the only nonzero fluxes, so the current has 2x3x3 values per component
-// (2x3 in 2-D), offsets from node N-1.
+// Window data of one axis for a particle whose nearest node moved by sg
+// (-1, 0 or +1) from N0, in the frame of the old node block N0-1 .. N0+1:
+// a0 = old quadratic weights, w1 = the new weights that fall inside the
+// block, fl = the two in-block Esirkepov fluxes (N0-1 -> N0, N0 -> N0+1),
+// and for sg != 0 the new weight wo and the flux fo just outside the block
+// (at N0+2 / N0+1 -> N0+2 for sg = +1, at N0-2 / N0-2 -> N0-1 for sg = -1).
+// The fluxes are the cumulative sums of W0 - W1 of grid_part_connect.f90
+// (1008-1030 and 1200-1240); for sg = 0 the two nonzero ones are written in
+// closed form, (s1-s0)(1-s0-s1)/2 and (s1-s0)(1+s0+s1)/2 times wq, which
+// avoids the cancellation of W0 - W1 for short moves.
+template <typename R>
+__device__ inline void axis_window(R s0, R s1, int sg, R wq, R a0[3], R w1[3], R fl[2], R& wo,
+                                   R& fo) {
+  const R zero = R(0), half = R(0.5), one = R(1);
+  R a1[3];
+  qweights<R>(s0, a0);
+  qweights<R>(s1, a1);
+  const bool up = sg > 0, dn = sg < 0;
+  w1[0] = up ? zero : (dn ? a1[1] : a1[0]);
+  w1[1] = up ? a1[0] : (dn ? a1[2] : a1[1]);
+  w1[2] = up ? a1[1] : (dn ? zero : a1[2]);
+  wo = up ? a1[2] : (dn ? a1[0] : zero);
+  // cumulative sums over the 5-slot frame N0-2 .. N0+2
+  const R lo = dn ? a1[0] : zero;  // new weight at N0-2
+  const R c0 = -lo;
+  const R c1 = c0 + a0[0] - w1[0];
+  const R c2 = c1 + a0[1] - w1[1];
+  const R c3 = c2 + a0[2] - w1[2];
+  const R h = half * wq * (s1 - s0);
+  fl[0] = sg == 0 ? h * ((one - s0) - s1) : wq * c1;
+  fl[1] = sg == 0 ? h * ((one + s0) + s1) : wq * c2;
+  fo = up ? wq * c3 : (dn ? wq * c0 : zero);
+}
+
+// Esirkepov current (grid_part_connect.f90:979-1060, 1168-1334) of the
+// particles of one cell, summed in registers: the part of each particle's
+// current on the cell's node block N0-1 .. N0+1, which is all of it for a
+// particle that keeps its nearest node (2x3x3 values per component, 2x3 in
+// 2-D, offsets from node N0-1).  A particle whose node moved on one axis
+// also has a layer just outside the block on that axis, added to LDS by
+// edge().
 template <int D, typename R>
 struct CellAcc {
   R jx[2][3][3], jy[3][2][3], jz[3][3][2];
@@ -112,22 +149,14 @@ struct CellAcc {
     }
   }
 
-  __device__ inline void add(const R s0[3], const R s1[3], R wq) {
-    const R half = R(0.5), one = R(1);
-    R a0[3][3], a1[3][3], fl[3][2];
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      qweights<R>(s0[d], a0[d]);
-      qweights<R>(s1[d], a1[d]);
-      const R h = half * wq * (s1[d] - s0[d]);
-      fl[d][0] = h * ((one - s0[d]) - s1[d]);
-      fl[d][1] = h * ((one + s0[d]) + s1[d]);
-    }
+  // a0, w1, fl per axis as axis_window returns them
+  __device__ inline void add(const R a0[3][3], const R w1[3][3], const R fl[3][2]) {
+    const R half = R(0.5);
     if constexpr (D == 2) {
 #pragma unroll
       for (int mm = 0; mm < 3; mm++) {
-        const R ty2 = a0[1][mm] + a1[1][mm];
-        const R tx2 = a0[0][mm] + a1[0][mm];
+        const R ty2 = a0[1][mm] + w1[1][mm];
+        const R tx2 = a0[0][mm] + w1[0][mm];
 #pragma unroll
         for (int f = 0; f < 2; f++) {
           jx[f][mm][0] = fma_of<R>(fl[0][f], ty2, jx[f][mm][0]);
@@ -137,18 +166,110 @@ struct CellAcc {
     } else {
 #pragma unroll
       for (int mm = 0; mm < 3; mm++) {
-        const R y0 = a0[1][mm] + half * a1[1][mm], y1 = a1[1][mm] + half * a0[1][mm];
-        const R x0 = a0[0][mm] + half * a1[0][mm], x1 = a1[0][mm] + half * a0[0][mm];
+        const R y0 = a0[1][mm] + half * w1[1][mm], y1 = w1[1][mm] + half * a0[1][mm];
+        const R x0 = a0[0][mm] + half * w1[0][mm], x1 = w1[0][mm] + half * a0[0][mm];
 #pragma unroll
         for (int l = 0; l < 3; l++) {
-          const R tyz = y0 * a0[2][l] + y1 * a1[2][l];
-          const R txz = x0 * a0[2][l] + x1 * a1[2][l];
-          const R txy = x0 * a0[1][l] + x1 * a1[1][l];
+          const R tyz = y0 * a0[2][l] + y1 * w1[2][l];
+          const R txz = x0 * a0[2][l] + x1 * w1[2][l];
+          const R txy = x0 * a0[1][l] + x1 * w1[1][l];
 #pragma unroll
           for (int f = 0; f < 2; f++) {
             jx[f][mm][l] = fma_of<R>(fl[0][f], tyz, jx[f][mm][l]);
             jy[mm][f][l] = fma_of<R>(fl[1][f], txz, jy[mm][f][l]);
             jz[mm][l][f] = fma_of<R>(fl[2][f], txy, jz[mm][l][f]);
+          }
+        }
+      }
+    }
+  }
+
+  // The layer outside the block of a particle whose node moved on one axis
+  // (sg nonzero on that axis only), added to the FP64 LDS patch; j1 points at
+  // node N0-1.  On the moved axis the layer holds the outer flux fo (its own
+  // component) and the outer node's weight wo (the transverse components);
+  // the other axes are inside the block.
+  template <int JX, int JY, int PVOL>
+  __device__ inline void edge(double* j1, const int sg[3], const R a0[3][3], const R w1[3][3],
+                              const R fl[3][2], const R wo[3], const R fo[3]) const {
+    const R half = R(0.5);
+    if constexpr (D == 2) {
+      if (sg[0] != 0) {
+        const int xf = sg[0] > 0 ? 2 : -1, xs = sg[0] > 0 ? 3 : -1;
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+          atomicAdd(j1 + xf + JX * m, (double)(fo[0] * (a0[1][m] + w1[1][m])));
+#pragma unroll
+        for (int f = 0; f < 2; f++) atomicAdd(j1 + PVOL + xs + JX * f, (double)(fl[1][f] * wo[0]));
+      }
+      if (sg[1] != 0) {
+        const int yf = sg[1] > 0 ? 2 : -1, ys = sg[1] > 0 ? 3 : -1;
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+          atomicAdd(j1 + PVOL + m + JX * yf, (double)(fo[1] * (a0[0][m] + w1[0][m])));
+#pragma unroll
+        for (int f = 0; f < 2; f++) atomicAdd(j1 + f + JX * ys, (double)(fl[0][f] * wo[1]));
+      }
+    } else {
+      if (sg[0] != 0) {
+        const int xf = sg[0] > 0 ? 2 : -1, xs = sg[0] > 0 ? 3 : -1;
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+          const R y0 = a0[1][m] + half * w1[1][m], y1 = w1[1][m] + half * a0[1][m];
+#pragma unroll
+          for (int l = 0; l < 3; l++)
+            atomicAdd(j1 + xf + JX * (m + JY * l), (double)(fo[0] * (y0 * a0[2][l] + y1 * w1[2][l])));
+        }
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+          const R tz = wo[0] * (half * a0[2][l] + w1[2][l]);
+          const R ty = wo[0] * (half * a0[1][l] + w1[1][l]);
+#pragma unroll
+          for (int f = 0; f < 2; f++) {
+            atomicAdd(j1 + PVOL + xs + JX * (f + JY * l), (double)(fl[1][f] * tz));
+            atomicAdd(j1 + 2 * PVOL + xs + JX * (l + JY * f), (double)(fl[2][f] * ty));
+          }
+        }
+      }
+      if (sg[1] != 0) {
+        const int yf = sg[1] > 0 ? 2 : -1, ys = sg[1] > 0 ? 3 : -1;
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+          const R x0 = a0[0][m] + half * w1[0][m], x1 = w1[0][m] + half * a0[0][m];
+#pragma unroll
+          for (int l = 0; l < 3; l++)
+            atomicAdd(j1 + PVOL + m + JX * (yf + JY * l),
+                      (double)(fo[1] * (x0 * a0[2][l] + x1 * w1[2][l])));
+        }
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+          const R tz = wo[1] * (half * a0[2][l] + w1[2][l]);
+          const R tx = wo[1] * (w1[0][l] + half * a0[0][l]);
+#pragma unroll
+          for (int f = 0; f < 2; f++) {
+            atomicAdd(j1 + f + JX * (ys + JY * l), (double)(fl[0][f] * tz));
+            atomicAdd(j1 + 2 * PVOL + l + JX * (ys + JY * f), (double)(fl[2][f] * tx));
+          }
+        }
+      }
+      if (sg[2] != 0) {
+        const int zf = sg[2] > 0 ? 2 : -1, zs = sg[2] > 0 ? 3 : -1;
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+          const R x0 = a0[0][m] + half * w1[0][m], x1 = w1[0][m] + half * a0[0][m];
+#pragma unroll
+          for (int l = 0; l < 3; l++)
+            atomicAdd(j1 + 2 * PVOL + m + JX * (l + JY * zf),
+                      (double)(fo[2] * (x0 * a0[1][l] + x1 * w1[1][l])));
+        }
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+          const R ty = wo[2] * (w1[1][l] + half * a0[1][l]);
+          const R tx = wo[2] * (w1[0][l] + half * a0[0][l]);
+#pragma unroll
+          for (int f = 0; f < 2; f++) {
+            atomicAdd(j1 + f + JX * (l + JY * zs), (double)(fl[0][f] * ty));
+            atomicAdd(j1 + PVOL + l + JX * (f + JY * zs), (double)(fl[1][f] * tx));
           }
         }
       }
@@ -185,7 +306,7 @@ struct CellAcc {
 //   Fl  R      [NF][GVOL]   field patch, nodes T0-1 .. T0+T per axis
 //   Jl  double [D][PVOL]    current patch, nodes T0-2 .. T0+T+1 per axis
 //   Q   per wave: R s[6][QCAP], float wq[QCAP], int meta[QCAP]: the queue of
-//       node-changing particles
+//       particles whose node moved on two or three axes
 //   offs, fillc int [NCL+1], [NCL]; misc int [3]; perm u16 [kChunk]
 template <int D, int TX, int TY, int TZ, int NT, typename R>
 struct TileShape {
@@ -214,8 +335,8 @@ struct WaveQueue {
         meta((int*)(base + 6 * QCAP * sizeof(R) + 4 * QCAP)) {}
 };
 
-// Drains n <= 64 queued node-changing particles, one per lane, through the
-// union-window deposit.
+// Drains n <= 64 queued particles, one per lane, through the union-window
+// deposit.
 template <int D, int TX, int TY, int TZ, int NT, typename R>
 __device__ inline void drain_queue(const WaveQueue<R, 128>& qu, int head, int n, double* Jl) {
   using S = TileShape<D, TX, TY, TZ, NT, R>;
@@ -422,8 +543,13 @@ __global__ __launch_bounds__(NT) void k_tile(
         const R dtg = push<D, R>(q.p, ef, sc.qch, sc.alp, sc.dt, sc.initial);
         keep = move_and_test<D, R>(q, dtg, sc.inv, sc.nn);
         wq = sc.qch * q.w;
-        cross = q.N[0] != N0[0] || q.N[1] != N0[1];
-        if (D == 3) cross = cross || q.N[2] != N0[2];
+        int sg[3] = {0, 0, 0}, nmoved = 0;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          sg[d] = q.N[d] - N0[d];
+          nmoved += sg[d] != 0 ? 1 : 0;
+        }
+        cross = nmoved > 1;  // node moved on two or three axes: the queue
         if (keep && !cross) {
           // a thread's slice runs through the cells in order: sum in
           // registers while the cell stays, add to LDS when it changes
@@ -432,10 +558,16 @@ __global__ __launch_bounds__(NT) void k_tile(
             acc.zero();
             acc_lc = lc;
           }
-          acc.add(so, q.s, wq);
+          R a0[3][3] = {{R(1), R(0), R(0)}, {R(1), R(0), R(0)}, {R(1), R(0), R(0)}};
+          R w1[3][3] = {{R(1), R(0), R(0)}, {R(1), R(0), R(0)}, {R(1), R(0), R(0)}};
+          R fl[3][2] = {}, wo[3] = {}, fo[3] = {};
+#pragma unroll
+          for (int d = 0; d < D; d++) axis_window<R>(so[d], q.s[d], sg[d], wq, a0[d], w1[d], fl[d], wo[d], fo[d]);
+          acc.add(a0, w1, fl);
+          if (nmoved) acc.template edge<JX, JY, PVOL>(patch_node(lc), sg, a0, w1, fl, wo, fo);
         }
       }
-      // node-changing particles go to the wave's queue
+      // particles whose node moved on two or three axes go to the wave's queue
       const unsigned long long cm = __ballot(keep && cross);
       if (keep && cross) {
         const int e = (qhead + qcount + __popcll(cm & lt_mask)) & (QCAP - 1);
