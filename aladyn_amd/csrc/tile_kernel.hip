@@ -15,11 +15,12 @@
 //      current on its old cell's node block to register sums of that cell
 //      (flushed into the LDS patch when the thread's slice moves to the next
 //      cell) and the layer beyond the block, if any, to the LDS patch; one
-//      whose node moved on two or three axes is queued per wave and
-//      deposited through the union window when 64 are waiting; survivors
-//      are written to the other buffer: stayers into
-//      slots reserved per chunk, movers into their new tile with
-//      wave-aggregated atomics, overflow into the global list;
+//      whose node moved on two or three axes is queued per wave (64
+//      entries) and deposited through the union window, one per lane, when
+//      the next round's entries would not fit; survivors are written to the
+//      other buffer: stayers into slots reserved per chunk, movers into
+//      their new tile with wave-aggregated atomics, overflow into the global
+//      list;
 //   4. write the LDS current patch to HBM for the fold.
 // LDS accumulation is FP64 because LDS FP32 atomic adds are far slower than
 // FP64 ones on gfx950; there are no global float atomics.
@@ -313,7 +314,7 @@ struct TileShape {
   static constexpr int NCL = TX * TY * TZ, NF = D == 3 ? 6 : 3, NW = NT / 64;
   static constexpr int GX = TX + 2, GY = TY + 2, GZ = D == 3 ? TZ + 2 : 1, GVOL = GX * GY * GZ;
   static constexpr int JX = TX + 4, JY = TY + 4, JZ = D == 3 ? TZ + 4 : 1, PVOL = JX * JY * JZ;
-  static constexpr int QCAP = 128;
+  static constexpr int QCAP = 64;  // one entry per lane: a drain takes all
   static constexpr int F_BYTES = (NF * GVOL * (int)sizeof(R) + 15) & ~15;
   static constexpr int J_BYTES = 8 * D * PVOL;
   static constexpr int QW_BYTES = QCAP * (6 * (int)sizeof(R) + 8);  // one wave's queue
@@ -335,14 +336,14 @@ struct WaveQueue {
         meta((int*)(base + 6 * QCAP * sizeof(R) + 4 * QCAP)) {}
 };
 
-// Drains n <= 64 queued particles, one per lane, through the union-window
-// deposit.
+// Drains the n <= 64 queued particles, one per lane, through the
+// union-window deposit.
 template <int D, int TX, int TY, int TZ, int NT, typename R>
-__device__ inline void drain_queue(const WaveQueue<R, 128>& qu, int head, int n, double* Jl) {
+__device__ inline void drain_queue(const WaveQueue<R, 64>& qu, int n, double* Jl) {
   using S = TileShape<D, TX, TY, TZ, NT, R>;
   const int lane = (int)__lane_id();
   if (lane < n) {
-    const int e = (head + lane) & (S::QCAP - 1);
+    const int e = lane;
     R s0[3] = {R(0), R(0), R(0)}, s1[3] = {R(0), R(0), R(0)};
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -369,7 +370,7 @@ __device__ inline void drain_queue(const WaveQueue<R, 128>& qu, int head, int n,
 }
 
 template <int D, int TX, int TY, int TZ, int NT, typename R>
-__global__ __launch_bounds__(NT) void k_tile(
+__global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
     TileArgs ta, StepConst<R> sc, const R* __restrict__ ebf, RecPtr A, RecPtr B,
     const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
     const int* __restrict__ cnt_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_cur,
@@ -417,8 +418,8 @@ __global__ __launch_bounds__(NT) void k_tile(
   const int me = threadIdx.x;
   const int lane = (int)__lane_id();
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
-  const WaveQueue<R, 128> qu(Q + (me >> 6) * S::QW_BYTES);
-  int qhead = 0, qcount = 0;  // wave-uniform ring state
+  const WaveQueue<R, QCAP> qu(Q + (me >> 6) * S::QW_BYTES);
+  int qcount = 0;  // wave-uniform: entries 0 .. qcount-1 are waiting
 
   const int cap_t = seg_cap[t];
   const int cnt = min(cnt_cur[t], cap_t);
@@ -569,8 +570,18 @@ __global__ __launch_bounds__(NT) void k_tile(
       }
       // particles whose node moved on two or three axes go to the wave's queue
       const unsigned long long cm = __ballot(keep && cross);
+      const int nnew = __popcll(cm);
+      if (qcount + nnew > QCAP) {  // no room for this round's entries: drain first
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        drain_queue<D, TX, TY, TZ, NT, R>(qu, qcount, Jl);
+        qcount = 0;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
       if (keep && cross) {
-        const int e = (qhead + qcount + __popcll(cm & lt_mask)) & (QCAP - 1);
+        const int e = qcount + __popcll(cm & lt_mask);
         int meta = lc;
 #pragma unroll
         for (int d = 0; d < D; d++) {
@@ -581,7 +592,7 @@ __global__ __launch_bounds__(NT) void k_tile(
         qu.wq[e] = (float)wq;
         qu.meta[e] = meta;
       }
-      qcount += __popcll(cm);
+      qcount += nnew;
       if (keep) {
         // write the survivor into the other buffer
         const int dtile = (q.N[0] - 3) / TX +
@@ -603,16 +614,6 @@ __global__ __launch_bounds__(NT) void k_tile(
         }
         store_rec<D, R>(B, dst, q, ta.bx, ta.by);
       }
-      if (qcount >= 64) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        drain_queue<D, TX, TY, TZ, NT, R>(qu, qhead, 64, Jl);
-        qhead = (qhead + 64) & (QCAP - 1);
-        qcount -= 64;
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-      }
     }
     __syncthreads();
     // reserved slots not taken by stayers become holes
@@ -628,7 +629,7 @@ __global__ __launch_bounds__(NT) void k_tile(
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    drain_queue<D, TX, TY, TZ, NT, R>(qu, qhead, qcount, Jl);
+    drain_queue<D, TX, TY, TZ, NT, R>(qu, qcount, Jl);
   }
   if (acc_lc >= 0) acc.template flush<JX, JY, PVOL>(patch_node(acc_lc));
   __syncthreads();

@@ -118,53 +118,65 @@ __global__ __launch_bounds__(256) void k_ovf_particles(
 }
 
 // ------------------------------------------------------------ fold kernel
-// jc(node) [+]= sum over the tile patches covering node (at most two tiles
-// per axis since T >= 4), compensated (Kahan) summation.
+// jc(node) [+]= the sum of the tile patches covering node: at most two tiles
+// per axis (T >= 4), the one whose interior holds the node and the neighbour
+// whose halo reaches it.  One thread per node of a 64 x 4 x 1 block of the
+// grid (no index division); the up to 8 patch values of a component are all
+// loaded before they are summed (compensated, Kahan), in the order of
+// increasing tile index.
 template <typename R>
 __global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g, const R* __restrict__ patch,
                                               R* __restrict__ jc, int accumulate) {
   const int D = g.ndim;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.ncell;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int idx[3] = {(int)(t % g.n1p) + 1, (int)((t / g.n1p) % g.n2p) + 1,
-                        (int)(t / ((int64_t)g.n1p * g.n2p)) + 1};
-    int tl[3][2], loc[3][2], nt[3];
-    for (int d = 0; d < 3; d++) {
-      nt[d] = 0;
-      if (d >= D) {
-        tl[d][0] = 0;
-        loc[d][0] = 0;
-        nt[d] = 1;
-        continue;
-      }
-      const int u = idx[d] - 3;  // >= -2
-      int thi = (u + 2) / tg.T[d];
-      if (thi > tg.nt[d] - 1) thi = tg.nt[d] - 1;
-      for (int tt = (thi > 0 ? thi - 1 : 0); tt <= thi; tt++) {
-        const int o = 1 + tt * tg.T[d];  // patch origin T0 - 2
-        const int l = idx[d] - o;
-        if (l >= 0 && l < tg.J[d]) {
-          tl[d][nt[d]] = tt;
-          loc[d][nt[d]] = l;
-          nt[d]++;
+  const int idx[3] = {(int)(blockIdx.x * 64 + (threadIdx.x & 63)) + 1,
+                      (int)(blockIdx.y * 4 + (threadIdx.x >> 6)) + 1, (int)blockIdx.z + 1};
+  if (idx[0] > g.n1p || idx[1] > g.n2p) return;
+  // per axis: candidate tiles t0 = t1 - 1 and t1, local patch indices, validity
+  int tl[3][2], loc[3][2];
+  bool ok[3][2];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (d >= D) {
+      tl[d][0] = tl[d][1] = 0;
+      loc[d][0] = loc[d][1] = 0;
+      ok[d][0] = true;
+      ok[d][1] = false;
+      continue;
+    }
+    const int sh = __ffs(tg.T[d]) - 1;  // T is a power of two
+    const int hi = min((idx[d] - 1) >> sh, tg.nt[d] - 1);  // (u + 2) / T, u = idx - 3
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      const int tt = hi - 1 + e;
+      const int l = idx[d] - (1 + tt * tg.T[d]);  // patch origin T0 - 2
+      ok[d][e] = tt >= 0 && l >= 0 && l < tg.J[d];
+      tl[d][e] = max(tt, 0);
+      loc[d][e] = ok[d][e] ? l : 0;
+    }
+  }
+  const int64_t o = (idx[0] - 1) + (int64_t)g.n1p * ((idx[1] - 1) + (int64_t)g.n2p * (idx[2] - 1));
+  for (int c = 0; c < D; c++) {
+    R v[8];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          const bool use = ok[0][e] && ok[1][b] && ok[2][a];
+          const int tile = tl[0][e] + tg.nt[0] * (tl[1][b] + tg.nt[1] * tl[2][a]);
+          const int l = loc[0][e] + tg.J[0] * (loc[1][b] + tg.J[1] * loc[2][a]);
+          v[a * 4 + b * 2 + e] = use ? patch[((int64_t)tile * D + c) * tg.pvol + l] : R(0);
         }
-      }
+    R sum = accumulate ? jc[(int64_t)c * g.ncell + o] : R(0), comp = R(0);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const R y = v[q] - comp;
+      const R s2 = sum + y;
+      comp = (s2 - sum) - y;
+      sum = s2;
     }
-    for (int c = 0; c < D; c++) {
-      R sum = accumulate ? jc[(int64_t)c * g.ncell + t] : R(0), comp = R(0);
-      for (int a = 0; a < nt[2]; a++)
-        for (int b = 0; b < nt[1]; b++)
-          for (int e = 0; e < nt[0]; e++) {
-            const int tile = tl[0][e] + tg.nt[0] * (tl[1][b] + tg.nt[1] * tl[2][a]);
-            const int l = loc[0][e] + tg.J[0] * (loc[1][b] + tg.J[1] * loc[2][a]);
-            const R v = patch[((int64_t)tile * D + c) * tg.pvol + l];
-            const R y = v - comp;
-            const R s2 = sum + y;
-            comp = (s2 - sum) - y;
-            sum = s2;
-          }
-      jc[(int64_t)c * g.ncell + t] = sum;
-    }
+    jc[(int64_t)c * g.ncell + o] = sum;
   }
 }
 
@@ -867,7 +879,10 @@ int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R
                               ts->seg_cap, ts->count[cur], ts->count[nxt], ts->ovf_count + cur,
                               ts->ovf_count + nxt, patch, ts->grow, s));
   if (ev_end) TS_CK(hipEventRecord(ev_end, s));
-  k_fold<R><<<grid_for(g.ncell, 8192), kB, 0, s>>>(tg, g, patch, jc, j_is_zero ? 0 : 1);
+  {
+    const dim3 fb((g.n1p + 63) / 64, (g.n2p + 3) / 4, g.n3p);
+    k_fold<R><<<fb, kB, 0, s>>>(tg, g, patch, jc, j_is_zero ? 0 : 1);
+  }
   if (D == 2)
     k_ovf_particles<2, R><<<512, kB, 0, s>>>(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt],
                                              ts->seg_start, ts->seg_cap, ts->ovf_count + cur,
