@@ -293,21 +293,24 @@ __device__ inline bool move_and_test(Rec<R>& q, R dtg, const R inv[3], const int
 // atomicAdd on counters[key] per distinct key; returns this lane's slot.
 __device__ inline int reserve_slot(int key, int* counters) {
   const int lane = (int)__lane_id();
+  // group the active lanes by key with ballots only (no memory traffic)
   unsigned long long todo = __ballot(1);
-  int slot = 0;
+  int leader = lane, rank = 0, count = 1;
   while (todo) {
-    const int leader = __ffsll((unsigned long long)todo) - 1;
-    const int k = __shfl(key, leader);
+    const int l = __ffsll((unsigned long long)todo) - 1;
+    const int k = __shfl(key, l);
     const unsigned long long same = __ballot(key == k);
     if (key == k) {
-      int b = 0;
-      if (lane == leader) b = atomicAdd(&counters[k], __popcll(same));
-      b = __shfl(b, leader);
-      slot = b + __popcll(same & ((1ull << lane) - 1ull));
+      leader = l;
+      rank = __popcll(same & ((1ull << lane) - 1ull));
+      count = __popcll(same);
     }
     todo &= ~same;
   }
-  return slot;
+  // one atomic per group, all groups in the same instruction
+  int b = 0;
+  if (lane == leader) b = atomicAdd(&counters[key], count);
+  return __shfl(b, leader) + rank;
 }
 
 // The same for one counter shared by all active lanes (LDS or global).

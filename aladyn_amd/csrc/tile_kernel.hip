@@ -308,7 +308,7 @@ struct CellAcc {
 //   Jl  double [D][PVOL]    current patch, nodes T0-2 .. T0+T+1 per axis
 //   Q   per wave: R s[6][QCAP], float wq[QCAP], int meta[QCAP]: the queue of
 //       particles whose node moved on two or three axes
-//   offs, fillc int [NCL+1], [NCL]; misc int [3]; perm u16 [kChunk]
+//   offs, fillc int [NCL+1], [NCL]; misc int [4]; wsum int [NW]; perm u16 [kChunk]
 template <int D, int TX, int TY, int TZ, int NT, typename R>
 struct TileShape {
   static constexpr int NCL = TX * TY * TZ, NF = D == 3 ? 6 : 3, NW = NT / 64;
@@ -319,7 +319,7 @@ struct TileShape {
   static constexpr int J_BYTES = 8 * D * PVOL;
   static constexpr int QW_BYTES = QCAP * (6 * (int)sizeof(R) + 8);  // one wave's queue
   static constexpr int Q_BYTES = NW * QW_BYTES;
-  static constexpr int I_BYTES = 4 * (2 * NCL + 4);
+  static constexpr int I_BYTES = 4 * (2 * NCL + 5 + NW);
   static constexpr int BYTES = F_BYTES + J_BYTES + Q_BYTES + I_BYTES + 2 * kChunk;
   static_assert(NT % 64 == 0 && NT >= NCL, "threads: whole wavefronts, at least one per cell");
   static_assert(kChunk % NT == 0, "a chunk is a whole number of records per thread");
@@ -386,13 +386,34 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
   int* offs = (int*)(Q + S::Q_BYTES);  // NCL + 1
   int* fillc = offs + NCL + 1;  // NCL
   int* misc = fillc + NCL;      // [0] chunk slot base, [1] stayers, [2] max per cell
-  unsigned short* perm = (unsigned short*)(misc + 4);
+  int* wsum = misc + 4;          // per-wave totals of the cell-count scan
+  unsigned short* perm = (unsigned short*)(wsum + S::NW);
 
   const int t = blockIdx.x;
   const int tx = t % ta.nt0, ty = (t / ta.nt0) % ta.nt1, tz = t / (ta.nt0 * ta.nt1);
   const int T0x = 3 + tx * TX, T0y = 3 + ty * TY, T0z = D == 3 ? 3 + tz * TZ : 1;
-  // field patch: nodes T0-1 .. T0+T; every load of the thread is issued
-  // before the first LDS store, so the HBM latency is paid once per tile
+  const int me = threadIdx.x;
+  const int lane = (int)__lane_id();
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  const int cap_t = seg_cap[t];
+  const int cnt = min(cnt_cur[t], cap_t);
+  const int64_t base = seg_start[t];
+  const int mi = (1 << ta.bx) - 1, mj = (1 << ta.by) - 1, shk = ta.bx + ta.by;
+  // the packed cells of a chunk, loaded once (all loads in flight together)
+  // and kept in registers for the histogram and the permutation
+  constexpr int CPT = kChunk / NT;
+  int lcv[CPT];
+  const auto load_cells = [&](int c0) {
+    const int m = min(kChunk, cnt - c0);
+#pragma unroll
+    for (int u = 0; u < CPT; u++) {
+      const int q = me + u * NT;
+      lcv[u] = q < m ? rec_cell<R>(A, base + c0 + q) : kHole;
+    }
+  };
+  // field patch: nodes T0-1 .. T0+T; every load of the thread, and those of
+  // the first chunk's cells, is issued before the first LDS store, so the
+  // HBM latency is paid once per tile
   {
     constexpr int FPT = (NF * GVOL + NT - 1) / NT;
     R fv[FPT];
@@ -407,6 +428,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
         fv[u] = ebf[(int64_t)c * ta.ncell + (i - 1) +
                     (int64_t)ta.n1p * ((j - 1) + (int64_t)ta.n2p * (k - 1))];
     }
+    load_cells(0);
 #pragma unroll
     for (int u = 0; u < FPT; u++) {
       const int q = threadIdx.x + u * NT;
@@ -415,16 +437,8 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
   }
   for (int q = threadIdx.x; q < D * PVOL; q += NT) Jl[q] = 0.0;
 
-  const int me = threadIdx.x;
-  const int lane = (int)__lane_id();
-  const unsigned long long lt_mask = (1ull << lane) - 1ull;
   const WaveQueue<R, QCAP> qu(Q + (me >> 6) * S::QW_BYTES);
   int qcount = 0;  // wave-uniform: entries 0 .. qcount-1 are waiting
-
-  const int cap_t = seg_cap[t];
-  const int cnt = min(cnt_cur[t], cap_t);
-  const int64_t base = seg_start[t];
-  const int mi = (1 << ta.bx) - 1, mj = (1 << ta.by) - 1, shk = ta.bx + ta.by;
   // LDS current patch node N-1 of local cell lc (the patch starts at T0-2)
   const auto patch_node = [&](int lc) -> double* {
     const int li = lc % TX, lj = (lc / TX) % TY, lk = D == 3 ? lc / (TX * TY) : 0;
@@ -435,19 +449,9 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
   int acc_lc = -1;
 
   for (int c0 = 0; c0 < cnt; c0 += kChunk) {
-    const int m = min(kChunk, cnt - c0);
     __syncthreads();  // patches staged / previous chunk consumed
     if (me < NCL) offs[me] = 0;
     if (me == 0) misc[2] = 0;
-    // the chunk's packed cells, loaded once (all loads in flight together)
-    // and kept in registers for the histogram and the permutation
-    constexpr int CPT = kChunk / NT;
-    int lcv[CPT];
-#pragma unroll
-    for (int u = 0; u < CPT; u++) {
-      const int q = me + u * NT;
-      lcv[u] = q < m ? rec_cell<R>(A, base + c0 + q) : kHole;
-    }
     // local cell index, kHole for a hole, -2 for a record of another tile
 #pragma unroll
     for (int u = 0; u < CPT; u++) {
@@ -463,27 +467,32 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
     for (int u = 0; u < CPT; u++)
       if (lcv[u] >= 0) atomicAdd(&offs[lcv[u]], 1);
     __syncthreads();
-    {  // exclusive scan of offs[0..NCL) (Hillis-Steele through fillc)
+    {  // exclusive scan of offs[0..NCL): within each wave by shuffles, then
+       // across the waves through wsum
       const int v = me < NCL ? offs[me] : 0;
-      if (me < NCL) fillc[me] = v;
-      if (v > 0) atomicMax(&misc[2], v);
-      __syncthreads();
-#pragma unroll 1
-      for (int off = 1; off < NCL; off <<= 1) {
-        const int add = (me < NCL && me >= off) ? fillc[me - off] : 0;
-        __syncthreads();
-        if (me < NCL) fillc[me] += add;
-        __syncthreads();
+      int x = v;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
       }
-      if (me < NCL) offs[me] = fillc[me] - v;
+      const int vmax = wave_max(v);
+      if (lane == 63) {
+        wsum[me >> 6] = x;
+        if (vmax > 0) atomicMax(&misc[2], vmax);
+      }
+      if (me < NCL) fillc[me] = 0;
+      __syncthreads();
+      int pre = 0;
+      for (int w = 0; w < (me >> 6); w++) pre += wsum[w];
+      const int incl = x + pre;
+      if (me < NCL) offs[me] = incl - v;
       if (me == NCL - 1) {
-        offs[NCL] = fillc[me];
+        offs[NCL] = incl;
         // slots for every valid particle of the chunk in this tile
-        misc[0] = atomicAdd(&cnt_next[t], fillc[me]);
+        misc[0] = atomicAdd(&cnt_next[t], incl);
         misc[1] = 0;
       }
-      __syncthreads();
-      if (me < NCL) fillc[me] = 0;
       __syncthreads();
     }
 #pragma unroll
@@ -593,20 +602,25 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
         qu.meta[e] = meta;
       }
       qcount += nnew;
+      // Survivors go to the other buffer: those that stay into the slots the
+      // chunk reserved in this tile's segment, the others into their new
+      // tile's segment; reserved slots left over become holes after the loop.
+      const int dtile = (q.N[0] - 3) / TX +
+                        ta.nt0 * ((q.N[1] - 3) / TY + ta.nt1 * (D == 3 ? (q.N[2] - 3) / TZ : 0));
+      const bool stay = keep && dtile == t;
       if (keep) {
-        // write the survivor into the other buffer
-        const int dtile = (q.N[0] - 3) / TX +
-                          ta.nt0 * ((q.N[1] - 3) / TY + ta.nt1 * (D == 3 ? (q.N[2] - 3) / TZ : 0));
         int64_t dst;
         int slot, cap;
-        if (dtile == t) {
+        if (stay) {
           slot = chunk_base + reserve_one(&misc[1]);
           cap = cap_t;
           dst = base + slot;
         } else {
-          slot = reserve_slot(dtile, cnt_next);
+          // the segment's place and size load while the slot atomic is out
           cap = seg_cap[dtile];
-          dst = (int64_t)seg_start[dtile] + slot;
+          const int64_t start = seg_start[dtile];
+          slot = reserve_slot(dtile, cnt_next);
+          dst = start + slot;
         }
         if (slot >= cap) {  // segment full: overflow list, and a larger segment next re-layout
           dst = sc.ovf_base + atomicAdd(ovf_next, 1);
@@ -616,14 +630,14 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
       }
     }
     __syncthreads();
-    // reserved slots not taken by stayers become holes
-    {
+    {  // reserved slots not taken by stayers become holes
       const int nstay = misc[1], nval = offs[NCL];
       for (int q = nstay + me; q < nval; q += NT) {
         const int slot = chunk_base + q;
         if (slot < cap_t) set_rec_cell<R>(B, base + slot, kHole);
       }
     }
+    if (c0 + kChunk < cnt) load_cells(c0 + kChunk);
   }
   if (qcount > 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
