@@ -16,10 +16,19 @@ static constexpr int kChunk = 4096;
 
 // Real-type helpers: R is the precision of the particle state and fields
 // (float for the FP32 path, double for the FP64 path).
+// FP64: correctly rounded.  FP32: the hardware square root, reciprocal and
+// reciprocal square root (v_sqrt_f32, v_rcp_f32, v_rsq_f32, 1 ulp).  The
+// push's arguments are 1 + p^2, gamma^2 + b^2 (>= 1) and
+// (1 + u^2 - b^2)^2 + 4(b^2 + (b.u)^2) (>= 1/4), far from denormals.
 template <typename R> __device__ inline R rsqrt_of(R x) { return R(1) / sqrt(x); }
-template <> __device__ inline float rsqrt_of<float>(float x) { return 1.f / sqrtf(x); }
+template <> __device__ inline float rsqrt_of<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 template <typename R> __device__ inline R sqrt_of(R x) { return sqrt(x); }
-template <> __device__ inline float sqrt_of<float>(float x) { return sqrtf(x); }
+template <> __device__ inline float sqrt_of<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <typename R> __device__ inline R rcp_of(R x) { return R(1) / x; }
+template <> __device__ inline float rcp_of<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+// a / sqrt(x)
+template <typename R> __device__ inline R div_sqrt(R a, R x) { return a / sqrt(x); }
+template <> __device__ inline float div_sqrt<float>(float a, float x) { return a * __builtin_amdgcn_rsqf(x); }
 template <typename R> __device__ inline R fma_of(R a, R b, R c) { return fma(a, b, c); }
 template <> __device__ inline float fma_of<float>(float a, float b, float c) { return fmaf(a, b, c); }
 
@@ -226,12 +235,12 @@ __device__ inline R push(R p[3], R ef[6], R qch, R alp, R dt, int initial) {
     const R gam02 = (one + (v1 * v1 + v2 * v2)) - b2;
     const R gam2 = half * (gam02 + sqrt_of<R>(gam02 * gam02 + R(4) * b2));
     const R gam = sqrt_of<R>(gam2);
-    const R r = one / (gam2 + b2);
+    const R r = rcp_of<R>(gam2 + b2);
     const R h1 = (gam2 * v1 + gam * v2 * v3) * r;
     const R h2 = (gam2 * v2 - gam * v1 * v3) * r;
     p[0] = two * h1 - p[0];
     p[1] = two * h2 - p[1];
-    return dt / sqrt_of<R>(one + (p[0] * p[0] + p[1] * p[1]));
+    return div_sqrt<R>(dt, one + (p[0] * p[0] + p[1] * p[1]));
   } else {
     if (initial) {
       R e[6];
@@ -250,14 +259,14 @@ __device__ inline R push(R p[3], R ef[6], R qch, R alp, R dt, int initial) {
     const R gam02 = (one + (vp[0] * vp[0] + vp[1] * vp[1] + vp[2] * vp[2])) - b2;
     const R gam2 = half * (gam02 + sqrt_of<R>(gam02 * gam02 + R(4) * (b2 + bv * bv)));
     const R gam = sqrt_of<R>(gam2);
-    const R r = one / (b2 + gam2);
+    const R r = rcp_of<R>(b2 + gam2);
     R vph[3];
     vph[0] = gam2 * vp[0] + bb[0] * bv + gam * (vp[1] * bb[2] - vp[2] * bb[1]);
     vph[1] = gam2 * vp[1] + bb[1] * bv + gam * (vp[2] * bb[0] - vp[0] * bb[2]);
     vph[2] = gam2 * vp[2] + bb[2] * bv + gam * (vp[0] * bb[1] - vp[1] * bb[0]);
 #pragma unroll
     for (int c = 0; c < 3; c++) p[c] = two * (vph[c] * r) - p[c];
-    return dt / sqrt_of<R>(one + (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
+    return div_sqrt<R>(dt, one + (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
   }
 }
 
