@@ -62,6 +62,9 @@ SIGNATURES = {
     "aladyn_lp_window_xshift": [_P, C.c_int32, C.c_int32, C.c_double, C.POINTER(C.c_int32)],
     "aladyn_inject_species": [_P, C.c_int32, _D, _I64, _I64],
     "aladyn_box": [_P, _D],
+    "aladyn_set_injection": [_P, C.c_int32, _D, _I64, _D, C.c_int32, _D, C.c_int32, _D, _D, _I64,
+                             C.c_double, C.c_double],
+    "aladyn_injection_state": [_P, C.c_int32, C.POINTER(C.c_int64)],
     "aladyn_energy_momenta": [_P, C.c_int32, _D, _D],
     "aladyn_field_energy": [_P, _D],
     "aladyn_grid_charge": [_P, C.c_int32, _D],

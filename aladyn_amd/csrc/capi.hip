@@ -57,6 +57,18 @@ struct TimerRing {
   }
 };
 
+// Target table of one species for injection at window moves (the arrays
+// particles_inject / add_particles read, window.f90:39-270).
+struct InjTable {
+  bool active = false;
+  std::vector<double> xpt;  // ascending column positions
+  int nyp = 0, nzp = 0;
+  double* dev = nullptr;    // xpt | ypt | zpt | wghpt | wghyz on the device
+  InjTableDev tb{};
+  int64_t nptx = 0;         // columns already consumed (nptx(ic))
+  double targ_in = 0.0, targ_end = 0.0;
+};
+
 struct aladyn_ctx {
   aladyn_config cfg{};
   DevGrid g{};
@@ -85,6 +97,7 @@ struct aladyn_ctx {
   // lp_window_xshift saved state (window.f90:278-280)
   double win_xlapse = 0.0;
   int win_wi1 = 0;
+  InjTable inj[ALADYN_MAX_SPECIES];
   // timing: event pairs around the particle kernel and the field solve,
   // recorded on the context stream (ring of kRing steps)
   TimerRing tpart, tfield;
@@ -238,6 +251,7 @@ extern "C" int aladyn_destroy(aladyn_ctx* c) {
   (void)hipFree(c->jc32);
   (void)hipFree(c->patch);
   for (auto& t : c->ts) tile_store_free(&t);
+  for (auto& t : c->inj) (void)hipFree(t.dev);
   (void)hipFree(c->d_red);
   if (c->h_red) (void)hipHostFree(c->h_red);
   (void)hipFree(c->rho0);
@@ -305,6 +319,96 @@ extern "C" int aladyn_species_count(aladyn_ctx* c, int32_t ic, int64_t* np) {
   return ALADYN_OK;
 }
 
+// particles_inject (window.f90:106-139) after a move, gated as in
+// lp_window_xshift (window.f90:339-341): with 1-based columns, i1 = nptx+1,
+// the scan stops at the first xpt(ix) > xmax, i2 = ix-1 (i2 = ix when it
+// stopped at the last column), nptx = i2, and columns i1..i2 are added only
+// if i2 > i1.  Columns outside (xmin, xmax] are consumed but not added: the
+// reference would remove them before their first deposit.
+static int inject_from_table(aladyn_ctx* c, int ic) {
+  InjTable& t = c->inj[ic];
+  if (!t.active) return 0;
+  const DevGrid& g = c->g;
+  if (!(t.targ_in <= g.xmx && t.targ_end > g.xmx)) return 0;
+  const int64_t nxp = (int64_t)t.xpt.size(), i1 = t.nptx + 1;
+  int64_t i2 = i1 - 1;
+  if (i1 <= nxp) {
+    int64_t ix = i1;
+    for (; ix <= nxp; ix++)
+      if (t.xpt[ix - 1] > g.xmx) break;
+    i2 = ix - 1;
+    if (ix == nxp) i2 = ix;
+  }
+  t.nptx = i2;
+  if (!(i2 > i1)) return 0;
+  int64_t a = i1, b = i2;  // 1-based, inclusive
+  while (a <= b && !(t.xpt[a - 1] > g.xmn)) a++;
+  while (b >= a && t.xpt[b - 1] > g.xmx) b--;
+  if (b < a) return 0;
+  std::string msg;
+  if (int r = tile_store_inject_table(&c->ts[ic], c->tg, g, t.tb, a - 1, b - a + 1, c->stream, &msg))
+    return fail(c, r, "%s", msg.c_str());
+  return 0;
+}
+
+extern "C" int aladyn_set_injection(aladyn_ctx* c, int32_t ic, const double* xpt, int64_t nxp,
+                                    const double* ypt, int32_t nyp, const double* zpt,
+                                    int32_t nzp, const double* wghpt, const double* wghyz,
+                                    int64_t nptx, double targ_in, double targ_end) {
+  if (int r = check_species(c, ic)) return r;
+  InjTable& t = c->inj[ic];
+  (void)hipFree(t.dev);
+  t = InjTable();
+  if (nxp == 0) return ALADYN_OK;  // table cleared
+  const DevGrid& g = c->g;
+  const int D = g.curr_ndim;
+  if (D == 2) nzp = 1;
+  if (nxp < 0 || nyp < 1 || nzp < 1 || !xpt || !ypt || !wghpt || !wghyz || (D == 3 && !zpt))
+    return fail(c, ALADYN_EINVAL, "set_injection: empty or missing table arrays");
+  if (nptx < 0 || nptx > nxp) return fail(c, ALADYN_EINVAL, "set_injection: nptx outside 0..nxp");
+  for (int64_t i = 0; i < nxp; i++)
+    if (!std::isfinite(xpt[i]) || !std::isfinite(wghpt[i]) || (i > 0 && xpt[i] < xpt[i - 1]))
+      return fail(c, ALADYN_EINVAL, "set_injection: xpt must be finite and ascending");
+  for (int j = 0; j < nyp; j++)
+    if (!(ypt[j] > g.ymn && ypt[j] <= g.ymx))
+      return fail(c, ALADYN_ERANGE, "set_injection: ypt[%d] = %g outside (%g, %g]", j, ypt[j],
+                  g.ymn, g.ymx);
+  if (D == 3)
+    for (int k = 0; k < nzp; k++)
+      if (!(zpt[k] > g.zmn && zpt[k] <= g.zmx))
+        return fail(c, ALADYN_ERANGE, "set_injection: zpt[%d] = %g outside (%g, %g]", k, zpt[k],
+                    g.zmn, g.zmx);
+  for (int64_t q = 0; q < (int64_t)nyp * nzp; q++)
+    if (!std::isfinite(wghyz[q])) return fail(c, ALADYN_EINVAL, "set_injection: wghyz not finite");
+  t.xpt.assign(xpt, xpt + nxp);
+  t.nyp = nyp;
+  t.nzp = nzp;
+  const size_t ny = (size_t)nyp, nz = (size_t)nzp, nx = (size_t)nxp;
+  const size_t tot = nx + ny + nz + nx + ny * nz;
+  HIPCK(c, hipMalloc(&t.dev, tot * sizeof(double)));
+  double* d = t.dev;
+  HIPCK(c, hipMemcpyAsync(d, xpt, nx * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCK(c, hipMemcpyAsync(d + nx, ypt, ny * 8, hipMemcpyHostToDevice, c->stream));
+  if (D == 3) HIPCK(c, hipMemcpyAsync(d + nx + ny, zpt, nz * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCK(c, hipMemcpyAsync(d + nx + ny + nz, wghpt, nx * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCK(c, hipMemcpyAsync(d + 2 * nx + ny + nz, wghyz, ny * nz * 8, hipMemcpyHostToDevice,
+                          c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));  // the host arrays are only borrowed
+  t.tb = InjTableDev{d, d + nx, d + nx + ny, d + nx + ny + nz, d + 2 * nx + ny + nz, nyp, nzp};
+  t.nptx = nptx;
+  t.targ_in = targ_in;
+  t.targ_end = targ_end;
+  t.active = true;
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_injection_state(aladyn_ctx* c, int32_t ic, int64_t* nptx) {
+  if (int r = check_species(c, ic)) return r;
+  if (!nptx) return fail(c, ALADYN_EINVAL, "nptx is NULL");
+  *nptx = c->inj[ic].nptx;
+  return ALADYN_OK;
+}
+
 extern "C" int aladyn_lp_window_xshift(aladyn_ctx* c, int32_t witr, int32_t init_iter,
                                        double w_speed, int32_t* nshx_out) {
   if (!c) return ALADYN_EINVAL;
@@ -326,22 +430,26 @@ extern "C" int aladyn_lp_window_xshift(aladyn_ctx* c, int32_t witr, int32_t init
   c->win_xlapse = xlapse;
   c->win_wi1 = (int)wi2;
   *nshx_out = (int32_t)nshx;
-  if (nshx == 0) return ALADYN_OK;
-  g.xmn += g.dx * nshx;
-  g.xmx += g.dx * nshx;
-  if (c->have_fields) {
-    if (f64(c))
-      fields_left_xshift<double>(g, c->ebf64, (int)nshx, c->stream);
-    else
-      fields_left_xshift<float>(g, c->ebf32, (int)nshx, c->stream);
-    HIPCK(c, hipGetLastError());
+  if (nshx != 0) {
+    g.xmn += g.dx * nshx;
+    g.xmx += g.dx * nshx;
+    if (c->have_fields) {
+      if (f64(c))
+        fields_left_xshift<double>(g, c->ebf64, (int)nshx, c->stream);
+      else
+        fields_left_xshift<float>(g, c->ebf32, (int)nshx, c->stream);
+      HIPCK(c, hipGetLastError());
+    }
+    // particles behind the new left face leave (cell_part_dist(.true.))
+    for (int ic = 0; ic < c->cfg.nsp; ic++) {
+      std::string msg;
+      if (int r = tile_store_window(&c->ts[ic], c->tg, g, (int)nshx, c->stream, &msg))
+        return fail(c, r, "%s", msg.c_str());
+    }
   }
-  // particles behind the new left face leave (cell_part_dist(.true.))
-  for (int ic = 0; ic < c->cfg.nsp; ic++) {
-    std::string msg;
-    if (int r = tile_store_window(&c->ts[ic], c->tg, g, (int)nshx, c->stream, &msg))
-      return fail(c, r, "%s", msg.c_str());
-  }
+  // plasma entering on the right from the registered target tables
+  for (int ic = 0; ic < c->cfg.nsp; ic++)
+    if (int r = inject_from_table(c, ic)) return r;
   return ALADYN_OK;
 }
 

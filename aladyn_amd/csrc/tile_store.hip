@@ -63,7 +63,7 @@ int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg) {
   const int bz = D == 3 ? bits_for(g.n3p) : 0;
   if (tg->bx + tg->by + bz > 31) return 2;
   tg->block = tile_threads(tg->T[0], tg->T[1], tg->T[2]);
-  tg->sort_every = cfg.sort_every > 0 ? cfg.sort_every : 16;
+  tg->sort_every = cfg.sort_every > 0 ? cfg.sort_every : 64;
   tg->rbytes = cfg.precision == ALADYN_PREC_F64 ? 8 : 4;
   tg->lds_bytes = tile_lds_bytes(D, tg->T[0], tg->T[1], tg->T[2], tg->rbytes);
   if (tg->lds_bytes > 160 * 1024) return 3;
@@ -722,9 +722,36 @@ static int grow_store(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t 
   return sort_into_cur<D, R>(ts, tg, n, s, msg);
 }
 
-// Appends n particles (reference layout, inside the box) to the species:
-// they are converted into the overflow list of the current buffer and the
-// store is re-laid out, so the next step moves them in their tiles
+// Appends n records to the species: fill(dst, id0) writes them into the
+// overflow list of the current buffer (dst points at its first free record,
+// id0 is the first new id) and the store is re-laid out, so the next step
+// moves them in their tiles.
+template <int D, typename R, typename Fill>
+static int append_impl(TileStore* ts, const TileGeom& tg, int64_t n, hipStream_t s,
+                       std::string* msg, Fill&& fill) {
+  int64_t live = 0;
+  if (int r = tile_store_count(ts, tg, s, &live, msg)) return r;
+  if (live + n > ts->ovf_cap)
+    if (int r = grow_store<D, R>(ts, tg, s, std::max(2 * ts->ovf_cap, live + n), msg)) return r;
+  const int cur = ts->cur;
+  int ovf = 0;
+  TS_CK(hipMemcpyAsync(&ovf, ts->ovf_count + cur, sizeof(int), hipMemcpyDeviceToHost, s));
+  TS_CK(hipStreamSynchronize(s));
+  // overflow records are live ones, so ovf + n <= live + n <= ovf_cap
+  RecPtr dst = ts->buf[cur];
+  const int64_t o = ts->ovf_base + ovf;
+  dst.r = (char*)dst.r + rec_size(tg) * (size_t)o;
+  if (dst.id) dst.id += o;
+  fill(dst, ts->next_id);
+  TS_CK(hipGetLastError());
+  ts->next_id += n;
+  const int total = ovf + (int)n;
+  TS_CK(hipMemcpyAsync(ts->ovf_count + cur, &total, sizeof(int), hipMemcpyHostToDevice, s));
+  TS_CK(hipStreamSynchronize(s));
+  return relayout_seg<D, R>(ts, tg, s, msg);
+}
+
+// Appends n particles (reference layout, inside the box) to the species
 // (add_particles / particles_inject, window.f90:39-270).
 template <int D, typename R>
 static int inject_impl(TileStore* ts, const TileGeom& tg, const DevGrid& g, const double* part,
@@ -739,33 +766,71 @@ static int inject_impl(TileStore* ts, const TileGeom& tg, const DevGrid& g, cons
       return ALADYN_EINVAL;
     }
   }
-  int64_t live = 0;
-  if (int r = tile_store_count(ts, tg, s, &live, msg)) return r;
-  if (live + n > ts->ovf_cap)
-    if (int r = grow_store<D, R>(ts, tg, s, std::max(2 * ts->ovf_cap, live + n), msg)) return r;
-  const int cur = ts->cur;
-  int ovf = 0;
-  TS_CK(hipMemcpyAsync(&ovf, ts->ovf_count + cur, sizeof(int), hipMemcpyDeviceToHost, s));
-  TS_CK(hipStreamSynchronize(s));
-  // overflow records are live ones, so ovf + n <= live + n <= ovf_cap
   const int ncol = 2 * D + 1;
   double* dpart = nullptr;
   TS_CK(hipMalloc(&dpart, (size_t)n * ncol * sizeof(double)));
   for (int c = 0; c < ncol; c++)
     TS_CK(hipMemcpyAsync(dpart + c * n, part + c * ld, (size_t)n * sizeof(double),
                          hipMemcpyHostToDevice, s));
-  RecPtr dst = ts->buf[cur];
-  const int64_t o = ts->ovf_base + ovf;
-  dst.r = (char*)dst.r + rec_size(tg) * (size_t)o;
-  if (dst.id) dst.id += o;
-  k_convert_in<D, R><<<grid_for(n), kB, 0, s>>>(g, tg.bx, tg.by, dpart, n, n, dst, ts->next_id);
-  TS_CK(hipGetLastError());
-  ts->next_id += n;
-  const int total = ovf + (int)n;
-  TS_CK(hipMemcpyAsync(ts->ovf_count + cur, &total, sizeof(int), hipMemcpyHostToDevice, s));
-  TS_CK(hipStreamSynchronize(s));
+  const int r = append_impl<D, R>(ts, tg, n, s, msg, [&](RecPtr dst, int64_t id0) {
+    k_convert_in<D, R><<<grid_for(n), kB, 0, s>>>(g, tg.bx, tg.by, dpart, n, n, dst, id0);
+  });
+  (void)hipStreamSynchronize(s);
   (void)hipFree(dpart);
-  return relayout_seg<D, R>(ts, tg, s, msg);
+  return r;
+}
+
+// add_particles (window.f90:39-104) on the device: record q of the columns
+// c0 .. c0+ncol-1 of the target table, in the reference's order (column
+// fastest, then y row j, then z row k), at x = xpt, y = ypt[j], z = zpt[k],
+// p = 0, with the FP32 weight real(wghpt*wghyz(j,k), sp).
+template <int D, typename R>
+__global__ void k_table_in(DevGrid g, int bx, int by, InjTableDev tb, int64_t c0, int64_t ncol,
+                           int64_t n, RecPtr U, int64_t id0) {
+  const double mn[3] = {g.xmn, g.ymn, g.zmn};
+  const double inv[3] = {g.dx_inv, g.dy_inv, g.dz_inv};
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ix = c0 + q % ncol, r = q / ncol;
+    const int j = (int)(r % tb.nyp), k = (int)(r / tb.nyp);
+    const double x[3] = {tb.xpt[ix], tb.ypt[j], D == 3 ? tb.zpt[k] : 0.0};
+    const float w = D == 3 ? (float)(tb.wghpt[ix] * tb.wghyz[j + (int64_t)tb.nyp * k])
+                           : (float)(tb.wghyz[j] * tb.wghpt[ix]);
+    int N[3] = {1, 1, 1};
+    R sv[3] = {R(0), R(0), R(0)}, pv[3] = {R(0), R(0), R(0)};
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      double sd;
+      ref_node(x[d], mn[d], inv[d], N[d], sd);
+      R sf = (R)sd;
+      if (sf >= R(0.5)) {  // keep the [-1/2, 1/2) convention after rounding
+        N[d] += 1;
+        sf -= R(1);
+      }
+      sv[d] = sf;
+    }
+    const int cell = N[0] | (N[1] << bx) | (D == 3 ? (N[2] << (bx + by)) : 0);
+    put_rec<D, R>(U, q, cell, sv, pv, w);
+    if (U.id) U.id[q] = (int)(id0 + q);
+  }
+}
+
+int tile_store_inject_table(TileStore* ts, const TileGeom& tg, const DevGrid& g,
+                            const InjTableDev& tb, int64_t c0, int64_t ncol, hipStream_t s,
+                            std::string* msg) {
+  const int64_t n = ncol * tb.nyp * (g.ndim == 3 ? tb.nzp : 1);
+  if (n <= 0) return 0;
+  if (!ts->loaded) {
+    *msg = "tile store: table injection needs a species that was put first";
+    return ALADYN_ESTATE;
+  }
+  return with_dr(tg, g.ndim, [&](auto dc, auto rv) {
+    constexpr int DD = decltype(dc)::value;
+    using R = decltype(rv);
+    return append_impl<DD, R>(ts, tg, n, s, msg, [&](RecPtr dst, int64_t id0) {
+      k_table_in<DD, R><<<grid_for(n), kB, 0, s>>>(g, tg.bx, tg.by, tb, c0, ncol, n, dst, id0);
+    });
+  });
 }
 
 int tile_store_inject(TileStore* ts, const TileGeom& tg, const DevGrid& g, const double* part,

@@ -92,6 +92,21 @@ int tile_store_window(TileStore* ts, const TileGeom& tg, const DevGrid& g, int w
                       hipStream_t s, std::string* msg);
 int tile_store_inject(TileStore* ts, const TileGeom& tg, const DevGrid& g, const double* part,
                       int64_t ld, int64_t n, hipStream_t s, bool track_ids, std::string* msg);
+// Device copy of a target table for injection at window moves (xpt,
+// wghpt: ncolumns; ypt: nyp; zpt: nzp; wghyz: nyp*nzp, j fastest).
+struct InjTableDev {
+  const double* xpt;
+  const double* ypt;
+  const double* zpt;
+  const double* wghpt;
+  const double* wghyz;
+  int nyp, nzp;
+};
+// Appends the particles of the ncol table columns from c0 (add_particles,
+// window.f90:39-104, cold): charge and part_ind are the species'.
+int tile_store_inject_table(TileStore* ts, const TileGeom& tg, const DevGrid& g,
+                            const InjTableDev& tb, int64_t c0, int64_t ncol, hipStream_t s,
+                            std::string* msg);
 int tile_store_get(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* part, int64_t ld,
                    hipStream_t s, std::string* msg);
 // One species-loop body: gather + push + box test + deposit + re-binning.

@@ -47,6 +47,7 @@ module aladyn_gpu
   public :: aladyn_lpf2_evolve, aladyn_pfields_prepare, aladyn_curr_clean
   public :: aladyn_particle_stage, aladyn_curr_mpi_collect, aladyn_advance_lpf_fields
   public :: aladyn_lp_window_xshift, aladyn_inject_species, aladyn_box
+  public :: aladyn_set_injection, aladyn_injection_state
   public :: aladyn_energy_momenta, aladyn_field_energy, aladyn_grid_charge
   public :: aladyn_continuity_residual, aladyn_set_diagnostics
   public :: aladyn_synchronize, aladyn_kernel_timing
@@ -164,6 +165,24 @@ module aladyn_gpu
       integer(c_int32_t), value :: ic
       real(c_double), intent(in) :: part(*)
       integer(c_int64_t), value :: ld, np
+    end function
+
+    integer(c_int) function aladyn_set_injection(ctx, ic, xpt, nxp, ypt, nyp, zpt, nzp, &
+        wghpt, wghyz, nptx, targ_in, targ_end) bind(C, name="aladyn_set_injection")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ic, nyp, nzp
+      integer(c_int64_t), value :: nxp, nptx
+      real(c_double), intent(in) :: xpt(*), ypt(*), zpt(*), wghpt(*), wghyz(*)
+      real(c_double), value :: targ_in, targ_end
+    end function
+
+    integer(c_int) function aladyn_injection_state(ctx, ic, nptx) &
+        bind(C, name="aladyn_injection_state")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ic
+      integer(c_int64_t), intent(out) :: nptx
     end function
 
     integer(c_int) function aladyn_box(ctx, out) bind(C, name="aladyn_box")

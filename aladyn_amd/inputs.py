@@ -91,6 +91,36 @@ def uniform_plasma(cfg: LPConfig, ppc, *, x_start_cell: float = 0.0, x_end_cell:
     return part
 
 
+def plasma_table(cfg: LPConfig, ppc, *, x_start_cell: float = 0.0, x_end_cell: float | None = None,
+                 n_rel: float = 1.0) -> dict:
+    """Target table of a uniform plasma for injection at window moves: the
+    arrays particles_inject / add_particles read (window.f90:39-139), with
+    particles at sub-cell centres (init_part_distrib.f90:513-516).
+
+    xpt: x of the plasma columns from cell x_start_cell to x_end_cell (cells
+    of the initial box, which may lie beyond it), ppc[0] per cell; ypt, zpt:
+    the transverse sub-cell centres of the box; wghpt = n_rel*n0_ref/ppc per
+    column and wghyz = 1 (the weight of add_particles is their product).
+    The particles of the columns inside the initial box are those of
+    uniform_plasma(..., jitter=False) with the same arguments.
+    """
+    D = cfg.curr_ndim
+    ppc = tuple(int(v) for v in ppc)[:D]
+    x1 = cfg.nx if x_end_cell is None else x_end_cell
+    cells_x = np.arange(int(math.ceil(x_start_cell)), int(math.ceil(x1)))
+    sub = lambda n: (np.arange(n) + 0.5) / n
+    xpt = cfg.xmn + (cells_x[:, None] + sub(ppc[0])[None, :]).ravel() * cfg.dx
+    ypt = cfg.ymn + (np.arange(cfg.ny)[:, None] + sub(ppc[1])[None, :]).ravel() * cfg.dy
+    table = {"xpt": xpt, "ypt": ypt,
+             "wghpt": np.full(xpt.size, n_rel * cfg.n0_ref / float(np.prod(ppc)))}
+    if D == 3:
+        table["zpt"] = cfg.zmn + (np.arange(cfg.nz)[:, None] + sub(ppc[2])[None, :]).ravel() * cfg.dz
+        table["wghyz"] = np.ones((ypt.size, table["zpt"].size), order="F")
+    else:
+        table["wghyz"] = np.ones((ypt.size, 1), order="F")
+    return table
+
+
 def gaussian_laser(cfg: LPConfig, *, a0: float, w0: float, tau_fwhm: float, xc: float,
                    lam0: float | None = None, polarization: str = "y") -> np.ndarray:
     """Linearly polarised Gaussian pulse, paraxial leading order, at its focus.

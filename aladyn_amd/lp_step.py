@@ -184,6 +184,29 @@ class LPStep:
         p = np.asfortranarray(part, dtype=np.float64)
         self._ck(self.lib.aladyn_inject_species(self._h, ic, _dptr(p), p.shape[0], p.shape[0]))
 
+    def set_injection(self, ic: int, table: dict, nptx: int = 0, targ_in: float | None = None,
+                      targ_end: float | None = None):
+        """Registers species ic's target table (particles_inject /
+        add_particles, window.f90:39-139): a dict with xpt, ypt, zpt (3-D),
+        wghpt and wghyz (nyp x nzp).  targ_in/targ_end default to the table's
+        first and last column plus one cell."""
+        xpt = np.ascontiguousarray(table["xpt"], dtype=np.float64)
+        ypt = np.ascontiguousarray(table["ypt"], dtype=np.float64)
+        zpt = np.ascontiguousarray(table.get("zpt", np.ones(1)), dtype=np.float64)
+        wghpt = np.ascontiguousarray(table["wghpt"], dtype=np.float64)
+        wghyz = np.asfortranarray(table["wghyz"], dtype=np.float64).ravel(order="F")
+        t_in = float(xpt[0]) if targ_in is None else float(targ_in)
+        t_end = float(xpt[-1] + self.cfg.dx) if targ_end is None else float(targ_end)
+        self._ck(self.lib.aladyn_set_injection(self._h, ic, _dptr(xpt), xpt.size, _dptr(ypt),
+                                               ypt.size, _dptr(zpt), zpt.size, _dptr(wghpt),
+                                               _dptr(wghyz), int(nptx), t_in, t_end))
+
+    def injection_state(self, ic: int) -> int:
+        """Columns of species ic's target table consumed so far (nptx)."""
+        n = C.c_int64()
+        self._ck(self.lib.aladyn_injection_state(self._h, ic, C.byref(n)))
+        return int(n.value)
+
     def box(self) -> np.ndarray:
         """The current particle box (xmin, xmax, ymin, ymax, zmin, zmax)."""
         out = np.zeros(6)

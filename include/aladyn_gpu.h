@@ -141,6 +141,30 @@ int aladyn_lp_window_xshift(aladyn_ctx *ctx, int32_t witr, int32_t init_iter,
  * box.  They are moved from the next step on. */
 int aladyn_inject_species(aladyn_ctx *ctx, int32_t ic, const double *part,
                           int64_t ld, int64_t np);
+/* Registers the target table of species ic, from which each
+ * aladyn_lp_window_xshift call injects the plasma entering on the right on
+ * the device (particles_inject and add_particles, window.f90:39-139, cold
+ * target: t0_pl = 0): xpt[0..nxp) the ascending x of the plasma columns
+ * (xpt(:,ic)), ypt[0..nyp) and zpt[0..nzp) the transverse positions of the
+ * box (loc_ypt, loc_zpt; zpt and nzp ignored in 2-D), wghpt[nxp] the column
+ * weights and wghyz[nyp*nzp] the transverse weights (loc_wghyz, j fastest),
+ * nptx the columns already in the box (nptx(ic)) and [targ_in, targ_end] the
+ * target's x extent.  After a move, while targ_in <= xmax < targ_end,
+ * columns nptx+1 .. i2 are appended with the reference's selection (1-based:
+ * the scan stops at the first xpt > xmax, i2 is the column before it, or
+ * that column itself when it is the last; nothing is added unless i2 > i1;
+ * nptx becomes i2), in the reference's order (column fastest, then y, then
+ * z), with p = 0, weight real(wghpt*wghyz, sp) and the species' charge and
+ * part_ind.  Columns outside (xmin, xmax] are consumed but not added (the
+ * reference would remove them before their first deposit).  nxp = 0 clears
+ * the table.  The species must have been put first (ALADYN_ESTATE at the
+ * move otherwise); ypt/zpt outside the box give ALADYN_ERANGE. */
+int aladyn_set_injection(aladyn_ctx *ctx, int32_t ic, const double *xpt, int64_t nxp,
+                         const double *ypt, int32_t nyp, const double *zpt, int32_t nzp,
+                         const double *wghpt, const double *wghyz, int64_t nptx,
+                         double targ_in, double targ_end);
+/* The columns of species ic's target table consumed so far (nptx(ic)). */
+int aladyn_injection_state(aladyn_ctx *ctx, int32_t ic, int64_t *nptx);
 /* The current particle box: out = {xmin, xmax, ymin, ymax, zmin, zmax}
  * (loc_*grid%gmin/gmax; x moves with the window). */
 int aladyn_box(aladyn_ctx *ctx, double out[6]);

@@ -1117,3 +1117,66 @@ int orc_window_xshift(orc_grid *g, orc_window *w, int witr, int init_iter,
   *np = p;
   return nshx;
 }
+
+int64_t orc_inject_select(const double *xpt, int64_t nxp, int64_t *nptx, double xmx,
+                          double targ_in, double targ_end, int64_t *c0, int64_t *c1) {
+  /* lp_window_xshift's gate (window.f90:339-341), then particles_inject
+   * (window.f90:106-139) in the reference's 1-based indices */
+  *c0 = 0;
+  *c1 = -1;
+  if (!(targ_in <= xmx && targ_end > xmx)) return 0;
+  const int64_t i1 = *nptx + 1;
+  int64_t i2;
+  if (i1 <= nxp) {
+    int64_t ix = i1;
+    for (; ix <= nxp; ix++)
+      if (xpt[ix - 1] > xmx) break;
+    i2 = ix - 1;
+    if (ix == nxp) i2 = ix;
+  } else {
+    i2 = i1 - 1;
+  }
+  *nptx = i2;
+  if (!(i2 > i1)) return 0;
+  int64_t last = i2;
+  while (last >= i1 && xpt[last - 1] > xmx) last--;
+  if (last < i1) return 0;
+  *c0 = i1 - 1;
+  *c1 = last - 1;
+  return last - i1 + 1;
+}
+
+int64_t orc_add_particles(const orc_grid *g, const double *xpt, int64_t c0, int64_t c1,
+                          const double *ypt, int nyp, const double *zpt, int nzp,
+                          const double *wghpt, const double *wghyz, int charge,
+                          double *part, int64_t ld, int64_t np) {
+  /* add_particles (window.f90:39-104) with t0_pl = 0 */
+  const int D = g->curr_ndim;
+  int64_t n = np;
+  if (D > 2) {
+    for (int k = 0; k < nzp; k++)
+      for (int j = 0; j < nyp; j++)
+        for (int64_t ix = c0; ix <= c1; ix++) {
+          const double whz = wghpt[ix] * wghyz[j + (int64_t)nyp * k];
+          PC(part, n, 1) = xpt[ix];
+          PC(part, n, 2) = ypt[j];
+          PC(part, n, 3) = zpt[k];
+          PC(part, n, 4) = 0.0;
+          PC(part, n, 5) = 0.0;
+          PC(part, n, 6) = 0.0;
+          PC(part, n, 7) = orc_pack_weight(charge, 0, (float)whz);
+          n++;
+        }
+  } else {
+    for (int j = 0; j < nyp; j++)
+      for (int64_t ix = c0; ix <= c1; ix++) {
+        PC(part, n, 1) = xpt[ix];
+        PC(part, n, 2) = ypt[j];
+        PC(part, n, 3) = 0.0;
+        PC(part, n, 4) = 0.0;
+        PC(part, n, 5) = orc_pack_weight(charge, 0, (float)(wghyz[j] * wghpt[ix]));
+        n++;
+      }
+  }
+  return n;
+}

@@ -148,6 +148,30 @@ int orc_window_xshift(orc_grid *g, orc_window *w, int witr, int init_iter,
                       double w_speed, double *ebf, int64_t *np, double *part,
                       int64_t ld);
 
+/* particles_inject's column selection (window.f90:106-139) on a target
+ * table xpt[0..nxp) of ascending column positions, nptx columns of which are
+ * already in the box.  With 1-based Fortran indices: i1 = nptx + 1; if
+ * i1 <= nxp, ix runs from i1 until xpt(ix) > xmx, i2 = ix - 1, and
+ * i2 = ix when the loop stopped at ix = nxp; otherwise i2 = i1 - 1.  nptx
+ * becomes i2 and columns i1..i2 are added only when i2 > i1.  The call
+ * itself is gated as in lp_window_xshift (window.f90:339-341): nothing
+ * happens unless targ_in <= xmx < targ_end.  Returns the number of columns
+ * to add (0 if none) and their 0-based range [*c0, *c1]; *nptx is updated.
+ * Columns with xpt > xmx inside [i1, i2] (the i2 = ix rule at the end of
+ * the table) are counted in *nptx but left out of the range. */
+int64_t orc_inject_select(const double *xpt, int64_t nxp, int64_t *nptx, double xmx,
+                          double targ_in, double targ_end, int64_t *c0, int64_t *c1);
+/* add_particles (window.f90:39-104), cold target (t0_pl = 0): appends the
+ * columns c0..c1 (0-based) of the table at part rows np.. in the
+ * reference's order (3-D: z row k, then y row j, then column; 2-D: y row,
+ * then column), x = xpt, y = ypt[j], z = zpt[k], p = 0, W packed from
+ * charge, part_ind 0 and the FP32 weight real(wghpt*wghyz(j,k), sp)
+ * (wghyz(j,k) at j + nyp*k).  Returns the new particle count. */
+int64_t orc_add_particles(const orc_grid *g, const double *xpt, int64_t c0, int64_t c1,
+                          const double *ypt, int nyp, const double *zpt, int nzp,
+                          const double *wghpt, const double *wghyz, int charge,
+                          double *part, int64_t ld, int64_t np);
+
 #ifdef __cplusplus
 }
 #endif

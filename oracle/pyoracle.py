@@ -90,6 +90,10 @@ class Oracle:
             "orc_field_energy": [_G, _D, _D],
             "orc_window_xshift": [_G, C.POINTER(OrcWindow), C.c_int, C.c_int, C.c_double, _D,
                                   C.POINTER(C.c_int64), _D, _I64],
+            "orc_inject_select": [_D, _I64, C.POINTER(C.c_int64), C.c_double, C.c_double,
+                                  C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+            "orc_add_particles": [_G, _D, _I64, _I64, _D, C.c_int, _D, C.c_int, _D, _D, C.c_int,
+                                  _D, _I64, _I64],
         }
         for name, at in sig.items():
             fn = getattr(L, name)
@@ -98,6 +102,8 @@ class Oracle:
                                           "orc_esirkepov_skipped") else None
             if name == "orc_window_xshift":
                 fn.restype = C.c_int
+            if name in ("orc_inject_select", "orc_add_particles"):
+                fn.restype = _I64
         self.L = L
         self.cfg = cfg
         g = OrcGrid()
@@ -193,6 +199,34 @@ class Oracle:
         nshx = self.L.orc_window_xshift(C.byref(self.g), C.byref(win), witr, init_iter, w_speed,
                                         _p(_f(ebf)), C.byref(n), _p(_f(part)), self._ld(part))
         return int(nshx), np.asfortranarray(part[:n.value])
+
+    def inject_select(self, xpt, nptx, xmx, targ_in, targ_end):
+        """particles_inject's column selection: returns (ncols, c0, c1, new nptx)."""
+        xpt = np.ascontiguousarray(xpt, dtype=np.float64)
+        n = C.c_int64(nptx)
+        c0, c1 = C.c_int64(0), C.c_int64(0)
+        k = self.L.orc_inject_select(_p(xpt), xpt.size, C.byref(n), xmx, targ_in, targ_end,
+                                     C.byref(c0), C.byref(c1))
+        return int(k), int(c0.value), int(c1.value), int(n.value)
+
+    def add_particles(self, part, table, c0, c1):
+        """add_particles of columns c0..c1 of table (a dict: xpt, ypt, zpt,
+        wghpt, wghyz, charge); returns part with the new rows appended."""
+        ypt = np.ascontiguousarray(table["ypt"], dtype=np.float64)
+        zpt = np.ascontiguousarray(table.get("zpt", np.zeros(1)), dtype=np.float64)
+        xpt = np.ascontiguousarray(table["xpt"], dtype=np.float64)
+        wghpt = np.ascontiguousarray(table["wghpt"], dtype=np.float64)
+        wghyz = np.ascontiguousarray(table["wghyz"], dtype=np.float64).ravel(order="F")
+        nyp, nzp = ypt.size, zpt.size if self.cfg.curr_ndim == 3 else 1
+        add = max(c1 - c0 + 1, 0) * nyp * nzp
+        np0 = part.shape[0]
+        out = np.zeros((np0 + add, part.shape[1]), order="F")
+        out[:np0] = part
+        n = self.L.orc_add_particles(C.byref(self.g), _p(xpt), c0, c1, _p(ypt), nyp, _p(zpt), nzp,
+                                     _p(wghpt), _p(wghyz), int(table["charge"]), _p(out),
+                                     out.shape[0], np0)
+        assert n == np0 + add
+        return out
 
     def grid_charge(self, part):
         rho = np.zeros((self.cfg.n1p, self.cfg.n2p, self.cfg.n3p), dtype=np.float64, order="F")

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from aladyn_amd import PREC_F32, PREC_F64
-from aladyn_amd.inputs import uniform_plasma
+from aladyn_amd.inputs import plasma_table, uniform_plasma, unpack_weight
 from cases import small_2d, small_3d
 from conftest import rel_maxerr
 from oracle.pyoracle import Oracle, OrcWindow
@@ -47,6 +47,59 @@ def test_oracle_window_shift_moves_fields_box_and_particles():
     # the second move: nint of the accumulated lapse minus what has moved
     nsh2, _ = orc.window_xshift(win, 10, 20, 1.0, ebf, kept.copy(order="F"))
     assert nsh + nsh2 == int(np.floor(cfg.dt * 20 / cfg.dx + 0.5))
+
+
+@pytest.mark.parametrize("nptx,xmx,targ,expect", [
+    # 1-based reading of particles_inject (window.f90:106-139) on
+    # xpt = 0.1 .. 0.5: (columns added, first, last (0-based), new nptx)
+    (0, 0.35, (0.0, 1.0), (3, 0, 2, 3)),   # scan stops at 0.4 > 0.35: i2 = 3
+    (1, 10.0, (0.0, 11.0), (4, 1, 4, 5)),  # scan runs off the end: i2 = 5
+    (3, 0.45, (0.0, 1.0), (1, 3, 3, 5)),   # stops at the last column: i2 = 5,
+                                           # column 5 (0.5 > xmax) consumed, not added
+    (0, 0.15, (0.0, 1.0), (0, 0, -1, 1)),  # i2 = i1 = 1: nothing added, nptx = 1
+    (5, 0.9, (0.0, 1.0), (0, 0, -1, 5)),   # table used up
+    (0, 0.35, (0.0, 0.3), (0, 0, -1, 0)),  # targ_end <= xmax: no call, nptx kept
+    (0, 0.35, (0.4, 1.0), (0, 0, -1, 0)),  # targ_in > xmax: no call
+])
+def test_oracle_inject_select_follows_the_reference_rules(nptx, xmx, targ, expect):
+    cfg, _, _ = small_2d()
+    orc = Oracle(cfg)
+    xpt = np.array([0.1, 0.2, 0.3, 0.4, 0.5])
+    assert orc.inject_select(xpt, nptx, xmx, *targ) == expect
+
+
+@pytest.mark.parametrize("dim", ["2d", "3d"])
+def test_oracle_add_particles_order_and_weights(dim):
+    """add_particles (window.f90:39-104): column fastest, then y, then z;
+    weight real(wghpt*wghyz, sp); p = 0; charge and part_ind 0."""
+    cfg, _, _ = {"2d": small_2d, "3d": small_3d}[dim]()
+    orc = Oracle(cfg)
+    D = cfg.curr_ndim
+    table = {"xpt": cfg.xmn + cfg.dx * np.array([1.25, 1.75, 2.25, 2.75]),
+             "ypt": cfg.ymn + cfg.dy * np.array([0.5, 3.5, 6.5]),
+             "wghpt": np.array([0.1, 0.2, 0.3, 0.7]), "charge": -1}
+    if D == 3:
+        table["zpt"] = cfg.zmn + cfg.dz * np.array([1.5, 4.5])
+        table["wghyz"] = np.array([[1.0, 3.0], [0.5, 2.0], [0.25, 1.0]])
+    else:
+        table["wghyz"] = np.array([[1.0], [0.5], [0.25]])
+    start = np.zeros((2, cfg.ncol), order="F")
+    out = orc.add_particles(start, table, 1, 2)
+    new = out[2:]
+    nz = 2 if D == 3 else 1
+    assert new.shape[0] == 2 * 3 * nz and np.array_equal(out[:2], start)
+    q = 0
+    for k in range(nz):
+        for j in range(3):
+            for ix in (1, 2):
+                assert new[q, 0] == table["xpt"][ix] and new[q, 1] == table["ypt"][j]
+                if D == 3:
+                    assert new[q, 2] == table["zpt"][k]
+                assert np.all(new[q, D:2 * D] == 0.0)
+                c, pi, w = unpack_weight(new[q:q + 1, 2 * D])
+                assert (c[0], pi[0]) == (-1, 0)
+                assert w[0] == np.float32(table["wghpt"][ix] * table["wghyz"][j, k])
+                q += 1
 
 
 def _entering_plasma(cfg, nsh, box_xmn, seed):
@@ -95,4 +148,57 @@ def test_window_run_matches_oracle(dim, precision, tol):
     assert rel_maxerr(pg[:, D:2 * D], p[:, D:2 * D]) <= tol
     assert rel_maxerr(eg, e) <= tol
     if precision == PREC_F64:  # FP32 currents are compared by the one-step tests
+        assert rel_maxerr(jg, jc) <= tol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [(PREC_F64, 1e-10), (PREC_F32, 1e-4)])
+@pytest.mark.parametrize("dim", ["2d", "3d"])
+def test_window_table_injection_matches_oracle(dim, precision, tol):
+    """The device injects the entering plasma from the registered target
+    table at each move; the oracle selects and adds the same columns."""
+    from aladyn_amd import LPStep
+
+    cfg, _, ebf = {"2d": small_2d, "3d": small_3d}[dim]()
+    ppc = (2, 2) if cfg.ndim == 2 else (2, 2, 2)
+    table = plasma_table(cfg, ppc, x_start_cell=cfg.nx // 2, x_end_cell=cfg.nx + 40)
+    table["charge"] = -1
+    orc = Oracle(cfg)
+    nin = int(np.sum(table["xpt"] <= cfg.xmx))  # columns inside the initial box
+    part = orc.add_particles(np.zeros((0, cfg.ncol), order="F"), table, 0, nin - 1)
+    targ = (table["xpt"][0], table["xpt"][-1] + cfg.dx)
+    w_sh, w_speed = 4, 1.0
+    print(f"window table {dim} precision={precision} np={part.shape[0]}", flush=True)
+    win = OrcWindow()
+    p, e, jc = part.copy(order="F"), ebf.copy(order="F"), orc.zeros_current()
+    nptx = nin
+    with LPStep(cfg, precision, track_ids=True) as gpu:
+        gpu.put_species(0, part)
+        gpu.put_fields(ebf)
+        gpu.set_injection(0, table, nptx=nin, targ_in=targ[0], targ_end=targ[1])
+        assert gpu.lp_window_xshift(w_sh, 0, w_speed) == 0
+        orc.window_xshift(win, w_sh, 0, w_speed, e, p)
+        added = 0
+        for it in range(3 * w_sh):
+            p = orc.lpf2_evolve(p, e, jc, initial_time=(it == 0))
+            gpu.lpf2_evolve(it, initial_time=(it == 0))
+            if (it + 1) % w_sh == 0:
+                nsh, p = orc.window_xshift(win, w_sh, it + 1, w_speed, e, p)
+                k, c0, c1, nptx = orc.inject_select(table["xpt"], nptx, orc.g.xmx, *targ)
+                if k:
+                    p = orc.add_particles(p, table, c0, c1)
+                    added += k
+                assert gpu.lp_window_xshift(w_sh, it + 1, w_speed) == nsh > 0
+                assert gpu.injection_state(0) == nptx
+        assert added > 0
+        pg, eg, jg = gpu.get_species(0), gpu.get_fields(), gpu.get_current()
+    assert pg.shape == p.shape, (pg.shape, p.shape)
+    D = cfg.curr_ndim
+    for d in range(D):
+        span = (cfg.xmx, cfg.ymx, cfg.zmx)[d] - (cfg.xmn, cfg.ymn, cfg.zmn)[d]
+        assert np.max(np.abs(pg[:, d] - p[:, d])) / span <= tol
+    assert rel_maxerr(pg[:, D:2 * D], p[:, D:2 * D]) <= tol
+    assert np.array_equal(pg[:, 2 * D], p[:, 2 * D])
+    assert rel_maxerr(eg, e) <= tol
+    if precision == PREC_F64:
         assert rel_maxerr(jg, jc) <= tol
