@@ -174,7 +174,7 @@ def main():
         f"particle kernel {r32['part_ms']:.3f} ms, fields {r32['field_ms']:.3f} ms")
     r64 = None
     if not args.no_fp64:
-        r64 = run_path(cfg, part, ebf, PREC_F64, max(5, args.steps // 5), 2)
+        r64 = run_path(cfg, part, ebf, PREC_F64, args.steps, args.warmup)  # the same steps
         log(f"fp64: {r64['ms_per_step']:.3f} ms/step, {r64['pps']:.4g} particle-steps/s, "
             f"particle kernel {r64['part_ms']:.3f} ms, fields {r64['field_ms']:.3f} ms")
 
