@@ -5,7 +5,11 @@ Workload: 3-D LWFA, grid 256x128x128, dx = 0.05, dy = dz = 0.2 (micron),
 n/nc = 0.005, 8 particles per cell on a 2x2x2 sub-grid with seeded jitter
 (33,554,432 electrons), linearly polarised Gaussian pulse a0 = 3, waist 10.
 A step is one full lpf2_evolve: field ghosts, gather, push, removal,
-Esirkepov deposit, current fold/normalisation and the Yee update.
+Esirkepov deposit, current fold/normalisation and the Yee update.  The
+default run times 200 steps (after 5 untimed ones) and, as the
+configuration has it, moves the window from step 100 on (every 20 steps, at
+c), with the entering plasma injected on the device from a target table;
+--no-window leaves the box fixed.
 
 The FP32 tile path is the measured value; the FP64 tile path (the same
 kernels on 64-byte double records) runs the same workload for the FP32/FP64
@@ -23,7 +27,7 @@ profiles/k_tile_3d_lwfa_256x128x128.md when it covers this workload; null
 otherwise.  `--calibrate-copy` runs only that copy (1 GiB read + 1 GiB
 written) for the calibration pass.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu] [--no-window]
        python bench.py --calibrate-copy
 """
 from __future__ import annotations
@@ -72,21 +76,44 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_path(cfg, part, ebf, precision, steps, warmup):
+# Moving window of the 3-D LWFA configuration ("enabled after step 100"):
+# lp_run's call (pic_evolve_in_time.f90:124-131) every WINDOW_EVERY steps from
+# WINDOW_START on, at the speed of light; the cadence w_sh is not given by the
+# configuration and is chosen here.
+WINDOW_START, WINDOW_EVERY, WINDOW_SPEED = 100, 20, 1.0
+
+
+def run_path(cfg, part, ebf, precision, steps, warmup, table=None):
+    """Times `steps` lpf2_evolve calls after `warmup` untimed ones.  With a
+    target table the box moves as lp_run moves it and the entering plasma is
+    injected on the device from the table."""
     from aladyn_amd import LPStep
 
     gpu = LPStep(cfg, precision)
     gpu.put_species(0, part)
     gpu.put_fields(ebf)
+    moves = 0
+    if table is not None:
+        gpu.set_injection(0, table, nptx=0, targ_in=cfg.xmn,
+                          targ_end=float(table["xpt"][-1]) + cfg.dx)
+
+    def step(it):
+        nonlocal moves
+        gpu.lpf2_evolve(it, initial_time=(it == 0))
+        if table is not None and it >= WINDOW_START and it % WINDOW_EVERY == 0:
+            gpu.lp_window_xshift(WINDOW_EVERY, it, WINDOW_SPEED)
+            moves += 1
+
     gpu.synchronize()
     for it in range(warmup):
-        gpu.lpf2_evolve(it, initial_time=(it == 0))
+        step(it)
     gpu.synchronize()
     n0 = gpu.species_count(0)
     gpu.kernel_timing(reset=True)
+    moves = 0
     t0 = time.perf_counter()
     for it in range(warmup, warmup + steps):
-        gpu.lpf2_evolve(it, initial_time=(it == 0))
+        step(it)
     gpu.synchronize()
     t1 = time.perf_counter()
     part_ms, field_ms = gpu.kernel_timing()
@@ -96,7 +123,7 @@ def run_path(cfg, part, ebf, precision, steps, warmup):
     gpu.close()
     wall = t1 - t0
     n_avg = 0.5 * (n0 + n1)
-    return dict(wall=wall, ms_per_step=1e3 * wall / steps, n0=n0, n1=n1,
+    return dict(wall=wall, ms_per_step=1e3 * wall / steps, n0=n0, n1=n1, moves=moves,
                 pps=n_avg * steps / wall, part_ms=part_ms, field_ms=field_ms,
                 kinetic=float(ek[6]), field_energy=float(fe.sum()))
 
@@ -144,7 +171,7 @@ def cpu_baseline(cfg, part, ebf, budget_s=15.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -152,6 +179,7 @@ def main():
     ap.add_argument("--ny", type=int, default=128)
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--no-window", action="store_true")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -162,21 +190,26 @@ def main():
         sys.exit(2)
 
     from aladyn_amd import PREC_F32, PREC_F64
-    from aladyn_amd.inputs import lwfa_3d_case
+    from aladyn_amd.inputs import lwfa_3d_case, plasma_table
 
     t = time.perf_counter()
     cfg, part, ebf = lwfa_3d_case(args.nx, args.ny, args.nz, seed=0)
     log(f"inputs: {part.shape[0]} particles, grid {cfg.nx}x{cfg.ny}x{cfg.nz} "
         f"({time.perf_counter() - t:.1f} s)")
 
-    r32 = run_path(cfg, part, ebf, PREC_F32, args.steps, args.warmup)
+    # the plasma continues beyond the box: the columns the window brings in
+    table = None if args.no_window else plasma_table(cfg, (2, 2, 2), x_start_cell=cfg.nx,
+                                                     x_end_cell=cfg.nx + 400)
+    r32 = run_path(cfg, part, ebf, PREC_F32, args.steps, args.warmup, table)
     log(f"fp32: {r32['ms_per_step']:.3f} ms/step, {r32['pps']:.4g} particle-steps/s, "
-        f"particle kernel {r32['part_ms']:.3f} ms, fields {r32['field_ms']:.3f} ms")
+        f"particle kernel {r32['part_ms']:.3f} ms, fields {r32['field_ms']:.3f} ms, "
+        f"{r32['moves']} window moves")
     r64 = None
     if not args.no_fp64:
-        r64 = run_path(cfg, part, ebf, PREC_F64, args.steps, args.warmup)  # the same steps
+        r64 = run_path(cfg, part, ebf, PREC_F64, args.steps, args.warmup, table)  # same steps
         log(f"fp64: {r64['ms_per_step']:.3f} ms/step, {r64['pps']:.4g} particle-steps/s, "
-            f"particle kernel {r64['part_ms']:.3f} ms, fields {r64['field_ms']:.3f} ms")
+            f"particle kernel {r64['part_ms']:.3f} ms, fields {r64['field_ms']:.3f} ms, "
+            f"{r64['moves']} window moves")
 
     n_p = 0.5 * (r32["n0"] + r32["n1"])
     cells = cfg.nx * cfg.ny * cfg.nz
@@ -210,6 +243,10 @@ def main():
             "particles": int(r32["n0"]),
             "cells": cells,
             "path": "fp32 tile-sorted (int32 cell + fp32 offset)",
+            "steps_timed": f"{args.warmup}..{args.warmup + args.steps - 1}",
+            "moving_window": None if table is None else
+            f"from step {WINDOW_START}, every {WINDOW_EVERY} steps at c, plasma injected from "
+            f"the target table ({r32['moves']} moves timed)",
         },
         "roofline": {
             "bound": "hbm",
