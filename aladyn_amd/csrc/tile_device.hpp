@@ -131,22 +131,36 @@ __device__ inline void load_rec(const RecPtr& r, int64_t n, Rec<R>& q, int bx, i
   decode_cell<D, R>(q, bx, by);
 }
 
-// writes a record from its fields (packed cell bits given)
+// the words of a record from its fields (packed cell bits given)
 template <int D, typename R>
-__device__ inline void put_rec(const RecPtr& r, int64_t n, int cell, const R s[3], const R p[3],
-                               float w) {
+__device__ inline void rec_words(int cell, const R s[3], const R p[3], float w,
+                                 typename RecWord<R>::V o[RecWord<R>::N]) {
   if constexpr (sizeof(R) == 4) {
-    float4* o = rec_at<R>(r, n);
     o[0] = make_float4(__int_as_float(cell), s[0], s[1], D == 3 ? s[2] : 0.f);
     o[1] = make_float4(p[0], p[1], D == 3 ? p[2] : 0.f, w);
   } else {
-    double2* o = rec_at<R>(r, n);
     const long long cw = (long long)(unsigned)cell | ((long long)__float_as_int(w) << 32);
     o[0] = make_double2(__longlong_as_double(cw), s[0]);
     o[1] = make_double2(s[1], D == 3 ? s[2] : 0.0);
     o[2] = make_double2(p[0], p[1]);
     o[3] = make_double2(D == 3 ? p[2] : 0.0, 0.0);
   }
+}
+
+// writes a record from its fields (packed cell bits given)
+template <int D, typename R>
+__device__ inline void put_rec(const RecPtr& r, int64_t n, int cell, const R s[3], const R p[3],
+                               float w) {
+  typename RecWord<R>::V o[RecWord<R>::N];
+  rec_words<D, R>(cell, s, p, w, o);
+  typename RecWord<R>::V* d = rec_at<R>(r, n);
+#pragma unroll
+  for (int k = 0; k < RecWord<R>::N; k++) d[k] = o[k];
+}
+
+template <int D>
+__device__ inline int pack_cell(const int N[3], int bx, int by) {
+  return N[0] | (N[1] << bx) | (D == 3 ? (N[2] << (bx + by)) : 0);
 }
 
 template <int D, typename R>

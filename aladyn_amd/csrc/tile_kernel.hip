@@ -308,6 +308,8 @@ struct CellAcc {
 //   Jl  double [D][PVOL]    current patch, nodes T0-2 .. T0+T+1 per axis
 //   Q   per wave: R s[6][QCAP], float wq[QCAP], int meta[QCAP]: the queue of
 //       particles whose node moved on two or three axes
+//   M   per wave: the record words of MCAP movers (word-major), their
+//       destination tiles and ids
 //   offs, fillc int [NCL+1], [NCL]; misc int [4]; wsum int [NW]; perm u16 [kChunk]
 template <int D, int TX, int TY, int TZ, int NT, typename R>
 struct TileShape {
@@ -319,8 +321,11 @@ struct TileShape {
   static constexpr int J_BYTES = 8 * D * PVOL;
   static constexpr int QW_BYTES = QCAP * (6 * (int)sizeof(R) + 8);  // one wave's queue
   static constexpr int Q_BYTES = NW * QW_BYTES;
+  static constexpr int MCAP = 64;  // movers staged per wave
+  static constexpr int MW_BYTES = MCAP * (8 * (int)sizeof(R) + 8);
+  static constexpr int M_BYTES = NW * MW_BYTES;
   static constexpr int I_BYTES = 4 * (2 * NCL + 5 + NW);
-  static constexpr int BYTES = F_BYTES + J_BYTES + Q_BYTES + I_BYTES + 2 * kChunk;
+  static constexpr int BYTES = F_BYTES + J_BYTES + Q_BYTES + M_BYTES + I_BYTES + 2 * kChunk;
   static_assert(NT % 64 == 0 && NT >= NCL, "threads: whole wavefronts, at least one per cell");
   static_assert(kChunk % NT == 0, "a chunk is a whole number of records per thread");
 };
@@ -369,6 +374,48 @@ __device__ inline void drain_queue(const WaveQueue<R, 64>& qu, int n, double* Jl
   }
 }
 
+// One wave's staging buffer of particles that move to another tile: their
+// slots in the destination segments are reserved when the buffer is written
+// out, one atomic per destination for up to 64 of them, instead of one
+// atomic round trip in every round.
+template <typename R, int MCAP>
+struct MoverStage {
+  using V = typename RecWord<R>::V;
+  static constexpr int NWD = RecWord<R>::N;
+  V* w;       // [NWD][MCAP]
+  int* tile;  // [MCAP]
+  int* id;    // [MCAP]
+  __device__ MoverStage(char* base)
+      : w((V*)base), tile((int*)(base + NWD * MCAP * sizeof(V))),
+        id((int*)(base + NWD * MCAP * sizeof(V) + 4 * MCAP)) {}
+};
+
+// Writes the n <= 64 staged movers, one per lane, into their destination
+// segments of B (the overflow list when a segment is full).
+template <typename R, int MCAP>
+__device__ inline void flush_movers(const MoverStage<R, MCAP>& ms, int n, const RecPtr& B,
+                                    const int* __restrict__ seg_start,
+                                    const int* __restrict__ seg_cap, int* __restrict__ cnt_next,
+                                    int* __restrict__ ovf_next, int* __restrict__ grow,
+                                    int64_t ovf_base) {
+  const int lane = (int)__lane_id();
+  if (lane < n) {
+    const int dtile = ms.tile[lane];
+    const int cap = seg_cap[dtile];
+    const int64_t start = seg_start[dtile];
+    const int slot = reserve_slot(dtile, cnt_next);
+    int64_t dst = start + slot;
+    if (slot >= cap) {  // segment full: overflow list, and a larger segment next re-layout
+      dst = ovf_base + atomicAdd(ovf_next, 1);
+      grow[dtile] = 1;
+    }
+    typename RecWord<R>::V* d = rec_at<R>(B, dst);
+#pragma unroll
+    for (int k = 0; k < RecWord<R>::N; k++) d[k] = ms.w[k * MCAP + lane];
+    if (B.id) B.id[dst] = ms.id[lane];
+  }
+}
+
 template <int D, int TX, int TY, int TZ, int NT, typename R>
 __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
     TileArgs ta, StepConst<R> sc, const R* __restrict__ ebf, RecPtr A, RecPtr B,
@@ -383,7 +430,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
   R* Fl = (R*)lds;
   double* Jl = (double*)(lds + S::F_BYTES);
   char* Q = lds + S::F_BYTES + S::J_BYTES;
-  int* offs = (int*)(Q + S::Q_BYTES);  // NCL + 1
+  int* offs = (int*)(Q + S::Q_BYTES + S::M_BYTES);  // NCL + 1
   int* fillc = offs + NCL + 1;  // NCL
   int* misc = fillc + NCL;      // [0] chunk slot base, [1] stayers, [2] max per cell
   int* wsum = misc + 4;          // per-wave totals of the cell-count scan
@@ -439,6 +486,8 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
 
   const WaveQueue<R, QCAP> qu(Q + (me >> 6) * S::QW_BYTES);
   int qcount = 0;  // wave-uniform: entries 0 .. qcount-1 are waiting
+  const MoverStage<R, S::MCAP> mst(Q + S::Q_BYTES + (me >> 6) * S::MW_BYTES);
+  int mcount = 0;  // wave-uniform: staged movers
   // LDS current patch node N-1 of local cell lc (the patch starts at T0-2)
   const auto patch_node = [&](int lc) -> double* {
     const int li = lc % TX, lj = (lc / TX) % TY, lk = D == 3 ? lc / (TX * TY) : 0;
@@ -608,23 +657,35 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
       const int dtile = (q.N[0] - 3) / TX +
                         ta.nt0 * ((q.N[1] - 3) / TY + ta.nt1 * (D == 3 ? (q.N[2] - 3) / TZ : 0));
       const bool stay = keep && dtile == t;
-      if (keep) {
-        int64_t dst;
-        int slot, cap;
-        if (stay) {
-          slot = chunk_base + reserve_one(&misc[1]);
-          cap = cap_t;
-          dst = base + slot;
-        } else {
-          // the segment's place and size load while the slot atomic is out
-          cap = seg_cap[dtile];
-          const int64_t start = seg_start[dtile];
-          slot = reserve_slot(dtile, cnt_next);
-          dst = start + slot;
-        }
-        if (slot >= cap) {  // segment full: overflow list, and a larger segment next re-layout
+      const bool mover = keep && !stay;
+      const unsigned long long mm = __ballot(mover);
+      const int mnew = __popcll(mm);
+      if (mcount + mnew > S::MCAP) {  // no room for this round's movers: write out first
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        flush_movers<R, S::MCAP>(mst, mcount, B, seg_start, seg_cap, cnt_next, ovf_next, grow,
+                                 sc.ovf_base);
+        mcount = 0;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+      if (mover) {
+        const int e = mcount + __popcll(mm & lt_mask);
+        typename RecWord<R>::V wd[RecWord<R>::N];
+        rec_words<D, R>(pack_cell<D>(q.N, ta.bx, ta.by), q.s, q.p, q.w, wd);
+#pragma unroll
+        for (int k = 0; k < RecWord<R>::N; k++) mst.w[k * S::MCAP + e] = wd[k];
+        mst.tile[e] = dtile;
+        mst.id[e] = q.id;
+      }
+      mcount += mnew;
+      if (stay) {
+        const int slot = chunk_base + reserve_one(&misc[1]);
+        int64_t dst = base + slot;
+        if (slot >= cap_t) {  // segment full: overflow list, and a larger segment next re-layout
           dst = sc.ovf_base + atomicAdd(ovf_next, 1);
-          grow[dtile] = 1;
+          grow[t] = 1;
         }
         store_rec<D, R>(B, dst, q, ta.bx, ta.by);
       }
@@ -638,6 +699,13 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
       }
     }
     if (c0 + kChunk < cnt) load_cells(c0 + kChunk);
+  }
+  if (mcount > 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    flush_movers<R, S::MCAP>(mst, mcount, B, seg_start, seg_cap, cnt_next, ovf_next, grow,
+                             sc.ovf_base);
   }
   if (qcount > 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
