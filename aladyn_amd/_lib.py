@@ -62,6 +62,7 @@ SIGNATURES = {
     "aladyn_lp_window_xshift": [_P, C.c_int32, C.c_int32, C.c_double, C.POINTER(C.c_int32)],
     "aladyn_inject_species": [_P, C.c_int32, _D, _I64, _I64],
     "aladyn_box": [_P, _D],
+    "aladyn_set_box_origin": [_P, C.c_double],
     "aladyn_set_injection": [_P, C.c_int32, _D, _I64, _D, C.c_int32, _D, C.c_int32, _D, _D, _I64,
                              C.c_double, C.c_double],
     "aladyn_injection_state": [_P, C.c_int32, C.POINTER(C.c_int64)],

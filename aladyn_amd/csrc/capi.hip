@@ -467,6 +467,18 @@ extern "C" int aladyn_inject_species(aladyn_ctx* c, int32_t ic, const double* pa
   return ALADYN_OK;
 }
 
+extern "C" int aladyn_set_box_origin(aladyn_ctx* c, double xmin) {
+  if (!c) return ALADYN_EINVAL;
+  if (!std::isfinite(xmin)) return fail(c, ALADYN_EINVAL, "set_box_origin: xmin not finite");
+  for (int ic = 0; ic < ALADYN_MAX_SPECIES; ic++)
+    if (c->ts[ic].loaded)
+      return fail(c, ALADYN_ESTATE, "set_box_origin: call it before put_species");
+  DevGrid& g = c->g;
+  g.xmn = xmin;
+  g.xmx = xmin + g.dx * (double)g.nx;
+  return ALADYN_OK;
+}
+
 extern "C" int aladyn_box(aladyn_ctx* c, double out[6]) {
   if (!c || !out) return ALADYN_EINVAL;
   const DevGrid& g = c->g;

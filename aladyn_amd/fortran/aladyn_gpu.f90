@@ -47,7 +47,7 @@ module aladyn_gpu
   public :: aladyn_lpf2_evolve, aladyn_pfields_prepare, aladyn_curr_clean
   public :: aladyn_particle_stage, aladyn_curr_mpi_collect, aladyn_advance_lpf_fields
   public :: aladyn_lp_window_xshift, aladyn_inject_species, aladyn_box
-  public :: aladyn_set_injection, aladyn_injection_state
+  public :: aladyn_set_injection, aladyn_injection_state, aladyn_set_box_origin
   public :: aladyn_energy_momenta, aladyn_field_energy, aladyn_grid_charge
   public :: aladyn_continuity_residual, aladyn_set_diagnostics
   public :: aladyn_synchronize, aladyn_kernel_timing
@@ -183,6 +183,13 @@ module aladyn_gpu
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: ic
       integer(c_int64_t), intent(out) :: nptx
+    end function
+
+    integer(c_int) function aladyn_set_box_origin(ctx, xmin) &
+        bind(C, name="aladyn_set_box_origin")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: xmin
     end function
 
     integer(c_int) function aladyn_box(ctx, out) bind(C, name="aladyn_box")

@@ -207,6 +207,11 @@ class LPStep:
         self._ck(self.lib.aladyn_injection_state(self._h, ic, C.byref(n)))
         return int(n.value)
 
+    def set_box_origin(self, xmin: float):
+        """Places the box at (xmin, xmin + nx*dx] before any put_species
+        (restart of a moved window, pic_dump.f90:520-531)."""
+        self._ck(self.lib.aladyn_set_box_origin(self._h, float(xmin)))
+
     def box(self) -> np.ndarray:
         """The current particle box (xmin, xmax, ymin, ymax, zmin, zmax)."""
         out = np.zeros(6)

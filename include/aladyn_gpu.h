@@ -165,6 +165,12 @@ int aladyn_set_injection(aladyn_ctx *ctx, int32_t ic, const double *xpt, int64_t
                          double targ_in, double targ_end);
 /* The columns of species ic's target table consumed so far (nptx(ic)). */
 int aladyn_injection_state(aladyn_ctx *ctx, int32_t ic, int64_t *nptx);
+/* Places the box at x in (xmin, xmin + nx*dx], as restart does with the
+ * dumped x(1) (pic_dump.f90:520-531) after the window has moved; the saved
+ * window state stays zero, as the reference's does on restart.  Only before
+ * any put_species (ALADYN_ESTATE otherwise): stored particles are relative
+ * to the box. */
+int aladyn_set_box_origin(aladyn_ctx *ctx, double xmin);
 /* The current particle box: out = {xmin, xmax, ymin, ymax, zmin, zmax}
  * (loc_*grid%gmin/gmax; x moves with the window). */
 int aladyn_box(aladyn_ctx *ctx, double out[6]);
