@@ -203,10 +203,10 @@ __device__ inline R contract(const FA& F, int c, const R* wx, const R* wy, const
 // quadratic weights on integer-node axes, linear on half-node axes
 // (Ex: half x; Ey: half y; Ez: half z; Bx: half y,z; By: half x,z;
 // Bz: half x,y).  F(c, i1, j1, k1) returns component c at node
-// (N-1+i1, N-1+j1, N-1+k1).
+// (N-1+i1, N-1+j1, N-1+k1); a[d] receives the quadratic weights of s[d].
 template <int D, typename R, class FA>
-__device__ inline void gather(const R s[3], const FA& F, R ef[6]) {
-  R a[3][3], ah[3][2];
+__device__ inline void gather(const R s[3], const FA& F, R ef[6], R a[3][3]) {
+  R ah[3][2];
 #pragma unroll
   for (int d = 0; d < D; d++) {
     qweights<R>(s[d], a[d]);
@@ -225,6 +225,13 @@ __device__ inline void gather(const R s[3], const FA& F, R ef[6]) {
     ef[4] = contract<2, 3, 2, R>(F, 4, ah[0], a[1], ah[2]);
     ef[5] = contract<2, 2, 3, R>(F, 5, ah[0], ah[1], a[2]);
   }
+}
+
+// The same without the quadratic weights (a[d] = the shape of s[d]).
+template <int D, typename R, class FA>
+__device__ inline void gather(const R s[3], const FA& F, R ef[6]) {
+  R a[3][3];
+  gather<D, R>(s, F, ef, a);
 }
 
 // Charge multiply + [init_lpf_momenta] + lpf_momenta_and_positions

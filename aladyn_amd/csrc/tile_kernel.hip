@@ -98,7 +98,7 @@ __device__ inline void deposit_general(double* jb, const R s0[3], const R s1[3],
 
 // Window data of one axis for a particle whose nearest node moved by sg
 // (-1, 0 or +1) from N0, in the frame of the old node block N0-1 .. N0+1:
-// a0 = old quadratic weights, w1 = the new weights that fall inside the
+// a0 = old quadratic weights (given), w1 = the new weights that fall inside the
 // block, fl = the two in-block Esirkepov fluxes (N0-1 -> N0, N0 -> N0+1),
 // and for sg != 0 the new weight wo and the flux fo just outside the block
 // (at N0+2 / N0+1 -> N0+2 for sg = +1, at N0-2 / N0-2 -> N0-1 for sg = -1).
@@ -107,11 +107,10 @@ __device__ inline void deposit_general(double* jb, const R s0[3], const R s1[3],
 // closed form, (s1-s0)(1-s0-s1)/2 and (s1-s0)(1+s0+s1)/2 times wq, which
 // avoids the cancellation of W0 - W1 for short moves.
 template <typename R>
-__device__ inline void axis_window(R s0, R s1, int sg, R wq, R a0[3], R w1[3], R fl[2], R& wo,
-                                   R& fo) {
+__device__ inline void axis_window(R s0, R s1, int sg, R wq, const R a0[3], R w1[3], R fl[2],
+                                   R& wo, R& fo) {
   const R zero = R(0), half = R(0.5), one = R(1);
   R a1[3];
-  qweights<R>(s0, a0);
   qweights<R>(s1, a1);
   const bool up = sg > 0, dn = sg < 0;
   w1[0] = up ? zero : (dn ? a1[1] : a1[0]);
@@ -476,10 +475,14 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
                     (int64_t)ta.n1p * ((j - 1) + (int64_t)ta.n2p * (k - 1))];
     }
     load_cells(0);
+    // the patch holds alp*charge*F: the push's charge multiply and alp
+    // factor (field_charge_multiply, boris_push.f90:75-168) are done here
+    // once per node instead of once per particle
+    const R fs = sc.alp * sc.qch;
 #pragma unroll
     for (int u = 0; u < FPT; u++) {
       const int q = threadIdx.x + u * NT;
-      if (q < NF * GVOL) Fl[q] = fv[u];
+      if (q < NF * GVOL) Fl[q] = fs * fv[u];
     }
   }
   for (int q = threadIdx.x; q < D * PVOL; q += NT) Jl[q] = 0.0;
@@ -582,6 +585,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
       if (qi + 1 < q1) load_raw<D, R>(A, cb + perm[qi + 1], nxt);
       bool keep = false, cross = false;
       R so[3] = {R(0), R(0), R(0)}, wq = R(0);
+      R a0[3][3] = {{R(1), R(0), R(0)}, {R(1), R(0), R(0)}, {R(1), R(0), R(0)}};
       int lc = 0, N0[3] = {0, 0, 0};
       if (qi < q1) {
         decode_cell<D, R>(q, ta.bx, ta.by);
@@ -593,13 +597,14 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
           return fb[c * GVOL + i1 + GX * (j1 + GY * k1)];
         };
         R ef[6];
-        gather<D, R>(q.s, F, ef);
+        gather<D, R>(q.s, F, ef, a0);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           so[d] = q.s[d];
           N0[d] = q.N[d];
         }
-        const R dtg = push<D, R>(q.p, ef, sc.qch, sc.alp, sc.dt, sc.initial);
+        // the fields are already alp*charge*F
+        const R dtg = push<D, R>(q.p, ef, R(1), R(1), sc.dt, sc.initial);
         keep = move_and_test<D, R>(q, dtg, sc.inv, sc.nn);
         wq = sc.qch * q.w;
         int sg[3] = {0, 0, 0}, nmoved = 0;
@@ -617,7 +622,6 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
             acc.zero();
             acc_lc = lc;
           }
-          R a0[3][3] = {{R(1), R(0), R(0)}, {R(1), R(0), R(0)}, {R(1), R(0), R(0)}};
           R w1[3][3] = {{R(1), R(0), R(0)}, {R(1), R(0), R(0)}, {R(1), R(0), R(0)}};
           R fl[3][2] = {}, wo[3] = {}, fo[3] = {};
 #pragma unroll
