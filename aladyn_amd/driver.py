@@ -1,0 +1,46 @@
+"""The LP time loop around the step (ALaDyn.F90:66-107, lp_run in
+pic_evolve_in_time.f90:113-140, timing in run_data_info.f90:38-72), for
+callers that drive the library from Python instead of the Fortran code.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+
+def dt_loc(dt: float, tmax: float) -> float:
+    """dt_loc = tmax/int(tmax/dt) when tmax > 0, else dt
+    (start_all.F90:219-226)."""
+    if tmax > 0.0:
+        return tmax / float(int(tmax / dt))
+    return dt
+
+
+def with_dt_loc(cfg, tmax: float):
+    """The configuration with its time step replaced by dt_loc."""
+    return dataclasses.replace(cfg, dt=dt_loc(cfg.dt, tmax))
+
+
+def lp_cycle(gpu, *, tmax: float, tstart: float = 0.0, w_speed: float = 0.0,
+             wi_time: float = 0.0, wf_time: float = math.inf, w_sh: int = 1, on_step=None):
+    """Lp_cycle (ALaDyn.F90:89-107) on an LPStep whose configuration carries
+    dt_loc: while tnow < tmax, lp_run(tnow, iter) -- lpf2_evolve(iter), then
+    lp_window_xshift(w_sh, iter) when w_speed > 0, wi_time <= tnow < wf_time
+    and mod(iter, w_sh) == 0 -- and timing's tnow += dt_loc, iter += 1.
+    initial_time holds for the first step when tstart < dt_loc.  The
+    reference trims the last step to tmax - tnow, a change of dt_loc by
+    rounding only; the device keeps dt_loc.  on_step(iter, tnow) runs after
+    each step (the place of Data_out).  Returns (iter, tnow)."""
+    dt = gpu.cfg.dt
+    tnow, it = tstart, 0
+    initial = tnow < dt
+    while tnow < tmax:
+        gpu.lpf2_evolve(it, initial_time=initial)
+        if w_speed > 0.0 and wi_time <= tnow < wf_time and it % w_sh == 0:
+            gpu.lp_window_xshift(w_sh, it, w_speed)
+        tnow += dt
+        it += 1
+        initial = False
+        if on_step is not None:
+            on_step(it, tnow)
+    return it, tnow
