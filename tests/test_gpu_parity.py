@@ -190,3 +190,51 @@ def test_error_paths():
             gpu.lpf2_evolve(0)
         with pytest.raises(AladynError, match="EINVAL"):
             gpu.put_species(7, part)
+
+
+@pytest.mark.parametrize("precision", [PREC_F64, PREC_F32])
+def test_error_paths_window_injection_restart(precision):
+    """The documented codes of the window, injection and restart calls."""
+    from aladyn_amd import AladynError
+    from aladyn_amd.inputs import pack_weight, plasma_table
+
+    cfg, part, ebf = small_2d()
+    table = plasma_table(cfg, (2, 2), x_start_cell=cfg.nx, x_end_cell=cfg.nx + 10)
+    with LPStep(cfg, precision) as gpu:
+        # a table before any put: the move reports the missing species
+        gpu.put_fields(ebf)
+        gpu.set_injection(0, table, nptx=0, targ_in=cfg.xmn, targ_end=100.0)
+        assert gpu.lp_window_xshift(4, 0, 1.0) == 0
+        with pytest.raises(AladynError, match="ESTATE"):  # moves ~6 cells, then injects
+            gpu.lp_window_xshift(10, 10, 1.0)
+    with LPStep(cfg, precision) as gpu:
+        gpu.set_box_origin(10 * cfg.dx)                  # allowed before put
+        assert gpu.box()[0] == pytest.approx(10 * cfg.dx)
+        with pytest.raises(AladynError, match="ERANGE"):  # plasma from 4 cells: outside
+            gpu.put_species(0, part)
+        gpu.set_box_origin(0.0)
+        gpu.put_species(0, part)
+        gpu.put_fields(ebf)
+        with pytest.raises(AladynError, match="ESTATE"):  # after put
+            gpu.set_box_origin(cfg.dx)
+        bad = dict(table)
+        bad["ypt"] = np.array([cfg.ymx + 1.0])
+        bad["wghyz"] = np.ones((1, 1))
+        with pytest.raises(AladynError, match="ERANGE"):
+            gpu.set_injection(0, bad)
+        rev = dict(table)
+        rev["xpt"] = table["xpt"][::-1].copy()
+        with pytest.raises(AladynError, match="EINVAL"):
+            gpu.set_injection(0, rev)
+        with pytest.raises(AladynError, match="EINVAL"):
+            gpu.set_injection(0, table, nptx=len(table["xpt"]) + 1)
+        # a shift of the whole box is the reference's ier = 2
+        assert gpu.lp_window_xshift(4, 0, 1.0) == 0
+        with pytest.raises(AladynError, match="ERANGE"):
+            gpu.lp_window_xshift(100000, 1, 1.0)
+        # injected particles must carry the species' charge
+        other = part[:4].copy(order="F")
+        D = cfg.curr_ndim
+        other[:, 2 * D] = pack_weight(1, 0, np.float32([0.5] * 4))
+        with pytest.raises(AladynError, match="EINVAL"):
+            gpu.inject_species(0, other)
