@@ -185,8 +185,9 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the step would shard by spatial domain; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the library runs one sub-domain per GPU "
+              "and has no multi-GPU exchange (the step would shard by spatial domain, as "
+              "the reference's MPI decomposition does)", file=sys.stderr)
         sys.exit(2)
 
     from aladyn_amd import PREC_F32, PREC_F64
