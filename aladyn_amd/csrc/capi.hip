@@ -94,6 +94,7 @@ struct aladyn_ctx {
   double* rho1 = nullptr;
   void* jraw = nullptr;
   bool have_resid = false;
+  bool jraw_is_raw = false;  // jraw already holds the stage's raw current
   // lp_window_xshift saved state (window.f90:278-280)
   double win_xlapse = 0.0;
   int win_wi1 = 0;
@@ -567,8 +568,33 @@ extern "C" int aladyn_pfields_prepare(aladyn_ctx* c) {
   return ALADYN_OK;
 }
 
+// The raw current of the last stage is jc - jraw (jraw holds jc before it).
+template <typename T>
+__global__ void k_sub(T* a, const T* b, int64_t n) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    a[t] = b[t] - a[t];
+}
+
+// Turns jraw into the last stage's raw current (jc - jraw) before anything
+// changes jc (clean, fold, normalisation), so the continuity residual of a
+// stage stays available after the rest of lpf2_evolve.
+static int freeze_resid(aladyn_ctx* c) {
+  if (!c->have_resid || c->jraw_is_raw) return 0;
+  const int64_t n = c->g.ncell * c->g.curr_ndim;
+  const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+  if (f64(c))
+    k_sub<double><<<nb, 256, 0, c->stream>>>((double*)c->jraw, c->jc64, n);
+  else
+    k_sub<float><<<nb, 256, 0, c->stream>>>((float*)c->jraw, c->jc32, n);
+  HIPCK(c, hipGetLastError());
+  c->jraw_is_raw = true;
+  return 0;
+}
+
 extern "C" int aladyn_curr_clean(aladyn_ctx* c) {
   if (!c) return ALADYN_EINVAL;
+  if (int r = freeze_resid(c)) return r;
   const int64_t n = c->g.ncell * c->g.curr_ndim;
   if (f64(c))
     fill<double>(c->jc64, n, 0.0, c->stream);
@@ -610,6 +636,8 @@ extern "C" int aladyn_particle_stage(aladyn_ctx* c, int32_t ic, int32_t initial_
     HIPCK(c, hipMemcpyAsync(c->jraw, f64(c) ? (void*)c->jc64 : (void*)c->jc32,
                             (size_t)g.ncell * g.curr_ndim * esz, hipMemcpyDeviceToDevice,
                             c->stream));
+    c->have_resid = false;
+    c->jraw_is_raw = false;
   }
   std::string msg;
   const int init = initial_time ? 1 : 0;
@@ -633,6 +661,7 @@ extern "C" int aladyn_particle_stage(aladyn_ctx* c, int32_t ic, int32_t initial_
 
 extern "C" int aladyn_curr_mpi_collect(aladyn_ctx* c) {
   if (!c) return ALADYN_EINVAL;
+  if (int r = freeze_resid(c)) return r;
   if (f64(c))
     curr_collect<double>(c->g, c->jc64, c->stream);
   else
@@ -714,28 +743,16 @@ extern "C" int aladyn_set_diagnostics(aladyn_ctx* c, int32_t enable) {
   return ALADYN_OK;
 }
 
-// The raw current of the last stage is jc - jraw (jraw holds jc before it).
-template <typename T>
-__global__ void k_sub(T* a, const T* b, int64_t n) {
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * blockDim.x)
-    a[t] = b[t] - a[t];
-}
-
 extern "C" int aladyn_continuity_residual(aladyn_ctx* c, double out[2]) {
   if (!c || !out) return fail(c, ALADYN_EINVAL, "continuity_residual: NULL argument");
   if (!c->have_resid)
     return fail(c, ALADYN_ESTATE, "continuity_residual needs set_diagnostics(1) then a particle_stage");
-  const int64_t n = c->g.ncell * c->g.curr_ndim;
-  const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+  if (int r = freeze_resid(c)) return r;
   HIPCK(c, hipMemsetAsync(c->d_red, 0, 2 * sizeof(double), c->stream));
-  if (f64(c)) {
-    k_sub<double><<<nb, 256, 0, c->stream>>>((double*)c->jraw, c->jc64, n);
+  if (f64(c))
     continuity<double>(c->g, c->rho0, c->rho1, (const double*)c->jraw, c->d_red, c->stream);
-  } else {
-    k_sub<float><<<nb, 256, 0, c->stream>>>((float*)c->jraw, c->jc32, n);
+  else
     continuity<float>(c->g, c->rho0, c->rho1, (const float*)c->jraw, c->d_red, c->stream);
-  }
   HIPCK(c, hipGetLastError());
   HIPCK(c, hipMemcpyAsync(c->h_red, c->d_red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCK(c, hipStreamSynchronize(c->stream));

@@ -190,7 +190,10 @@ int aladyn_grid_charge(aladyn_ctx *ctx, int32_t ic, double *rho);
  * interior, with rho from the particle positions before and after the stage
  * and J the raw (pre-djc) current of that stage.  out[0] = max|r|,
  * out[1] = max|rho^{n+1} - rho^n|.  Requires aladyn_set_diagnostics(ctx, 1)
- * before the stage (it keeps rho^n and the raw current). */
+ * before the stage (it keeps rho^n and the raw current).  The raw current is
+ * kept before curr_clean or curr_mpi_collect change jc, so the residual of
+ * the last stage of an lpf2_evolve step can be read after the step; the next
+ * particle_stage, set_diagnostics or grid_charge call replaces it. */
 int aladyn_continuity_residual(aladyn_ctx *ctx, double out[2]);
 int aladyn_set_diagnostics(aladyn_ctx *ctx, int32_t enable);
 

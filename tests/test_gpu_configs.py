@@ -160,3 +160,39 @@ def test_scan_like_2d_runs(nx, ny, ppc, nnc, a0, precision, tol):
             gpu.lpf2_evolve(it, initial_time=(it == 0))
         pg, eg, jg = gpu.get_species(0), gpu.get_fields(), gpu.get_current()
     _compare(cfg, p, pg, e, eg, jc if precision == PREC_F64 else None, jg, tol)
+
+
+@pytest.mark.parametrize("precision,tol", [(PREC_F64, 1e-12), (PREC_F32, 1e-5)])
+def test_lwfa_3d_full_size_invariants(precision, tol):
+    """The benchmark's 3-D LWFA grid at its full size (256x128x128, 2x2x2
+    jittered ppc, a0 = 3 pulse), checked by what holds at any size: the
+    discrete continuity residual of each checked step (set_grid_charge,
+    grid_part_util.f90:240-335, against esirkepov_3d_curr's current), and the
+    grid charge summing to the particles' charge (the quadratic weights are a
+    partition of unity).  The plasma stops 4 cells short of every face so
+    that no particle leaves in the 4 steps: a particle removed before its
+    deposit (cell_part_dist, as the reference does it) changes rho without a
+    current, and continuity does not hold at the faces."""
+    from aladyn_amd.inputs import lwfa_3d_config
+
+    cfg = lwfa_3d_config()
+    part = uniform_plasma(cfg, (2, 2, 2), x_start_cell=4, x_end_cell=cfg.nx - 4,
+                          y_cells=(4, cfg.ny - 4), z_cells=(4, cfg.nz - 4), seed=0)
+    ebf = gaussian_laser(cfg, a0=3.0, w0=10.0, tau_fwhm=6.0, xc=cfg.nx * cfg.dx / 4)
+    print(f"3-D LWFA full size np={part.shape[0]} precision={precision}", flush=True)
+    with LPStep(cfg, precision) as gpu:
+        gpu.put_species(0, part)
+        gpu.put_fields(ebf)
+        for it in range(4):
+            check = it in (0, 3)
+            gpu.set_diagnostics(check)
+            gpu.lpf2_evolve(it, initial_time=(it == 0))
+            if check:
+                r, dmax = gpu.continuity_residual()
+                assert dmax > 0
+                assert r / dmax <= tol, (it, r, dmax)
+        ek, _ = gpu.energy_momenta(0)
+        rho = gpu.grid_charge(0)
+        n = gpu.species_count(0)
+    assert n == part.shape[0]
+    assert abs(rho.sum() - ek[5]) <= (1e-12 if precision == PREC_F64 else 1e-6) * abs(ek[5])
