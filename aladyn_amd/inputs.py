@@ -222,3 +222,45 @@ def lwfa_3d_case(nx=256, ny=128, nz=128, seed=0, ppc=(2, 2, 2)):
     part, ebf = lwfa_case(cfg, ppc=ppc, a0=3.0, w0=10.0, tau_fwhm=6.0, xc=nx * cfg.dx / 4,
                           x_start_cell=0.0, seed=seed)
     return cfg, part, ebf
+
+
+# ------------------------------------------------- batched 2-D parameter scan
+def scan_2d_params(seed: int = 7, nruns: int = 64) -> list[dict]:
+    """Parameters of the batched 2-D scan: nruns independent LP runs.
+
+    Drawn from numpy's PCG64 generator (np.random.Generator(PCG64(seed)),
+    the generator's published algorithm, so the table is the same on every
+    machine with numpy >= 1.17), one run after another, in this order:
+    nx from {128, 160, ..., 1024}, ny from {64, 96, ..., 512}, n/nc
+    log-uniform on [1e-3, 2e-2], a0 uniform on [1, 4], particles per cell
+    per axis from {1, 2, 3, 4} (x then y).  Run i is seeded by i.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    runs = []
+    for i in range(nruns):
+        nx = 128 + 32 * int(rng.integers(0, 29))
+        ny = 64 + 32 * int(rng.integers(0, 15))
+        nnc = float(math.exp(rng.uniform(math.log(1e-3), math.log(2e-2))))
+        a0 = float(rng.uniform(1.0, 4.0))
+        ppc = [int(v) for v in rng.integers(1, 5, size=2)]
+        runs.append({"run": i, "nx": nx, "ny": ny, "n_over_nc": nnc, "a0": a0, "ppc": ppc,
+                     "seed": i})
+    return runs
+
+
+def scan_2d_case(run: dict):
+    """One run of the batched 2-D scan: (cfg, part, ebf).
+
+    The cells are those of the 2-D LWFA workload (dx = dy = 0.05, cfl 0.9).
+    The pulse (FWHM tau = min(6, nx*dx/8) periods, waist ny*dy/4) is centred
+    at 1.5 tau, fully inside the box; the plasma starts 1.5 tau ahead of the
+    pulse centre, at x = 3 tau, with jittered particles seeded by the run.
+    """
+    nx, ny = run["nx"], run["ny"]
+    cfg = LPConfig(ndim=2, nx=nx, ny=ny, k0=20.0, yx_rat=1.0, cfl=0.9,
+                   n_over_nc=run["n_over_nc"])
+    tau = min(6.0, nx * cfg.dx / 8.0)
+    part = uniform_plasma(cfg, tuple(run["ppc"]), x_start_cell=3.0 * tau / cfg.dx,
+                          seed=run["seed"])
+    ebf = gaussian_laser(cfg, a0=run["a0"], w0=ny * cfg.dy / 4.0, tau_fwhm=tau, xc=1.5 * tau)
+    return cfg, part, ebf
