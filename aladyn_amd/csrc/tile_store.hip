@@ -588,20 +588,30 @@ int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const do
   ts->part_ind = pind;
   ts->cur = 0;
   ts->next_id = np;
+  // The host columns go through a device staging area of at most kPutChunk
+  // particles, converted chunk by chunk into buffer 1 (stream order keeps a
+  // chunk's copy behind the previous chunk's conversion), so a put needs
+  // little device memory beyond the store itself.
+  constexpr int64_t kPutChunk = 1 << 22;
   double* dpart = nullptr;
   const int ncol = 2 * D + 1;
-  if (np > 0) {
-    TS_CK(hipMalloc(&dpart, (size_t)np * ncol * sizeof(double)));
-    for (int c = 0; c < ncol; c++)
-      TS_CK(hipMemcpyAsync(dpart + c * np, part + c * ld, (size_t)np * sizeof(double),
-                           hipMemcpyHostToDevice, s));
-  }
+  const int64_t chunk = std::min<int64_t>(np, kPutChunk);
+  if (np > 0) TS_CK(hipMalloc(&dpart, (size_t)chunk * ncol * sizeof(double)));
   fill<int>(ts->grow, tg.ntiles, 0, s);
   const int r = with_dr(tg, D, [&](auto dc, auto rv) {
     constexpr int DD = decltype(dc)::value;
     using R = decltype(rv);
-    if (np > 0)
-      k_convert_in<DD, R><<<grid_for(np), kB, 0, s>>>(g, tg.bx, tg.by, dpart, np, np, ts->buf[1]);
+    for (int64_t c0 = 0; c0 < np; c0 += chunk) {
+      const int64_t m = std::min<int64_t>(chunk, np - c0);
+      for (int c = 0; c < ncol; c++)
+        TS_CK(hipMemcpyAsync(dpart + c * chunk, part + c * ld + c0, (size_t)m * sizeof(double),
+                             hipMemcpyHostToDevice, s));
+      RecPtr out = ts->buf[1];
+      out.r = rec_at<R>(out, c0);
+      if (out.id) out.id += c0;
+      k_convert_in<DD, R><<<grid_for(m), kB, 0, s>>>(g, tg.bx, tg.by, dpart, chunk, m, out, c0);
+      TS_CK(hipGetLastError());
+    }
     return sort_into_cur<DD, R>(ts, tg, np, s, msg);
   });
   TS_CK(hipStreamSynchronize(s));

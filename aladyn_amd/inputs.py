@@ -264,3 +264,86 @@ def scan_2d_case(run: dict):
                           seed=run["seed"])
     ebf = gaussian_laser(cfg, a0=run["a0"], w0=ny * cfg.dy / 4.0, tau_fwhm=tau, xc=1.5 * tau)
     return cfg, part, ebf
+
+
+# ------------------------------------------- plasma of the large 3-D study
+def uniform_plasma_slabs(cfg: LPConfig, ppc, *, seed: int = 0, thermal: float = 0.0,
+                         ramp_cells: float = 0.0, charge: int = -1, jitter: bool = True,
+                         slab_cells: int = 32, workers: int = 8) -> np.ndarray:
+    """The plasma of uniform_plasma over the whole box, made slab by slab in
+    threads for sizes where one numpy pass is too slow (hundreds of millions
+    of particles): the same sub-cell centres, U(-1/2, 1/2) sub-cell jitter,
+    clipping, N(0, thermal) momenta and linear weight ramp.  The random
+    numbers come from one PCG64 stream per slab of slab_cells x-cells,
+    seeded by SeedSequence([seed, slab]), so the particles are not those of
+    uniform_plasma with the same seed.  Particles are x-fastest within a
+    slab.  Returns part(np, 2D+1), column-major.
+    """
+    from concurrent.futures import ThreadPoolExecutor
+
+    D = cfg.curr_ndim
+    ppc = tuple(int(v) for v in ppc)[:D]
+    dims = (cfg.nx, cfg.ny, cfg.nz)[:D]
+    mins = (cfg.xmn, cfg.ymn, cfg.zmn)
+    sizes = (cfg.dx, cfg.dy, cfg.dz)
+    per_x = int(np.prod([dims[d] * ppc[d] for d in range(1, D)]))
+    nslab = (cfg.nx + slab_cells - 1) // slab_cells
+    n = cfg.nx * ppc[0] * per_x
+    part = np.empty((n, 2 * D + 1), dtype=np.float64, order="F")
+    w0 = cfg.n0_ref / float(np.prod(ppc))
+
+    def sub_axis(c0, c1, d):
+        return ((np.arange(c0, c1)[:, None] + (np.arange(ppc[d]) + 0.5)[None, :] / ppc[d])
+                .ravel())
+
+    def one(k):
+        c0, c1 = k * slab_cells, min(cfg.nx, (k + 1) * slab_cells)
+        o0 = c0 * ppc[0] * per_x
+        m = (c1 - c0) * ppc[0] * per_x
+        rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence([seed, k])))
+        shape = (dims[2] * ppc[2], dims[1] * ppc[1], (c1 - c0) * ppc[0]) if D == 3 else \
+            (dims[1] * ppc[1], (c1 - c0) * ppc[0])
+        axes = [sub_axis(c0, c1, 0)] + [sub_axis(0, dims[d], d) for d in range(1, D)]
+        for d in range(D):
+            col = part[o0:o0 + m, d]
+            view = col.reshape(shape)
+            bshape = [1] * D
+            bshape[D - 1 - d] = axes[d].size
+            np.copyto(view, axes[d].reshape(bshape))
+            if jitter:
+                col += (rng.random(m) - 0.5) / ppc[d]
+            np.clip(col, 1e-6, dims[d] - 1e-6, out=col)
+            col *= sizes[d]
+            col += mins[d]
+        for d in range(D):
+            col = part[o0:o0 + m, D + d]
+            if thermal > 0:
+                rng.standard_normal(out=col)
+                col *= thermal
+            else:
+                col[:] = 0.0
+        if ramp_cells > 0:
+            xl = (part[o0:o0 + m, 0] - cfg.xmn) / cfg.dx
+            wgh = (w0 * np.clip(xl / ramp_cells, 0.0, 1.0)).astype(np.float32)
+        else:
+            wgh = np.full(m, w0, dtype=np.float32)
+        words = part[o0:o0 + m, 2 * D].view(np.uint32).reshape(m, 2)
+        words[:, 0] = np.uint32(charge & 0xFFFF)
+        words[:, 1] = wgh.view(np.uint32)
+        return m
+
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        assert sum(ex.map(one, range(nslab))) == n
+    return part
+
+
+def lwfa_3d_large_case(nx=1024, ny=256, nz=256, seed=0, workers=8):
+    """The large FP32 study: 1024x256x256 cells of the 3-D LWFA workload,
+    n/nc = 0.002 with a linear weight up-ramp over the first 128 cells, 2x2x2
+    ppc with Maxwellian momenta N(0, 1e-3), a0 = 4, w0 = 10, FWHM 6 periods,
+    pulse centred at nx*dx/4."""
+    cfg = lwfa_3d_config(nx, ny, nz, n_over_nc=0.002)
+    part = uniform_plasma_slabs(cfg, (2, 2, 2), seed=seed, thermal=1e-3,
+                                ramp_cells=128.0 * nx / 1024, workers=workers)
+    ebf = gaussian_laser(cfg, a0=4.0, w0=10.0, tau_fwhm=6.0, xc=nx * cfg.dx / 4)
+    return cfg, part, ebf

@@ -415,3 +415,27 @@ def test_3d_open_box_is_stable_at_the_configured_cfl():
     assert max(h[2:]) <= e0 and h[-1] <= 1.01 * h[3]
     e0, h = energies[0.9]
     assert h[-1] > 1e6 * e0
+
+
+def test_slab_plasma_matches_uniform_plasma():
+    """uniform_plasma_slabs (threaded, per-slab streams) makes uniform_plasma's
+    particles: without jitter the same set of rows; with jitter each particle
+    within half a sub-cell of a sub-cell centre, momenta N(0, thermal)."""
+    from aladyn_amd.config import LPConfig
+    from aladyn_amd.inputs import uniform_plasma, uniform_plasma_slabs
+
+    cfg = LPConfig(ndim=3, nx=40, ny=12, nz=8, k0=20.0, yx_rat=4.0, zx_rat=4.0, cfl=0.8,
+                   n_over_nc=0.002)
+    a = uniform_plasma_slabs(cfg, (2, 3, 1), jitter=False, ramp_cells=10.0, slab_cells=16,
+                             workers=3)
+    b = uniform_plasma(cfg, (2, 3, 1), jitter=False, ramp_cells=10.0)
+    key = lambda p: p[np.lexsort(p[:, ::-1].T)]
+    assert a.shape == b.shape
+    assert np.array_equal(key(a), key(b))
+    c = uniform_plasma_slabs(cfg, (2, 3, 1), seed=5, thermal=0.01, slab_cells=16, workers=3)
+    for d, (mn, h, pp) in enumerate(((cfg.xmn, cfg.dx, 2), (cfg.ymn, cfg.dy, 3), (cfg.zmn, cfg.dz, 1))):
+        u = (c[:, d] - mn) / h * pp           # sub-cell units; centres at k + 1/2
+        assert np.all(np.abs(u - np.floor(u) - 0.5) <= 0.5 + 1e-9)
+    assert abs(np.std(c[:, 3:6]) / 0.01 - 1.0) < 0.05
+    assert not np.array_equal(c, uniform_plasma_slabs(cfg, (2, 3, 1), seed=6, thermal=0.01,
+                                                      slab_cells=16, workers=3))
