@@ -347,3 +347,136 @@ def lwfa_3d_large_case(nx=1024, ny=256, nz=256, seed=0, workers=8):
                                 ramp_cells=128.0 * nx / 1024, workers=workers)
     ebf = gaussian_laser(cfg, a0=4.0, w0=10.0, tau_fwhm=6.0, xc=nx * cfg.dx / 4)
     return cfg, part, ebf
+
+
+# ------------------------------------------------ PWFA-like bunch in plasma
+def pwfa_3d_config(nx=512, ny=192, nz=192, n0_ref=0.1, cfl=0.8) -> LPConfig:
+    """3-D box in plasma skin depths: dx = 0.1, dy = dz = 0.2 kp^-1 at the
+    density n0_ref (1e18 cm^-3 units; 0.1 = 1e17 cm^-3), lengths in micron
+    as everywhere (kp^2 = ompe*n0_ref, so kp^-1 = 16.8 um at 1e17 cm^-3).
+    Two electron species: the plasma (0) and the bunch (1)."""
+    ncrit = PI / (2.81794033 * 1.0)
+    probe = LPConfig(ndim=3, nx=8, ny=8, nz=8)
+    skin = 1.0 / math.sqrt(probe.ompe * n0_ref)
+    return LPConfig(ndim=3, nx=nx, ny=ny, nz=nz, k0=1.0 / (0.1 * skin), yx_rat=2.0, zx_rat=2.0,
+                    cfl=cfl, n_over_nc=1.0e-3 * n0_ref / ncrit, nsp=2,
+                    lorentz_fact=(1.0, 1.0, 1.0, 1.0))
+
+
+def skin_depth(cfg: LPConfig) -> float:
+    """kp^-1 in micron at the configuration's n0_ref."""
+    return 1.0 / math.sqrt(cfg.ompe * cfg.n0_ref)
+
+
+def gaussian_bunch(cfg: LPConfig, n: int, *, xc: float, sigma, gamma: float, spread: float,
+                   eps_n: float, nb_over_n0: float, seed: int, charge: int = -1,
+                   cut: float = 4.5) -> np.ndarray:
+    """Electron bunch: x, y, z ~ N((xc, 0, 0), sigma) cut at `cut` sigma
+    (redrawn), p_x = gamma*(1 + spread*N(0,1)), p_y, p_z ~ N(0, eps_n/sigma_y,z)
+    (normalised emittance eps_n = sigma_y*sigma_py, micron).  Equal weights
+    that give a peak density nb_over_n0*n0_ref: w = nb/n0 * n0_ref *
+    (2 pi)^(3/2) sx sy sz / (n dx dy dz), as plasma weights n0_ref/ppc give
+    the density n0_ref.  Returns part(n, 7), column-major."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+
+    def cut_normal(size):
+        v = rng.standard_normal(size)
+        bad = np.abs(v) > cut
+        while bad.any():
+            v[bad] = rng.standard_normal(int(bad.sum()))
+            bad = np.abs(v) > cut
+        return v
+
+    sx, sy, sz = sigma
+    part = np.zeros((n, 7), dtype=np.float64, order="F")
+    part[:, 0] = xc + sx * cut_normal(n)
+    part[:, 1] = sy * cut_normal(n)
+    part[:, 2] = sz * cut_normal(n)
+    part[:, 3] = gamma * (1.0 + spread * rng.standard_normal(n))
+    part[:, 4] = (eps_n / sy) * rng.standard_normal(n)
+    part[:, 5] = (eps_n / sz) * rng.standard_normal(n)
+    w = nb_over_n0 * cfg.n0_ref * (2.0 * PI) ** 1.5 * sx * sy * sz / (n * cfg.dx * cfg.dy * cfg.dz)
+    part[:, 6] = pack_weight(charge, 0, np.full(n, w, dtype=np.float32))
+    return part
+
+
+def grid_charge_host(cfg: LPConfig, part: np.ndarray) -> np.ndarray:
+    """rho(n1p, n2p, n3p) = sum charge*wgh*W_x W_y W_z with the quadratic
+    node weights of set_grid_charge (grid_part_util.f90:240-335; qden_*,
+    grid_part_lib.f90:337-358, 388-417), for initial conditions."""
+    D = cfg.curr_ndim
+    c, _, wgh = unpack_weight(part[:, 2 * D])
+    q = c.astype(np.float64) * wgh.astype(np.float64)
+    dims = (cfg.n1p, cfg.n2p, cfg.n3p)
+    idx, wts = [], []
+    for d in range(D):
+        mn, h = (cfg.xmn, cfg.ymn, cfg.zmn)[d], (cfg.dx, cfg.dy, cfg.dz)[d]
+        xx = 3.0 + (part[:, d] - mn) / h
+        ix = np.floor(xx + 0.5).astype(np.int64)
+        s = xx - ix
+        a1 = 0.75 - s * s
+        a2 = 0.5 * (0.25 + s * s + s)
+        a0 = 1.0 - a1 - a2
+        idx.append(ix - 2)  # 0-based index of node ix-1
+        wts.append((a0, a1, a2))
+    rho = np.zeros(int(np.prod(dims)), dtype=np.float64)
+    offs = [(i, j, k) for k in range(3 if D == 3 else 1) for j in range(3) for i in range(3)]
+    for i, j, k in offs:
+        flat = (idx[0] + i) + dims[0] * ((idx[1] + j) + dims[1] * ((idx[2] + k) if D == 3 else 0))
+        w = q * wts[0][i] * wts[1][j] * (wts[2][k] if D == 3 else 1.0)
+        rho += np.bincount(flat, weights=w, minlength=rho.size)
+    return rho.reshape(dims, order="F")
+
+
+def bunch_fields(cfg: LPConfig, bunch: np.ndarray, ebf: np.ndarray | None = None) -> np.ndarray:
+    """Space-charge field of an ultra-relativistic bunch, added to ebf.
+
+    Per x node plane, the discrete transverse Poisson problem
+    D_y^2 phi + D_z^2 phi = -ompe*rho (the Yee grid's 5-point operator,
+    phi = 0 on the outermost nodes) is solved with a type-I sine transform;
+    E_y = -D_y phi, E_z = -D_z phi at their Yee points, so that the discrete
+    div E equals ompe*rho (the Gauss law the step conserves), and
+    B_z = beta*E_y, B_y = -beta*E_z averaged to their half-integer x.
+    E_x, of order 1/gamma^2, is left at zero.  beta from the mean p_x."""
+    from scipy.fft import dstn, idstn
+
+    if cfg.ndim != 3:
+        raise ValueError("bunch_fields is 3-D")
+    rho = grid_charge_host(cfg, bunch)
+    n1, n2, n3 = rho.shape
+    M2, M3 = n2 - 2, n3 - 2
+    lam_y = -(4.0 / cfg.dy ** 2) * np.sin(PI * np.arange(1, M2 + 1) / (2.0 * (M2 + 1))) ** 2
+    lam_z = -(4.0 / cfg.dz ** 2) * np.sin(PI * np.arange(1, M3 + 1) / (2.0 * (M3 + 1))) ** 2
+    lam = lam_y[:, None] + lam_z[None, :]
+    src = -cfg.ompe * rho[:, 1:n2 - 1, 1:n3 - 1]
+    phi = np.zeros_like(rho)
+    phi[:, 1:n2 - 1, 1:n3 - 1] = idstn(dstn(src, type=1, axes=(1, 2)) / lam[None], type=1,
+                                       axes=(1, 2))
+    ey = np.zeros_like(rho)
+    ez = np.zeros_like(rho)
+    ey[:, :-1, :] = -(phi[:, 1:, :] - phi[:, :-1, :]) / cfg.dy
+    ez[:, :, :-1] = -(phi[:, :, 1:] - phi[:, :, :-1]) / cfg.dz
+    pxm = float(np.mean(bunch[:, 3]))
+    beta = pxm / math.sqrt(1.0 + pxm * pxm)
+    out = np.zeros(cfg.field_shape, dtype=np.float64, order="F") if ebf is None else ebf
+    out[:, :, :, 1] += ey
+    out[:, :, :, 2] += ez
+    out[:-1, :, :, 5] += beta * 0.5 * (ey[:-1] + ey[1:])
+    out[:-1, :, :, 4] -= beta * 0.5 * (ez[:-1] + ez[1:])
+    return out
+
+
+def pwfa_3d_case(nx=512, ny=192, nz=192, *, n_bunch=2_000_000, sigma_kp=(2.0, 1.5, 1.5),
+                 xc_kp=12.0, gamma=2000.0, spread=0.01, eps_n=2.0, nb_over_n0=1.0,
+                 ppc=(1, 2, 2), seed=3, workers=8):
+    """Electron bunch (sigma in kp^-1, gamma 2000, 1 % spread, eps_n 2 mm mrad
+    = 2 micron) at xc in a uniform plasma of 1e17 cm^-3 filling the box,
+    4 particles per cell, with the bunch's space-charge field.
+    Returns (cfg, plasma, bunch, ebf)."""
+    cfg = pwfa_3d_config(nx, ny, nz)
+    kp = skin_depth(cfg)
+    plasma = uniform_plasma_slabs(cfg, ppc, seed=seed + 1, workers=workers)
+    bunch = gaussian_bunch(cfg, n_bunch, xc=xc_kp * kp, sigma=tuple(s * kp for s in sigma_kp),
+                           gamma=gamma, spread=spread, eps_n=eps_n, nb_over_n0=nb_over_n0,
+                           seed=seed)
+    return cfg, plasma, bunch, bunch_fields(cfg, bunch)
