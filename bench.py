@@ -17,8 +17,9 @@ ratio.  Inputs are resident in HBM before
 timing starts.  `roofline` refers to the dominant kernel (the FP32 particle
 kernel), its duration measured with HIP events on the context stream over the
 timed steps; `cpu_baseline` times the reference's own Fortran gather + push +
-deposit (oracle/_ref, built from the reference sources) on one host core over
-a bounded particle sample.
+deposit (oracle/_ref, built from the reference sources) over a bounded
+particle sample, one single-rank process per host core (at most 16) at once,
+with the one-core rate beside it.
 
 `roofline.traffic` is the particle kernel's measured HBM bytes per launch
 (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a plain
@@ -128,20 +129,51 @@ def run_path(cfg, part, ebf, precision, steps, warmup, table=None):
                 kinetic=float(ek[6]), field_energy=float(fe.sum()))
 
 
+_CPU = {}  # the sample, shared with forked workers (copy-on-write)
+
+
+def _cpu_timed(q, k):
+    """Seconds for k reference steps (gather+push+deposit) of particles q."""
+    import numpy as np
+
+    t0 = time.perf_counter()
+    _CPU["ref"].gather_push_deposit(np.asfortranarray(q), _CPU["ebf"], k)
+    return time.perf_counter() - t0
+
+
+def _cpu_worker(w):
+    """Worker w of the many-core leg: its own share of the sample, its own
+    copy of the reference harness (one single-rank process per core, as the
+    reference would run without MPI)."""
+    from oracle.pyoracle import RefHarness
+
+    _CPU["ref"] = RefHarness(_CPU["cfg"])
+    q = _CPU["shares"][w]
+    t1 = _cpu_timed(q, 1)
+    tk = _cpu_timed(q, _CPU["steps"])
+    return q.shape[0], t1, tk
+
+
 def cpu_baseline(cfg, part, ebf, budget_s=15.0):
-    """Reference Fortran gather+push+deposit on one core over a sample.
+    """Reference Fortran gather+push+deposit (FP64) on the host's cores.
 
     The reference routines have no box test, so the sample is drawn from
     particles at least 24 cells from every face and runs 20 steps (under one
     cell per step on each axis, since dt < dx): every index stays inside the
-    grid.  The sample size fills about budget_s seconds."""
+    grid.  Each call copies ebf and clears jc first, so a rate is taken from
+    the difference of a 20-step and a 1-step call.  First one core over a
+    sample of about budget_s seconds; then one process per core (the box's
+    share: at most 16), each over its own share of the inner particles, run
+    at the same time, their rates summed.  Must run before the process
+    touches the GPU: the workers are forked."""
+    import multiprocessing as mp
+
     import numpy as np
 
     from oracle.pyoracle import REF_LIB, RefHarness
 
     if not REF_LIB.exists():
         return None
-    ref = RefHarness(cfg)
     steps, margin = 20, 24
     lo = np.array([cfg.xmn, cfg.ymn, cfg.zmn])[: cfg.curr_ndim]
     hi = np.array([cfg.xmx, cfg.ymx, cfg.zmx])[: cfg.curr_ndim]
@@ -149,23 +181,29 @@ def cpu_baseline(cfg, part, ebf, budget_s=15.0):
     x = part[:, : cfg.curr_ndim]
     inner = np.all((x > lo + margin * dd) & (x < hi - margin * dd), axis=1)
     pool = part[inner]
-    # each call copies ebf and clears jc first: time 1 step and `steps` steps
-    # and take the rate from the difference, so that set-up is not counted
-    def timed(q, k):
-        t0 = time.perf_counter()
-        ref.gather_push_deposit(np.asfortranarray(q), ebf, k)
-        return time.perf_counter() - t0
+    _CPU.update(cfg=cfg, ebf=ebf, steps=steps, ref=RefHarness(cfg))
 
     n0 = min(200_000, pool.shape[0])
-    per = max(timed(pool[:n0], 3) - timed(pool[:n0], 1), 1e-6) / (2 * n0)
+    per = max(_cpu_timed(pool[:n0], 3) - _cpu_timed(pool[:n0], 1), 1e-6) / (2 * n0)
     n = int(min(pool.shape[0], max(n0, budget_s / (per * steps))))
-    t1 = timed(pool[:n], 1)
-    tk = timed(pool[:n], steps)
-    return dict(value=n * (steps - 1) / (tk - t1), unit="particle-steps/s", cores=1,
-                kind="reference",
-                sample=f"{n} particles of this workload (at least {margin} cells from every face) "
-                       f"x {steps} steps minus 1 step (gather+push+Esirkepov, FP64, reference "
-                       f"Fortran routines compiled -O2 -fdefault-real-8), {tk:.1f} s")
+    t1 = _cpu_timed(pool[:n], 1)
+    tk = _cpu_timed(pool[:n], steps)
+    one = n * (steps - 1) / (tk - t1)
+
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    share = min(pool.shape[0] // cores, n)
+    _CPU["shares"] = [pool[w * share:(w + 1) * share] for w in range(cores)]
+    with mp.get_context("fork").Pool(cores) as workers:
+        res = workers.map(_cpu_worker, range(cores))
+    many = sum(m * (steps - 1) / (b - a) for m, a, b in res)
+    wall = max(b for _, _, b in res)
+    return dict(value=many, unit="particle-steps/s", cores=cores, kind="reference",
+                value_one_core=one,
+                sample=f"{cores} processes at once, each {share} particles of this workload "
+                       f"(at least {margin} cells from every face) x {steps} steps minus 1 "
+                       f"step, rates summed ({wall:.1f} s); one core alone: {n} particles, "
+                       f"{one:.3g} particle-steps/s. Gather+push+Esirkepov, FP64, reference "
+                       f"Fortran routines compiled -O2 -fdefault-real-8")
 
 
 def main():
@@ -198,6 +236,12 @@ def main():
     log(f"inputs: {part.shape[0]} particles, grid {cfg.nx}x{cfg.ny}x{cfg.nz} "
         f"({time.perf_counter() - t:.1f} s)")
 
+    # the CPU leg forks its workers, so it runs before anything touches the GPU
+    cpu = None if args.no_cpu else cpu_baseline(cfg, part, ebf)
+    if cpu:
+        log(f"cpu baseline: {cpu['value']:.4g} particle-steps/s on {cpu['cores']} cores "
+            f"({cpu['sample']})")
+
     # the plasma continues beyond the box: the columns the window brings in
     table = None if args.no_window else plasma_table(cfg, (2, 2, 2), x_start_cell=cfg.nx,
                                                      x_end_cell=cfg.nx + 400)
@@ -220,10 +264,6 @@ def main():
     achieved = kern_bytes / (r32["part_ms"] * 1e-3) / 1e9 if r32["part_ms"] > 0 else 0.0
     step_bytes = cfg.algorithmic_bytes(n_p, 32)
     step_gbs = step_bytes / (r32["ms_per_step"] * 1e-3) / 1e9
-
-    cpu = None if args.no_cpu else cpu_baseline(cfg, part, ebf)
-    if cpu:
-        log(f"cpu baseline: {cpu['value']:.4g} particle-steps/s ({cpu['sample']})")
 
     out = {
         "metric": METRIC,
