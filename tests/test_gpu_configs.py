@@ -13,6 +13,10 @@ the one-species tests leave out.
 * 2-D runs of the batched scan's kind (grid multiples of 32 that are not
   multiples of the tile, other densities and ppc): ten steps against the
   oracle.
+* The 3-D LWFA grid at full size: continuity and charge invariants.
+* A small copy of the reference's TNSA example: three species (electrons,
+  Al9+, protons) with their charges and masses, 108 electrons per cell in a
+  dense foil, an a0 = 23.5 pulse: five steps against the oracle.
 """
 import dataclasses
 
@@ -196,3 +200,55 @@ def test_lwfa_3d_full_size_invariants(precision, tol):
         n = gpu.species_count(0)
     assert n == part.shape[0]
     assert abs(rho.sum() - ek[5]) <= (1e-12 if precision == PREC_F64 else 1e-6) * abs(ek[5])
+
+
+def _tnsa_case():
+    """The reference's TNSA example (examples/input_tnsa.nml) in small: 2-D,
+    lam0 = 0.8, k0 = 100, a foil of Al9+ ions (mass 27 m_p, charge 9) with
+    its electrons at 10 n0 (n0_ref = 1000: n0 ~ 0.57 n_c), electrons at
+    T = 0.3 keV (t0_pl 0.0003 MeV), 12x9 electrons and 2x3 ions per cell as
+    the example's np_per_xc/yc, a rear proton layer; a0 = 23.5 pulse.
+    Species: electrons (1/m = 1), Al (1/(27*1836.15)), protons (1/1836.15),
+    as set_init_param.f90:193-206 makes them."""
+    mp = 1836.15267
+    base = LPConfig(ndim=2, nx=128, ny=64, k0=100.0, yx_rat=2.0, cfl=0.9, lam0=0.8,
+                    n_over_nc=1.0)
+    ncrit = 3.141592653589793 / (2.81794033 * 0.8 * 0.8)
+    cfg = dataclasses.replace(base, n_over_nc=1e-3 * 1000.0 / ncrit, nsp=3,
+                              lorentz_fact=(1.0, 1.0 / (27.0 * mp), 1.0 / mp, 1.0))
+    th = (0.0003 / 0.511) ** 0.5
+    elec = uniform_plasma(cfg, (12, 9), x_start_cell=60, x_end_cell=70, seed=21, n_rel=10.0,
+                          thermal=th)
+    al = uniform_plasma(cfg, (2, 3), x_start_cell=60, x_end_cell=70, seed=22, n_rel=10.0 / 9.0,
+                        charge=9)
+    prot = uniform_plasma(cfg, (2, 3), x_start_cell=70, x_end_cell=72, seed=23, n_rel=1.0,
+                          charge=1)
+    ebf = gaussian_laser(cfg, a0=23.5, w0=0.3, tau_fwhm=0.5, xc=40 * cfg.dx)
+    return cfg, [elec, al, prot], ebf
+
+
+@pytest.mark.parametrize("precision,tol", [(PREC_F64, 1e-11), (PREC_F32, 1e-4)])
+def test_tnsa_like_three_species_match_oracle(precision, tol):
+    cfg, parts, ebf = _tnsa_case()
+    orcs = [Oracle(cfg, lorentz_fact=cfg.lorentz_fact[ic]) for ic in range(3)]
+    ps = [p.copy(order="F") for p in parts]
+    e, jc = ebf.copy(order="F"), orcs[0].zeros_current()
+    print(f"tnsa-like precision={precision} np={[p.shape[0] for p in parts]}", flush=True)
+    with LPStep(cfg, precision, track_ids=True) as gpu:
+        for ic in range(3):
+            gpu.put_species(ic, parts[ic])
+        gpu.put_fields(ebf)
+        for it in range(5):
+            orcs[0].field_xyzbd(e)
+            jc[...] = 0.0
+            for ic in range(3):
+                ps[ic], _ = orcs[ic].particle_stage(ps[ic], e, jc, initial_time=(it == 0))
+            orcs[0].curr_collect(jc)
+            orcs[0].advance_fields(e, jc)
+            gpu.lpf2_evolve(it, initial_time=(it == 0))
+        pg = [gpu.get_species(ic) for ic in range(3)]
+        eg, jg = gpu.get_fields(), gpu.get_current()
+    for ic in range(3):
+        _compare(cfg, ps[ic], pg[ic], e, eg, None, None, tol)
+    if precision == PREC_F64:
+        assert rel_maxerr(jg, jc) <= tol
