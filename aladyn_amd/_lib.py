@@ -54,6 +54,7 @@ SIGNATURES = {
     "aladyn_get_fields": [_P, _D],
     "aladyn_get_current": [_P, _D],
     "aladyn_lpf2_evolve": [_P, C.c_int32, C.c_int32],
+    "aladyn_set_graphs": [_P, C.c_int32],
     "aladyn_pfields_prepare": [_P],
     "aladyn_curr_clean": [_P],
     "aladyn_particle_stage": [_P, C.c_int32, C.c_int32],

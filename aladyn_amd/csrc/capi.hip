@@ -102,7 +102,22 @@ struct aladyn_ctx {
   // timing: event pairs around the particle kernel and the field solve,
   // recorded on the context stream (ring of kRing steps)
   TimerRing tpart, tfield;
+  // captured lpf2_evolve steps (aladyn_set_graphs): one executable graph per
+  // state of the stores and of the box, see step_key
+  bool use_graphs = false;
+  int64_t grid_version = 0;  // bumped when the box moves
+  struct StepGraph {
+    std::vector<int64_t> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<StepGraph> graphs;
 };
+
+static void clear_graphs(aladyn_ctx* c) {
+  for (auto& gr : c->graphs)
+    if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
+  c->graphs.clear();
+}
 
 static int fail(aladyn_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -258,6 +273,7 @@ extern "C" int aladyn_destroy(aladyn_ctx* c) {
   (void)hipFree(c->rho0);
   (void)hipFree(c->rho1);
   (void)hipFree(c->jraw);
+  clear_graphs(c);
   c->tpart.destroy();
   c->tfield.destroy();
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -434,6 +450,7 @@ extern "C" int aladyn_lp_window_xshift(aladyn_ctx* c, int32_t witr, int32_t init
   if (nshx != 0) {
     g.xmn += g.dx * nshx;
     g.xmx += g.dx * nshx;
+    c->grid_version++;
     if (c->have_fields) {
       if (f64(c))
         fields_left_xshift<double>(g, c->ebf64, (int)nshx, c->stream);
@@ -477,6 +494,7 @@ extern "C" int aladyn_set_box_origin(aladyn_ctx* c, double xmin) {
   DevGrid& g = c->g;
   g.xmn = xmin;
   g.xmx = xmin + g.dx * (double)g.nx;
+  c->grid_version++;
   return ALADYN_OK;
 }
 
@@ -680,10 +698,101 @@ extern "C" int aladyn_advance_lpf_fields(aladyn_ctx* c) {
   return ALADYN_OK;
 }
 
+// ------------------------------------------------------- captured steps
+// What a captured step depends on besides the configuration: the box
+// position (kernel arguments) and, per species, the buffer in use, the
+// pinned slot of the overflow count and the layout version (segment
+// arrays and buffers).
+static std::vector<int64_t> step_key(const aladyn_ctx* c) {
+  std::vector<int64_t> k{c->grid_version};
+  for (int ic = 0; ic < c->cfg.nsp; ic++) {
+    const TileStore& t = c->ts[ic];
+    k.insert(k.end(), {(int64_t)t.loaded, (int64_t)t.cur, (int64_t)t.ovf_slot, t.layout_version});
+  }
+  return k;
+}
+
+// The device work of one step after the first (field ghosts, jc = 0, the
+// species' particle kernels, fold and normalisation, Yee update), with
+// nothing that waits: what a graph holds.
+static int enqueue_step(aladyn_ctx* c) {
+  const DevGrid& g = c->g;
+  const int64_t nj = g.ncell * g.curr_ndim;
+  std::string msg;
+  if (f64(c)) {
+    field_xyzbd<double>(g, c->ebf64, g.nfield, c->stream);
+    fill<double>(c->jc64, nj, 0.0, c->stream);
+  } else {
+    field_xyzbd<float>(g, c->ebf32, g.nfield, c->stream);
+    fill<float>(c->jc32, nj, 0.0f, c->stream);
+  }
+  for (int ic = 0; ic < c->cfg.nsp; ic++) {
+    const double alp = 0.5 * g.dt * c->cfg.lorentz_fact[ic];
+    const int r =
+        f64(c) ? tile_store_step_enqueue<double>(&c->ts[ic], c->tg, g, c->ebf64, c->jc64,
+                                                 (double*)c->patch, alp, 0, ic == 0, c->stream,
+                                                 nullptr, nullptr, &msg)
+               : tile_store_step_enqueue<float>(&c->ts[ic], c->tg, g, c->ebf32, c->jc32,
+                                                (float*)c->patch, (float)alp, 0, ic == 0,
+                                                c->stream, nullptr, nullptr, &msg);
+    if (r) return fail(c, r, "%s", msg.c_str());
+  }
+  if (f64(c)) {
+    curr_collect<double>(g, c->jc64, c->stream);
+    advance_fields<double>(g, c->ebf64, c->jc64, c->stream);
+  } else {
+    curr_collect<float>(g, c->jc32, c->stream);
+    advance_fields<float>(g, c->ebf32, c->jc32, c->stream);
+  }
+  HIPCK(c, hipGetLastError());
+  return ALADYN_OK;
+}
+
+// One step through a graph: captured on first use for the current key,
+// replayed after; the host side of each species' step (buffer swap,
+// overflow count, re-layout) follows the launch.
+static int lpf2_evolve_graph(aladyn_ctx* c) {
+  const std::vector<int64_t> key = step_key(c);
+  hipGraphExec_t ex = nullptr;
+  for (auto& gr : c->graphs)
+    if (gr.key == key) ex = gr.exec;
+  if (!ex) {
+    if (c->graphs.size() >= 8) clear_graphs(c);
+    HIPCK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int r = enqueue_step(c);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (r) {
+      if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+      return r;
+    }
+    HIPCK(c, e);
+    const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCK(c, ei);
+    c->graphs.push_back({key, ex});
+  }
+  HIPCK(c, hipGraphLaunch(ex, c->stream));
+  std::string msg;
+  for (int ic = 0; ic < c->cfg.nsp; ic++)
+    if (int r = tile_store_step_finish(&c->ts[ic], c->tg, c->g, c->stream, &msg))
+      return fail(c, r, "%s", msg.c_str());
+  c->j_is_zero = false;
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_set_graphs(aladyn_ctx* c, int32_t enable) {
+  if (!c) return ALADYN_EINVAL;
+  c->use_graphs = enable != 0;
+  if (!c->use_graphs) clear_graphs(c);
+  return ALADYN_OK;
+}
+
 extern "C" int aladyn_lpf2_evolve(aladyn_ctx* c, int32_t iter, int32_t initial_time) {
   (void)iter;
   if (!c) return ALADYN_EINVAL;
   if (!c->have_fields) return fail(c, ALADYN_ESTATE, "lpf2_evolve before put_fields");
+  if (c->use_graphs && !initial_time && !c->diag) return lpf2_evolve_graph(c);
   if (int r = aladyn_pfields_prepare(c)) return r;
   if (int r = aladyn_curr_clean(c)) return r;
   for (int ic = 0; ic < c->cfg.nsp; ic++)

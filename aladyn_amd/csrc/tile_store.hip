@@ -539,6 +539,7 @@ static const char* path_name(const TileGeom& tg) {
 int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const double* part,
                    int64_t ld, int64_t np, hipStream_t s, bool track_ids, std::string* msg) {
   const int D = g.ndim;
+  ts->layout_version++;
   // the record keeps the FP32 weight; charge and part_ind are per species
   int charge = 0, pind = 0;
   for (int64_t n = 0; n < np; n++) {
@@ -675,6 +676,7 @@ static int relayout_seg(TileStore* ts, const TileGeom& tg, hipStream_t s, std::s
 int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
                         std::string* msg) {
   if (!ts->loaded) return 0;
+  ts->layout_version++;
   return with_dr(tg, g.ndim, [&](auto dc, auto rv) {
     return relayout_seg<decltype(dc)::value, decltype(rv)>(ts, tg, s, msg);
   });
@@ -686,6 +688,7 @@ int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hip
 int tile_store_window(TileStore* ts, const TileGeom& tg, const DevGrid& g, int window_cells,
                       hipStream_t s, std::string* msg) {
   if (!ts->loaded || window_cells == 0) return 0;
+  ts->layout_version++;
   return with_dr(tg, g.ndim, [&](auto dc, auto rv) {
     return relayout_seg<decltype(dc)::value, decltype(rv)>(ts, tg, s, msg, window_cells);
   });
@@ -834,6 +837,7 @@ int tile_store_inject_table(TileStore* ts, const TileGeom& tg, const DevGrid& g,
     *msg = "tile store: table injection needs a species that was put first";
     return ALADYN_ESTATE;
   }
+  ts->layout_version++;
   return with_dr(tg, g.ndim, [&](auto dc, auto rv) {
     constexpr int DD = decltype(dc)::value;
     using R = decltype(rv);
@@ -847,6 +851,7 @@ int tile_store_inject(TileStore* ts, const TileGeom& tg, const DevGrid& g, const
                       int64_t ld, int64_t n, hipStream_t s, bool track_ids, std::string* msg) {
   if (n <= 0) return 0;
   if (!ts->loaded) return tile_store_put(ts, tg, g, part, ld, n, s, track_ids, msg);
+  ts->layout_version++;
   return with_dr(tg, g.ndim, [&](auto dc, auto rv) {
     return inject_impl<decltype(dc)::value, decltype(rv)>(ts, tg, g, part, ld, n, s, msg);
   });
@@ -926,9 +931,9 @@ int tile_store_get(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* 
 }
 
 template <typename R>
-int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf, R* jc,
-                    R* patch, R alp, int initial, bool j_is_zero, hipStream_t s,
-                    hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg) {
+int tile_store_step_enqueue(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf,
+                            R* jc, R* patch, R alp, int initial, bool j_is_zero, hipStream_t s,
+                            hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg) {
   if (!ts->loaded) return 0;
   if ((int)sizeof(R) != tg.rbytes) {
     *msg = "tile store: step called with the other precision's arrays";
@@ -967,16 +972,25 @@ int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R
                                              ts->seg_start, ts->seg_cap, ts->ovf_count + cur,
                                              ts->count[nxt], ts->ovf_count + nxt, jc, ts->grow);
   TS_CK(hipGetLastError());
-  ts->cur = nxt;
+  // the overflow count of this step goes to the pinned slot of this step
+  TS_CK(hipMemcpyAsync(ts->h_ovf + ts->ovf_slot, ts->ovf_count + nxt, sizeof(int),
+                       hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+// The host side of a step whose work tile_store_step_enqueue queued (and
+// that may have run inside a graph launched since).
+int tile_store_step_finish(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
+                           std::string* msg) {
+  if (!ts->loaded) return 0;
+  ts->cur = 1 - ts->cur;
   ts->steps_since_sort++;
   // Re-layout on cadence, or as soon as the overflow list is long enough to
   // cost more than a re-layout amortises (its particles deposit with global
-  // atomics).  The count of this step is copied into one pinned slot; the
+  // atomics).  The count of this step was copied into one pinned slot; the
   // host waits only for the previous step's slot, so one step stays queued
   // on the GPU while the host decides.
   const int slot = ts->ovf_slot, prev = 1 - slot;
-  TS_CK(hipMemcpyAsync(ts->h_ovf + slot, ts->ovf_count + nxt, sizeof(int), hipMemcpyDeviceToHost,
-                       s));
   TS_CK(hipEventRecord(ts->ovf_ev[slot], s));
   ts->ovf_valid[slot] = true;
   ts->ovf_slot = prev;
@@ -991,12 +1005,26 @@ int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R
   }
   return 0;
 }
-template int tile_store_step<float>(TileStore*, const TileGeom&, const DevGrid&, const float*,
-                                    float*, float*, float, int, bool, hipStream_t, hipEvent_t,
-                                    hipEvent_t, std::string*);
-template int tile_store_step<double>(TileStore*, const TileGeom&, const DevGrid&, const double*,
-                                     double*, double*, double, int, bool, hipStream_t, hipEvent_t,
-                                     hipEvent_t, std::string*);
+
+template <typename R>
+int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf, R* jc,
+                    R* patch, R alp, int initial, bool j_is_zero, hipStream_t s,
+                    hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg) {
+  if (int r = tile_store_step_enqueue<R>(ts, tg, g, ebf, jc, patch, alp, initial, j_is_zero, s,
+                                         ev_begin, ev_end, msg))
+    return r;
+  return tile_store_step_finish(ts, tg, g, s, msg);
+}
+#define ALADYN_STEP_INST(R)                                                                   \
+  template int tile_store_step<R>(TileStore*, const TileGeom&, const DevGrid&, const R*, R*, R*, \
+                                  R, int, bool, hipStream_t, hipEvent_t, hipEvent_t,           \
+                                  std::string*);                                                \
+  template int tile_store_step_enqueue<R>(TileStore*, const TileGeom&, const DevGrid&,         \
+                                          const R*, R*, R*, R, int, bool, hipStream_t,          \
+                                          hipEvent_t, hipEvent_t, std::string*);
+ALADYN_STEP_INST(float)
+ALADYN_STEP_INST(double)
+#undef ALADYN_STEP_INST
 
 // ------------------------------------------------------------ diagnostics
 template <int D, typename R>

@@ -72,6 +72,9 @@ struct TileStore {
   int64_t np = 0;       // particles at the last put / re-layout
   int64_t next_id = 0;  // id of the next injected particle (ids tracked)
   int steps_since_sort = 0;
+  // bumped by every change of the layout or of the buffers (put, injection,
+  // window, re-layout): work captured into a graph is valid for one version
+  int64_t layout_version = 0;
   int* grow = nullptr;   // ntiles flags: the tile spilled since the last re-layout
   // overflow count of recent steps, read back through two pinned slots: the
   // host waits only for the step before the one just queued
@@ -116,6 +119,16 @@ template <typename R>
 int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf, R* jc,
                     R* patch, R alp, int initial, bool j_is_zero, hipStream_t s,
                     hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg);
+// The two halves of tile_store_step: the device work (kernels and the copy
+// of the overflow count, nothing that waits, so it may be captured into a
+// graph) and the host side (buffer swap, overflow event, re-layout
+// decision), which runs after the work was launched.
+template <typename R>
+int tile_store_step_enqueue(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf,
+                            R* jc, R* patch, R alp, int initial, bool j_is_zero, hipStream_t s,
+                            hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg);
+int tile_store_step_finish(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
+                           std::string* msg);
 int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
                         std::string* msg);
 void tile_store_grid_charge(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* rho,

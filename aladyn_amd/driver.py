@@ -46,14 +46,19 @@ def lp_cycle(gpu, *, tmax: float, tstart: float = 0.0, w_speed: float = 0.0,
     return it, tnow
 
 
-def run_interleaved(steppers, steps: int, *, start_iter: int = 0, initial_time: bool = True):
+def run_interleaved(steppers, steps: int, *, start_iter: int = 0, initial_time: bool = True,
+                    graphs: bool = False):
     """Advances independent runs (one LPStep each) by `steps` lpf2_evolve
     steps, one step of every run in turn.  Each context has its own stream
     and a step waits on the host only for that context's previous step, so
     the runs' kernels overlap on the GPU: small grids, which leave most of
     the chip idle one at a time, fill it together.  initial_time applies to
-    the first step of every run.  The runs share nothing; the result of each
-    is that of running it alone."""
+    the first step of every run.  graphs: replay the steps as captured HIP
+    graphs (aladyn_set_graphs), which cuts the host's launch work per step.
+    The runs share nothing; the result of each is that of running it alone."""
+    if graphs:
+        for gpu in steppers:
+            gpu.set_graphs(True)
     for k in range(steps):
         it = start_iter + k
         first = initial_time and k == 0

@@ -44,7 +44,7 @@ module aladyn_gpu
   public :: aladyn_create, aladyn_destroy, aladyn_last_error, aladyn_error_message
   public :: aladyn_put_species, aladyn_species_count, aladyn_get_species
   public :: aladyn_put_fields, aladyn_get_fields, aladyn_get_current
-  public :: aladyn_lpf2_evolve, aladyn_pfields_prepare, aladyn_curr_clean
+  public :: aladyn_lpf2_evolve, aladyn_set_graphs, aladyn_pfields_prepare, aladyn_curr_clean
   public :: aladyn_particle_stage, aladyn_curr_mpi_collect, aladyn_advance_lpf_fields
   public :: aladyn_lp_window_xshift, aladyn_inject_species, aladyn_box
   public :: aladyn_set_injection, aladyn_injection_state, aladyn_set_box_origin
@@ -119,6 +119,12 @@ module aladyn_gpu
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: iter, initial_time
+    end function
+
+    integer(c_int) function aladyn_set_graphs(ctx, enable) bind(C, name="aladyn_set_graphs")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: enable
     end function
 
     integer(c_int) function aladyn_pfields_prepare(ctx) bind(C, name="aladyn_pfields_prepare")

@@ -129,6 +129,10 @@ class LPStep:
     def lpf2_evolve(self, iter_loc: int = 0, initial_time: bool = False):
         self._ck(self.lib.aladyn_lpf2_evolve(self._h, iter_loc, 1 if initial_time else 0))
 
+    def set_graphs(self, enable: bool = True):
+        """Replays lpf2_evolve steps as captured HIP graphs (aladyn_set_graphs)."""
+        self._ck(self.lib.aladyn_set_graphs(self._h, 1 if enable else 0))
+
     def pfields_prepare(self):
         self._ck(self.lib.aladyn_pfields_prepare(self._h))
 

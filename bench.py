@@ -84,15 +84,17 @@ def log(*a):
 WINDOW_START, WINDOW_EVERY, WINDOW_SPEED = 100, 20, 1.0
 
 
-def run_path(cfg, part, ebf, precision, steps, warmup, table=None):
+def run_path(cfg, part, ebf, precision, steps, warmup, table=None, graphs=False):
     """Times `steps` lpf2_evolve calls after `warmup` untimed ones.  With a
     target table the box moves as lp_run moves it and the entering plasma is
-    injected on the device from the table."""
+    injected on the device from the table.  graphs: steps replayed as HIP
+    graphs (the per-kernel HIP-event timing is then not recorded)."""
     from aladyn_amd import LPStep
 
     gpu = LPStep(cfg, precision)
     gpu.put_species(0, part)
     gpu.put_fields(ebf)
+    gpu.set_graphs(graphs)
     moves = 0
     if table is not None:
         gpu.set_injection(0, table, nptx=0, targ_in=cfg.xmn,
@@ -218,6 +220,8 @@ def main():
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--no-window", action="store_true")
+    ap.add_argument("--graphs", action="store_true",
+                    help="also time the FP32 path with steps replayed as HIP graphs")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -249,6 +253,10 @@ def main():
     log(f"fp32: {r32['ms_per_step']:.3f} ms/step, {r32['pps']:.4g} particle-steps/s, "
         f"particle kernel {r32['part_ms']:.3f} ms, fields {r32['field_ms']:.3f} ms, "
         f"{r32['moves']} window moves")
+    rg = None
+    if args.graphs:
+        rg = run_path(cfg, part, ebf, PREC_F32, args.steps, args.warmup, table, graphs=True)
+        log(f"fp32 graphs: {rg['ms_per_step']:.3f} ms/step, {rg['pps']:.4g} particle-steps/s")
     r64 = None
     if not args.no_fp64:
         r64 = run_path(cfg, part, ebf, PREC_F64, args.steps, args.warmup, table)  # same steps

@@ -105,6 +105,16 @@ int aladyn_get_current(aladyn_ctx *ctx, double *jc);
  * curr_accumulate}, curr_mpi_collect, advance_lpf_fields.  initial_time != 0
  * applies the first-step momentum back-shift (ALaDyn.F90:67). */
 int aladyn_lpf2_evolve(aladyn_ctx *ctx, int32_t iter, int32_t initial_time);
+/* No counterpart in the reference (its steps are Fortran calls).  For
+ * callers whose steps are short enough that launching them costs as much
+ * as running them (many small runs stepped in turn): with enable != 0,
+ * aladyn_lpf2_evolve replays the step's device work as a HIP graph,
+ * captured once per state of the particle stores and of the box (a
+ * re-layout, a window move or a put makes the next step capture again).
+ * The first step (initial_time) and steps with diagnostics on run as usual.
+ * Results are those of the ordinary path; aladyn_kernel_timing does not
+ * time graph steps. */
+int aladyn_set_graphs(aladyn_ctx *ctx, int32_t enable);
 
 /* The stages of the step, for drivers that interleave their own work:
  * pfields_prepare(ebf, nfield, 2, 2) (curr_and_fields_util.f90:156-174) */
