@@ -361,17 +361,6 @@ __device__ inline int wave_max(int v) {
   return v;
 }
 
-// Workgroup b of a one-workgroup-per-tile launch -> tile.  Workgroups are
-// dealt to the 8 XCDs in turn (b mod 8), so XCD x is given the contiguous
-// tile range [x*m, (x+1)*m) (m = ntiles/8, the remainder maps to itself):
-// neighbouring tiles, which share field halos and current patches, then
-// meet in one L2.  A bijection of [0, ntiles).
-__device__ inline int xcd_tile(int b, int ntiles) {
-  const int m = ntiles >> 3;
-  if (b >= (m << 3)) return b;
-  return (b & 7) * m + (b >> 3);
-}
-
 __device__ inline int tile_of(const TileGeom& tg, int cell, int D) {
   const int i = cell & ((1 << tg.bx) - 1);
   const int j = (cell >> tg.bx) & ((1 << tg.by) - 1);
