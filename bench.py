@@ -28,7 +28,11 @@ profiles/k_tile_3d_lwfa_256x128x128.md when it covers this workload; null
 otherwise.  `--calibrate-copy` runs only that copy (1 GiB read + 1 GiB
 written) for the calibration pass.
 
+--workload lwfa2d measures the 2-D LWFA configuration instead (512x256,
+458,752 particles, no window, no CPU leg).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu] [--no-window]
+                       [--workload lwfa3d|lwfa2d] [--graphs]
        python bench.py --calibrate-copy
 """
 from __future__ import annotations
@@ -220,6 +224,9 @@ def main():
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--no-window", action="store_true")
+    ap.add_argument("--workload", choices=("lwfa3d", "lwfa2d"), default="lwfa3d",
+                    help="lwfa3d: the 3-D LWFA configuration (default); lwfa2d: the 2-D "
+                         "LWFA configuration, 512x256 (no window, CPU leg skipped)")
     ap.add_argument("--graphs", action="store_true",
                     help="also time the FP32 path with steps replayed as HIP graphs")
     args = ap.parse_args()
@@ -233,10 +240,15 @@ def main():
         sys.exit(2)
 
     from aladyn_amd import PREC_F32, PREC_F64
-    from aladyn_amd.inputs import lwfa_3d_case, plasma_table
+    from aladyn_amd.inputs import lwfa_2d_case, lwfa_3d_case, plasma_table
 
     t = time.perf_counter()
-    cfg, part, ebf = lwfa_3d_case(args.nx, args.ny, args.nz, seed=0)
+    two_d = args.workload == "lwfa2d"
+    if two_d:
+        cfg, part, ebf = lwfa_2d_case()
+        args.no_window, args.no_cpu = True, True
+    else:
+        cfg, part, ebf = lwfa_3d_case(args.nx, args.ny, args.nz, seed=0)
     log(f"inputs: {part.shape[0]} particles, grid {cfg.nx}x{cfg.ny}x{cfg.nz} "
         f"({time.perf_counter() - t:.1f} s)")
 
@@ -266,9 +278,11 @@ def main():
 
     n_p = 0.5 * (r32["n0"] + r32["n1"])
     cells = cfg.nx * cfg.ny * cfg.nz
-    # dominant kernel = FP32 particle kernel: particle state 60 B/particle
-    # (32 B read, 28 B written) + field gather 24 B/cell + current 12 B/cell
-    kern_bytes = n_p * 60.0 + cells * (24.0 + 12.0)
+    # dominant kernel = FP32 particle kernel: particle state read and written
+    # (3-D: 32 B read, 28 B written; 2-D: 24 + 20) + field gather + current
+    # (3-D: 24 + 12 B/cell; 2-D: 12 + 8)
+    bp32, bc32, bp64, bc64 = (44.0, 20.0, 72.0, 40.0) if two_d else (60.0, 36.0, 104.0, 72.0)
+    kern_bytes = n_p * bp32 + cells * bc32
     achieved = kern_bytes / (r32["part_ms"] * 1e-3) / 1e9 if r32["part_ms"] > 0 else 0.0
     step_bytes = cfg.algorithmic_bytes(n_p, 32)
     step_gbs = step_bytes / (r32["ms_per_step"] * 1e-3) / 1e9
@@ -287,8 +301,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded plasma + Gaussian laser, no checkpoint)",
         "config": {
-            "workload": f"3-D LWFA {cfg.nx}x{cfg.ny}x{cfg.nz}, dx=0.05 dy=dz=0.2, n/nc=0.005, "
-                        f"8 ppc (2x2x2, jittered, seed 0), a0=3 w0=10, quadratic shapes, Yee 2nd order",
+            "workload": (f"2-D LWFA {cfg.nx}x{cfg.ny}, dx=dy=0.05, n/nc=0.01, 2x2 ppc for x > "
+                         f"{cfg.nx // 8} cells (jittered, seed 0), a0=2 w0=8, quadratic shapes, "
+                         f"Yee 2nd order") if two_d else
+                        (f"3-D LWFA {cfg.nx}x{cfg.ny}x{cfg.nz}, dx=0.05 dy=dz=0.2, n/nc=0.005, "
+                         f"8 ppc (2x2x2, jittered, seed 0), a0=3 w0=10, quadratic shapes, Yee 2nd order"),
             "particles": int(r32["n0"]),
             "cells": cells,
             "path": "fp32 tile-sorted (int32 cell + fp32 offset)",
@@ -303,8 +320,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(cfg.nx, cfg.ny, cfg.nz),
-            "kernel": "k_tile<3, 8x8x4 tiles, float>",
+            "traffic": None if two_d else measured_traffic(cfg.nx, cfg.ny, cfg.nz),
+            "kernel": "k_tile<2, 16x16 tiles, float>" if two_d else "k_tile<3, 8x8x4 tiles, float>",
             "kernel_ms": r32["part_ms"],
             "algorithmic_bytes_per_launch": kern_bytes,
         },
@@ -312,9 +329,10 @@ def main():
         "cpu_baseline": cpu,
     }
     if r64:
-        # FP64 particle kernel: 104 B/particle (56 R + 48 W) + gather 48 + J 24 B/cell
+        # FP64 particle kernel: 3-D 104 B/particle (56 R + 48 W) + gather 48 +
+        # J 24 B/cell; 2-D 72 B/particle + 40 B/cell
         n64 = 0.5 * (r64["n0"] + r64["n1"])
-        b64 = n64 * 104.0 + cells * (48.0 + 24.0)
+        b64 = n64 * bp64 + cells * bc64
         a64 = b64 / (r64["part_ms"] * 1e-3) / 1e9 if r64["part_ms"] > 0 else 0.0
         out["fp64"] = {"value": r64["pps"], "ms_per_step": r64["ms_per_step"],
                        "particle_kernel_ms": r64["part_ms"], "field_ms": r64["field_ms"],
