@@ -88,14 +88,14 @@ def log(*a):
 WINDOW_START, WINDOW_EVERY, WINDOW_SPEED = 100, 20, 1.0
 
 
-def run_path(cfg, part, ebf, precision, steps, warmup, table=None, graphs=False):
+def run_path(cfg, part, ebf, precision, steps, warmup, table=None, graphs=False, sort_every=0):
     """Times `steps` lpf2_evolve calls after `warmup` untimed ones.  With a
     target table the box moves as lp_run moves it and the entering plasma is
     injected on the device from the table.  graphs: steps replayed as HIP
     graphs (the per-kernel HIP-event timing is then not recorded)."""
     from aladyn_amd import LPStep
 
-    gpu = LPStep(cfg, precision)
+    gpu = LPStep(cfg, precision, sort_every=sort_every)
     gpu.put_species(0, part)
     gpu.put_fields(ebf)
     gpu.set_graphs(graphs)
@@ -227,6 +227,9 @@ def main():
     ap.add_argument("--workload", choices=("lwfa3d", "lwfa2d"), default="lwfa3d",
                     help="lwfa3d: the 3-D LWFA configuration (default); lwfa2d: the 2-D "
                          "LWFA configuration, 512x256 (no window, CPU leg skipped)")
+    ap.add_argument("--sort-every", type=int, default=0,
+                    help="steps between re-layouts of the particle store (0: the library's "
+                         "default)")
     ap.add_argument("--graphs", action="store_true",
                     help="also time the FP32 path with steps replayed as HIP graphs")
     args = ap.parse_args()
@@ -261,17 +264,20 @@ def main():
     # the plasma continues beyond the box: the columns the window brings in
     table = None if args.no_window else plasma_table(cfg, (2, 2, 2), x_start_cell=cfg.nx,
                                                      x_end_cell=cfg.nx + 400)
-    r32 = run_path(cfg, part, ebf, PREC_F32, args.steps, args.warmup, table)
+    r32 = run_path(cfg, part, ebf, PREC_F32, args.steps, args.warmup, table,
+                   sort_every=args.sort_every)
     log(f"fp32: {r32['ms_per_step']:.3f} ms/step, {r32['pps']:.4g} particle-steps/s, "
         f"particle kernel {r32['part_ms']:.3f} ms, fields {r32['field_ms']:.3f} ms, "
         f"{r32['moves']} window moves")
     rg = None
     if args.graphs:
-        rg = run_path(cfg, part, ebf, PREC_F32, args.steps, args.warmup, table, graphs=True)
+        rg = run_path(cfg, part, ebf, PREC_F32, args.steps, args.warmup, table, graphs=True,
+                      sort_every=args.sort_every)
         log(f"fp32 graphs: {rg['ms_per_step']:.3f} ms/step, {rg['pps']:.4g} particle-steps/s")
     r64 = None
     if not args.no_fp64:
-        r64 = run_path(cfg, part, ebf, PREC_F64, args.steps, args.warmup, table)  # same steps
+        r64 = run_path(cfg, part, ebf, PREC_F64, args.steps, args.warmup, table,  # same steps
+                       sort_every=args.sort_every)
         log(f"fp64: {r64['ms_per_step']:.3f} ms/step, {r64['pps']:.4g} particle-steps/s, "
             f"particle kernel {r64['part_ms']:.3f} ms, fields {r64['field_ms']:.3f} ms, "
             f"{r64['moves']} window moves")
