@@ -719,12 +719,15 @@ static int enqueue_step(aladyn_ctx* c) {
   const DevGrid& g = c->g;
   const int64_t nj = g.ncell * g.curr_ndim;
   std::string msg;
+  // jc = 0: the first species' fold writes every node of jc (it does not
+  // add), so the fill is needed only when that species holds no store
+  const bool fold_sets_jc = c->ts[0].loaded;
   if (f64(c)) {
     field_xyzbd<double>(g, c->ebf64, g.nfield, c->stream);
-    fill<double>(c->jc64, nj, 0.0, c->stream);
+    if (!fold_sets_jc) fill<double>(c->jc64, nj, 0.0, c->stream);
   } else {
     field_xyzbd<float>(g, c->ebf32, g.nfield, c->stream);
-    fill<float>(c->jc32, nj, 0.0f, c->stream);
+    if (!fold_sets_jc) fill<float>(c->jc32, nj, 0.0f, c->stream);
   }
   for (int ic = 0; ic < c->cfg.nsp; ic++) {
     const double alp = 0.5 * g.dt * c->cfg.lorentz_fact[ic];
@@ -794,7 +797,14 @@ extern "C" int aladyn_lpf2_evolve(aladyn_ctx* c, int32_t iter, int32_t initial_t
   if (!c->have_fields) return fail(c, ALADYN_ESTATE, "lpf2_evolve before put_fields");
   if (c->use_graphs && !initial_time && !c->diag) return lpf2_evolve_graph(c);
   if (int r = aladyn_pfields_prepare(c)) return r;
-  if (int r = aladyn_curr_clean(c)) return r;
+  if (c->ts[0].loaded && !c->diag) {
+    // jc = 0 without the fill: the first species' fold writes every node
+    // of jc (diagnostics copy jc before the stage, so they keep the fill)
+    if (int r = freeze_resid(c)) return r;
+    c->j_is_zero = true;
+  } else if (int r = aladyn_curr_clean(c)) {
+    return r;
+  }
   for (int ic = 0; ic < c->cfg.nsp; ic++)
     if (int r = aladyn_particle_stage(c, ic, initial_time)) return r;
   if (int r = aladyn_curr_mpi_collect(c)) return r;
