@@ -162,11 +162,13 @@ __global__ void k_jc_zfold(DevGrid g, T* jc) {
 // curr(:,:,:,i) = djc(i)*curr(:,:,:,i), i = 1..ndim (curr_and_fields_util.f90:115-119)
 template <typename T>
 __global__ void k_jc_scale(DevGrid g, T* jc) {
-  const int64_t n = g.ncell * g.ndim;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    int c = (int)(t / g.ncell);
-    jc[t] = T(g.djc[c]) * jc[t];
+  const int64_t n = g.ncell;
+  for (int c = 0; c < g.ndim; c++) {
+    const T f = T(g.djc[c]);
+    T* a = jc + c * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x)
+      a[t] = f * a[t];
   }
 }
 
@@ -184,7 +186,7 @@ void curr_collect(const DevGrid& g, T* jc, hipStream_t s) {
     int64_t n = (int64_t)(g.ix2 - g.ix1 + 1) * (g.jy2 - g.jy1 + 1) * g.curr_ndim;
     k_jc_zfold<T><<<nblocks(n), kFieldBlock, 0, s>>>(g, jc);
   }
-  k_jc_scale<T><<<nblocks(g.ncell * g.ndim), kFieldBlock, 0, s>>>(g, jc);
+  k_jc_scale<T><<<nblocks(g.ncell), kFieldBlock, 0, s>>>(g, jc);
 }
 
 // ------------------------------------------------------------ ef_bds / bf_bds
