@@ -24,7 +24,6 @@ The envelope, fluid and bunch files of other models are not written.
 """
 from __future__ import annotations
 
-import os
 import struct
 from pathlib import Path
 

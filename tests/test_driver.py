@@ -1,9 +1,7 @@
 """The LP time loop (ALaDyn.F90:66-107, pic_evolve_in_time.f90:113-140,
 run_data_info.f90:68-71, start_all.F90:219-226) on a recording stand-in for
 the device, with expectations worked out from those lines."""
-import math
 
-import pytest
 
 from aladyn_amd.driver import dt_loc, lp_cycle, with_dt_loc
 from cases import small_2d

@@ -26,7 +26,6 @@ import numpy as np
 import pytest
 
 from aladyn_amd.config import LPConfig
-from aladyn_amd.inputs import pack_weight
 from cases import random_fields, random_particles, small_2d, small_3d
 from conftest import rel_maxerr
 from oracle.pyoracle import Oracle

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from aladyn_amd.config import LPConfig
-from aladyn_amd.inputs import pack_weight, uniform_plasma
+from aladyn_amd.inputs import uniform_plasma
 from aladyn_amd.outputs import (IPAR, RPAR, data_out_gpu, fields_out, fortran_e, i_end,
                                 part_pdata_out, read_fields_bin, read_part_bin)
 
