@@ -84,7 +84,7 @@ struct aladyn_ctx {
   // tile-sorted particle stores, one per species
   TileStore ts[ALADYN_MAX_SPECIES];
   TileGeom tg{};
-  void* patch = nullptr;  // per-tile J partial sums (FP32 or FP64)
+  double* patch = nullptr;  // per-tile J partial sums (FP64 in both paths)
   bool j_is_zero = true;
   double* d_red = nullptr;      // reduction outputs (16 doubles)
   double* h_red = nullptr;      // pinned mirror
@@ -92,9 +92,8 @@ struct aladyn_ctx {
   bool diag = false;
   double* rho0 = nullptr;
   double* rho1 = nullptr;
-  void* jraw = nullptr;
+  double* jraw = nullptr;  // the last stage's raw current, summed in FP64
   bool have_resid = false;
-  bool jraw_is_raw = false;  // jraw already holds the stage's raw current
   // lp_window_xshift saved state (window.f90:278-280)
   double win_xlapse = 0.0;
   int win_wi1 = 0;
@@ -251,7 +250,7 @@ extern "C" int aladyn_create(const aladyn_config* cfg, aladyn_ctx** out) {
   }
   if (int r = tile_geom_init(g, a, &c->tg)) return fail(c, ALADYN_EINVAL, "%s", tile_geom_error(r));
   const size_t npatch = (size_t)c->tg.ntiles * c->tg.pvol * g.curr_ndim;
-  HIPCK(c, hipMalloc(&c->patch, npatch * c->tg.rbytes));
+  HIPCK(c, hipMalloc(&c->patch, npatch * sizeof(double)));
   HIPCK(c, hipMalloc(&c->d_red, 32 * sizeof(double)));
   HIPCK(c, hipHostMalloc(&c->h_red, 32 * sizeof(double), hipHostMallocDefault));
   HIPCK(c, hipStreamSynchronize(c->stream));
@@ -586,33 +585,8 @@ extern "C" int aladyn_pfields_prepare(aladyn_ctx* c) {
   return ALADYN_OK;
 }
 
-// The raw current of the last stage is jc - jraw (jraw holds jc before it).
-template <typename T>
-__global__ void k_sub(T* a, const T* b, int64_t n) {
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * blockDim.x)
-    a[t] = b[t] - a[t];
-}
-
-// Turns jraw into the last stage's raw current (jc - jraw) before anything
-// changes jc (clean, fold, normalisation), so the continuity residual of a
-// stage stays available after the rest of lpf2_evolve.
-static int freeze_resid(aladyn_ctx* c) {
-  if (!c->have_resid || c->jraw_is_raw) return 0;
-  const int64_t n = c->g.ncell * c->g.curr_ndim;
-  const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
-  if (f64(c))
-    k_sub<double><<<nb, 256, 0, c->stream>>>((double*)c->jraw, c->jc64, n);
-  else
-    k_sub<float><<<nb, 256, 0, c->stream>>>((float*)c->jraw, c->jc32, n);
-  HIPCK(c, hipGetLastError());
-  c->jraw_is_raw = true;
-  return 0;
-}
-
 extern "C" int aladyn_curr_clean(aladyn_ctx* c) {
   if (!c) return ALADYN_EINVAL;
-  if (int r = freeze_resid(c)) return r;
   const int64_t n = c->g.ncell * c->g.curr_ndim;
   if (f64(c))
     fill<double>(c->jc64, n, 0.0, c->stream);
@@ -625,10 +599,9 @@ extern "C" int aladyn_curr_clean(aladyn_ctx* c) {
 
 static int ensure_diag_buffers(aladyn_ctx* c) {
   const size_t nc = (size_t)c->g.ncell;
-  const size_t esz = f64(c) ? sizeof(double) : sizeof(float);
   if (!c->rho0) HIPCK(c, hipMalloc(&c->rho0, nc * sizeof(double)));
   if (!c->rho1) HIPCK(c, hipMalloc(&c->rho1, nc * sizeof(double)));
-  if (!c->jraw) HIPCK(c, hipMalloc(&c->jraw, nc * c->g.curr_ndim * esz));
+  if (!c->jraw) HIPCK(c, hipMalloc(&c->jraw, nc * c->g.curr_ndim * sizeof(double)));
   return 0;
 }
 
@@ -648,24 +621,20 @@ extern "C" int aladyn_particle_stage(aladyn_ctx* c, int32_t ic, int32_t initial_
     if (int r = ensure_diag_buffers(c)) return r;
     fill<double>(c->rho0, g.ncell, 0.0, c->stream);
     if (int r = grid_charge_dev(c, ic, c->rho0)) return r;
-    // jraw = jc before this species' deposit; the species' raw current is
-    // recovered as jc(after) - jraw below
-    const size_t esz = f64(c) ? sizeof(double) : sizeof(float);
-    HIPCK(c, hipMemcpyAsync(c->jraw, f64(c) ? (void*)c->jc64 : (void*)c->jc32,
-                            (size_t)g.ncell * g.curr_ndim * esz, hipMemcpyDeviceToDevice,
-                            c->stream));
+    // the stage writes its own raw current, summed in FP64, into jraw
+    fill<double>(c->jraw, g.ncell * g.curr_ndim, 0.0, c->stream);
     c->have_resid = false;
-    c->jraw_is_raw = false;
   }
   std::string msg;
   const int init = initial_time ? 1 : 0;
+  double* jraw = c->diag ? c->jraw : nullptr;
   const int r =
-      f64(c) ? tile_store_step<double>(&c->ts[ic], c->tg, g, c->ebf64, c->jc64, (double*)c->patch,
-                                       alp, init, c->j_is_zero, c->stream, c->tpart.begin(),
+      f64(c) ? tile_store_step<double>(&c->ts[ic], c->tg, g, c->ebf64, c->jc64, c->patch, alp,
+                                       init, c->j_is_zero, jraw, c->stream, c->tpart.begin(),
                                        c->tpart.end(), &msg)
-             : tile_store_step<float>(&c->ts[ic], c->tg, g, c->ebf32, c->jc32, (float*)c->patch,
-                                      (float)alp, init, c->j_is_zero, c->stream, c->tpart.begin(),
-                                      c->tpart.end(), &msg);
+             : tile_store_step<float>(&c->ts[ic], c->tg, g, c->ebf32, c->jc32, c->patch,
+                                      (float)alp, init, c->j_is_zero, jraw, c->stream,
+                                      c->tpart.begin(), c->tpart.end(), &msg);
   if (r) return fail(c, r, "%s", msg.c_str());
   if (c->ts[ic].loaded) c->tpart.commit();
   c->j_is_zero = false;
@@ -679,7 +648,6 @@ extern "C" int aladyn_particle_stage(aladyn_ctx* c, int32_t ic, int32_t initial_
 
 extern "C" int aladyn_curr_mpi_collect(aladyn_ctx* c) {
   if (!c) return ALADYN_EINVAL;
-  if (int r = freeze_resid(c)) return r;
   if (f64(c))
     curr_collect<double>(c->g, c->jc64, c->stream);
   else
@@ -733,10 +701,10 @@ static int enqueue_step(aladyn_ctx* c) {
     const double alp = 0.5 * g.dt * c->cfg.lorentz_fact[ic];
     const int r =
         f64(c) ? tile_store_step_enqueue<double>(&c->ts[ic], c->tg, g, c->ebf64, c->jc64,
-                                                 (double*)c->patch, alp, 0, ic == 0, c->stream,
+                                                 c->patch, alp, 0, ic == 0, nullptr, c->stream,
                                                  nullptr, nullptr, &msg)
                : tile_store_step_enqueue<float>(&c->ts[ic], c->tg, g, c->ebf32, c->jc32,
-                                                (float*)c->patch, (float)alp, 0, ic == 0,
+                                                c->patch, (float)alp, 0, ic == 0, nullptr,
                                                 c->stream, nullptr, nullptr, &msg);
     if (r) return fail(c, r, "%s", msg.c_str());
   }
@@ -797,10 +765,8 @@ extern "C" int aladyn_lpf2_evolve(aladyn_ctx* c, int32_t iter, int32_t initial_t
   if (!c->have_fields) return fail(c, ALADYN_ESTATE, "lpf2_evolve before put_fields");
   if (c->use_graphs && !initial_time && !c->diag) return lpf2_evolve_graph(c);
   if (int r = aladyn_pfields_prepare(c)) return r;
-  if (c->ts[0].loaded && !c->diag) {
+  if (c->ts[0].loaded) {
     // jc = 0 without the fill: the first species' fold writes every node
-    // of jc (diagnostics copy jc before the stage, so they keep the fill)
-    if (int r = freeze_resid(c)) return r;
     c->j_is_zero = true;
   } else if (int r = aladyn_curr_clean(c)) {
     return r;
@@ -866,12 +832,8 @@ extern "C" int aladyn_continuity_residual(aladyn_ctx* c, double out[2]) {
   if (!c || !out) return fail(c, ALADYN_EINVAL, "continuity_residual: NULL argument");
   if (!c->have_resid)
     return fail(c, ALADYN_ESTATE, "continuity_residual needs set_diagnostics(1) then a particle_stage");
-  if (int r = freeze_resid(c)) return r;
   HIPCK(c, hipMemsetAsync(c->d_red, 0, 2 * sizeof(double), c->stream));
-  if (f64(c))
-    continuity<double>(c->g, c->rho0, c->rho1, (const double*)c->jraw, c->d_red, c->stream);
-  else
-    continuity<float>(c->g, c->rho0, c->rho1, (const float*)c->jraw, c->d_red, c->stream);
+  continuity<double>(c->g, c->rho0, c->rho1, c->jraw, c->d_red, c->stream);
   HIPCK(c, hipGetLastError());
   HIPCK(c, hipMemcpyAsync(c->h_red, c->d_red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCK(c, hipStreamSynchronize(c->stream));

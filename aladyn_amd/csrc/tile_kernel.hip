@@ -420,7 +420,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
     TileArgs ta, StepConst<R> sc, const R* __restrict__ ebf, RecPtr A, RecPtr B,
     const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
     const int* __restrict__ cnt_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_cur,
-    int* __restrict__ ovf_next, R* __restrict__ patch, int* __restrict__ grow) {
+    int* __restrict__ ovf_next, double* __restrict__ patch, int* __restrict__ grow) {
   using S = TileShape<D, TX, TY, TZ, NT, R>;
   constexpr int NCL = S::NCL, NF = S::NF, GX = S::GX, GY = S::GY, GVOL = S::GVOL;
   constexpr int JX = S::JX, JY = S::JY, PVOL = S::PVOL, QCAP = S::QCAP;
@@ -606,7 +606,9 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
         // the fields are already alp*charge*F
         const R dtg = push<D, R>(q.p, ef, R(1), R(1), sc.dt, sc.initial);
         keep = move_and_test<D, R>(q, dtg, sc.inv, sc.nn);
-        wq = sc.qch * q.w;
+        // real(sp) :: wght = charge*wgh (grid_part_connect.f90:983, 1205):
+        // the product is rounded to FP32 in both paths
+        wq = (R)((float)sc.qch * q.w);
         int sg[3] = {0, 0, 0}, nmoved = 0;
 #pragma unroll
         for (int d = 0; d < D; d++) {
@@ -719,8 +721,8 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
   }
   if (acc_lc >= 0) acc.template flush<JX, JY, PVOL>(patch_node(acc_lc));
   __syncthreads();
-  R* P = patch + (int64_t)t * D * PVOL;
-  for (int q = threadIdx.x; q < D * PVOL; q += NT) P[q] = (R)Jl[q];
+  double* P = patch + (int64_t)t * D * PVOL;
+  for (int q = threadIdx.x; q < D * PVOL; q += NT) P[q] = Jl[q];
 }
 
 // threads per workgroup: one per node-cell of the tile (4 wavefronts for an
@@ -739,7 +741,7 @@ static int lds_bytes_for() {
 template <int D, int TX, int TY, int TZ, typename R>
 static hipError_t launch(const TileGeom& tg, const TileArgs& ta, const StepConst<R>& sc,
                          const R* ebf, const RecPtr& A, const RecPtr& B, const int* ss,
-                         const int* sc_, const int* cc, int* cn, int* oc, int* on, R* patch,
+                         const int* sc_, const int* cc, int* cn, int* oc, int* on, double* patch,
                          int* grow, hipStream_t s) {
   constexpr int NT = tile_threads_for<D, TX, TY, TZ>();
   k_tile<D, TX, TY, TZ, NT, R><<<tg.ntiles, NT, lds_bytes_for<D, TX, TY, TZ, R>(), s>>>(
@@ -776,8 +778,8 @@ template <typename R>
 hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepConst<R>& sc,
                               const R* ebf, const RecPtr& A, const RecPtr& B,
                               const int* seg_start, const int* seg_cap, const int* cnt_cur,
-                              int* cnt_next, int* ovf_cur, int* ovf_next, R* patch, int* grow,
-                              hipStream_t s) {
+                              int* cnt_next, int* ovf_cur, int* ovf_next, double* patch,
+                              int* grow, hipStream_t s) {
   TileArgs ta{tg.nt[0], tg.nt[1], tg.bx, tg.by, g.n1p, g.n2p, g.n3p, g.ncell};
   const int* T = tg.T;
 #define ALADYN_TILE_CASE(DD, X, Y, Z)                                                           \
@@ -798,7 +800,7 @@ hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepCo
 template hipError_t launch_tile_kernel<float>(const TileGeom&, const DevGrid&,
                                               const StepConst<float>&, const float*,
                                               const RecPtr&, const RecPtr&, const int*, const int*,
-                                              const int*, int*, int*, int*, float*, int*,
+                                              const int*, int*, int*, int*, double*, int*,
                                               hipStream_t);
 template hipError_t launch_tile_kernel<double>(const TileGeom&, const DevGrid&,
                                                const StepConst<double>&, const double*,

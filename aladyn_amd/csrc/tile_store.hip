@@ -20,8 +20,8 @@ template <typename R>
 hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepConst<R>& sc,
                               const R* ebf, const RecPtr& A, const RecPtr& B,
                               const int* seg_start, const int* seg_cap, const int* cnt_cur,
-                              int* cnt_next, int* ovf_cur, int* ovf_next, R* patch, int* grow,
-                              hipStream_t s);
+                              int* cnt_next, int* ovf_cur, int* ovf_next, double* patch,
+                              int* grow, hipStream_t s);
 
 // ------------------------------------------------------------ geometry
 static int bits_for(int n) {  // bits to hold values 0..n
@@ -70,27 +70,34 @@ int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg) {
   return 0;
 }
 
-template <typename R>
-struct GlobalAcc {
-  R* jc;
-  const DevGrid* g;
+// Adds a particle's current to the FP64 patch of its tile (nodes T0-2 ..
+// T0+T+1 per axis cover the Esirkepov window of every particle whose old
+// node lies in the tile).
+template <int D>
+struct PatchAcc {
+  double* P;        // the tile's patch, component stride pvol
+  int o[3];         // Fortran node of patch index 0 per axis (T0 - 2)
+  int J0, J1, pvol;
+  template <typename R>
   __device__ void operator()(int c, int i, int j, int k, R v) const {
-    unsafeAtomicAdd(jc + (int64_t)c * g->ncell + fofs(*g, i, j, k), v);
+    const int l = (i - o[0]) + J0 * ((j - o[1]) + J1 * (D == 3 ? k - o[2] : 0));
+    unsafeAtomicAdd(P + c * pvol + l, (double)v);
   }
 };
 
 // ----------------------------------------------------- overflow kernel
-// Particles whose destination segment was full: gather from HBM, deposit
-// with global atomics (after the fold has written jc), re-bin like the tile
-// kernel.  Runs every step; usually there is nothing to do.
+// Particles whose destination segment was full: gather from HBM, push,
+// deposit into the FP64 patch of the tile they start in (global FP64
+// atomics, before the fold sums the patches onto jc, so their current is
+// rounded to the grid's type once like everyone else's), re-bin like the
+// tile kernel.  Runs every step; usually there is nothing to do.
 template <int D, typename R>
 __global__ __launch_bounds__(256) void k_ovf_particles(
     TileGeom tg, DevGrid g, StepConst<R> sc, const R* __restrict__ ebf, RecPtr A, RecPtr B,
     const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
     const int* __restrict__ ovf_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_next,
-    R* __restrict__ jc, int* __restrict__ grow) {
+    double* __restrict__ patch, int* __restrict__ grow) {
   const int cnt = *ovf_cur;
-  const GlobalAcc<R> acc{jc, &g};
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < cnt; n += gridDim.x * blockDim.x) {
     Rec<R> q;
     load_rec<D, R>(A, sc.ovf_base + n, q, tg.bx, tg.by);
@@ -102,9 +109,17 @@ __global__ __launch_bounds__(256) void k_ovf_particles(
     gather<D, R>(q.s, F, ef);
     const int io[3] = {q.N[0], q.N[1], q.N[2]};
     const R so[3] = {q.s[0], q.s[1], q.s[2]};
+    // the tile the particle starts in and its patch
+    int tt[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; d++) tt[d] = (io[d] - 3) / tg.T[d];
+    const int t0 = tt[0] + tg.nt[0] * (tt[1] + tg.nt[1] * tt[2]);
+    const PatchAcc<D> acc{patch + (int64_t)t0 * D * tg.pvol,
+                          {1 + tt[0] * tg.T[0], 1 + tt[1] * tg.T[1], 1 + tt[2] * tg.T[2]},
+                          tg.J[0], tg.J[1], tg.pvol};
     const R dtg = push<D, R>(q.p, ef, sc.qch, sc.alp, sc.dt, sc.initial);
     if (!move_and_test<D, R>(q, dtg, sc.inv, sc.nn)) continue;
-    esirkepov<D, R>(io, so, q.N, q.s, (R)(sc.qch * q.w), acc);
+    esirkepov<D, R>(io, so, q.N, q.s, (R)((float)sc.qch * q.w), acc);  // FP32 charge*wgh
     const int dtile = (q.N[0] - 3) / tg.T[0] +
                       tg.nt[0] * ((q.N[1] - 3) / tg.T[1] + tg.nt[1] * (D == 3 ? (q.N[2] - 3) / tg.T[2] : 0));
     const int slot = reserve_slot(dtile, cnt_next);
@@ -121,11 +136,12 @@ __global__ __launch_bounds__(256) void k_ovf_particles(
 // jc(node) [+]= the sum of the tile patches covering node: at most two tiles
 // per axis (T >= 4), the one whose interior holds the node and the neighbour
 // whose halo reaches it.  One thread per node of a 64 x 4 x 1 block of the
-// grid (no index division); the up to 8 patch values of a component are all
-// loaded before they are summed (compensated, Kahan), in the order of
-// increasing tile index.
+// grid (no index division); the up to 8 FP64 patch values of a component are
+// all loaded before they are summed in FP64, in the order of increasing tile
+// index, and the sum is rounded to the grid's type once.
 template <typename R>
-__global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g, const R* __restrict__ patch,
+__global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g,
+                                              const double* __restrict__ patch,
                                               R* __restrict__ jc, int accumulate) {
   const int D = g.ndim;
   const int idx[3] = {(int)(blockIdx.x * 64 + (threadIdx.x & 63)) + 1,
@@ -156,7 +172,7 @@ __global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g, const R* _
   }
   const int64_t o = (idx[0] - 1) + (int64_t)g.n1p * ((idx[1] - 1) + (int64_t)g.n2p * (idx[2] - 1));
   for (int c = 0; c < D; c++) {
-    R v[8];
+    double v[8];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -166,17 +182,12 @@ __global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g, const R* _
           const bool use = ok[0][e] && ok[1][b] && ok[2][a];
           const int tile = tl[0][e] + tg.nt[0] * (tl[1][b] + tg.nt[1] * tl[2][a]);
           const int l = loc[0][e] + tg.J[0] * (loc[1][b] + tg.J[1] * loc[2][a]);
-          v[a * 4 + b * 2 + e] = use ? patch[((int64_t)tile * D + c) * tg.pvol + l] : R(0);
+          v[a * 4 + b * 2 + e] = use ? patch[((int64_t)tile * D + c) * tg.pvol + l] : 0.0;
         }
-    R sum = accumulate ? jc[(int64_t)c * g.ncell + o] : R(0), comp = R(0);
+    double sum = accumulate ? (double)jc[(int64_t)c * g.ncell + o] : 0.0;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const R y = v[q] - comp;
-      const R s2 = sum + y;
-      comp = (s2 - sum) - y;
-      sum = s2;
-    }
-    jc[(int64_t)c * g.ncell + o] = sum;
+    for (int q = 0; q < 8; q++) sum += v[q];
+    jc[(int64_t)c * g.ncell + o] = (R)sum;
   }
 }
 
@@ -223,25 +234,57 @@ static inline int64_t seg_budget(int64_t np, int64_t ntiles) {
   return np * kSegBudgetNum / kSegBudgetDen + 256 * ntiles + 64;
 }
 
+// nb[t] = the largest count among tile t and its face, edge and corner
+// neighbours.  Plasma streams from tile to tile: a tile that a dense region
+// (a bunch, the rear of a wake) is about to enter holds few particles at a
+// re-layout but needs room for its neighbour's.
+__global__ __launch_bounds__(kB) void k_nbmax(TileGeom tg, const int* __restrict__ v,
+                                              int* __restrict__ nb) {
+  const int t = blockIdx.x * kB + threadIdx.x;
+  if (t >= tg.ntiles) return;
+  const int x = t % tg.nt[0], y = (t / tg.nt[0]) % tg.nt[1], z = t / (tg.nt[0] * tg.nt[1]);
+  int m = 0;
+  for (int c = max(z - 1, 0); c <= min(z + 1, tg.nt[2] - 1); c++)
+    for (int b = max(y - 1, 0); b <= min(y + 1, tg.nt[1] - 1); b++)
+      for (int a = max(x - 1, 0); a <= min(x + 1, tg.nt[0] - 1); a++)
+        m = max(m, v[a + tg.nt[0] * (b + tg.nt[1] * c)]);
+  nb[t] = m;
+}
+
 // out[t] = exclusive scan of v[t], out[n] = total (one workgroup).  With
 // cap_out, v[t] is first turned into the segment capacity of tile t (base
 // capacity plus an even share of what the budget leaves; the tile's grow
-// flag is consumed).
+// flag is consumed).  With nb (k_nbmax of v), the base capacity is that of
+// the neighbourhood's largest count when all of those fit the budget, else
+// the tile's own.
 __global__ __launch_bounds__(kB) void k_scan_tiles(const int* v_in, int n, int* out, int* cap_out,
-                                                   int* grow = nullptr, int64_t budget = 0) {
+                                                   int* grow = nullptr, int64_t budget = 0,
+                                                   const int* nb = nullptr) {
   __shared__ int sh[kB];
-  __shared__ long long red[kB];
+  __shared__ long long red[2][kB];
+  bool use_nb = false;
+  long long base_sum = 0;
   if (cap_out) {
-    long long part = 0;
-    for (int t = threadIdx.x; t < n; t += kB) part += tile_base_capacity(v_in[t], grow && grow[t]);
-    red[threadIdx.x] = part;  // block sum of the base capacities
+    long long own = 0, wide = 0;
+    for (int t = threadIdx.x; t < n; t += kB) {
+      const bool gr = grow && grow[t];
+      own += tile_base_capacity(v_in[t], gr);
+      if (nb) wide += tile_base_capacity(nb[t], gr);
+    }
+    red[0][threadIdx.x] = own;  // block sums of the base capacities
+    red[1][threadIdx.x] = wide;
     __syncthreads();
     for (int o = kB / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      if (threadIdx.x < o) {
+        red[0][threadIdx.x] += red[0][threadIdx.x + o];
+        red[1][threadIdx.x] += red[1][threadIdx.x + o];
+      }
       __syncthreads();
     }
+    use_nb = nb && red[1][0] <= budget;
+    base_sum = use_nb ? red[1][0] : red[0][0];
   }
-  const long long left = cap_out ? budget - red[0] : 0;
+  const long long left = cap_out ? budget - base_sum : 0;
   const long long extra = left > 0 ? left / n : 0;
   __syncthreads();
   long long carry = 0;
@@ -251,7 +294,7 @@ __global__ __launch_bounds__(kB) void k_scan_tiles(const int* v_in, int n, int* 
     if (t < n) {
       v = v_in[t];
       if (cap_out) {
-        v = (int)(tile_base_capacity(v, grow && grow[t]) + extra);
+        v = (int)(tile_base_capacity(use_nb ? nb[t] : v, grow && grow[t]) + extra);
         if (grow) grow[t] = 0;
         cap_out[t] = v;
       }
@@ -510,8 +553,9 @@ static int sort_into_cur(TileStore* ts, const TileGeom& tg, int64_t n, hipStream
   int* h = ts->count[cur];
   fill<int>(h, tg.ntiles, 0, s);
   if (n > 0) k_hist<D, R><<<grid_for(n), kB, 0, s>>>(tg, ts->buf[oth], n, h);
+  k_nbmax<<<(tg.ntiles + kB - 1) / kB, kB, 0, s>>>(tg, h, ts->scratch);
   k_scan_tiles<<<1, kB, 0, s>>>(h, tg.ntiles, ts->seg_start, ts->seg_cap, ts->grow,
-                                seg_budget(ts->ovf_cap, tg.ntiles));
+                                seg_budget(ts->ovf_cap, tg.ntiles), ts->scratch);
   int total = 0;
   TS_CK(hipMemcpyAsync(&total, ts->seg_start + tg.ntiles, sizeof(int), hipMemcpyDeviceToHost, s));
   TS_CK(hipStreamSynchronize(s));
@@ -656,8 +700,9 @@ static int relayout_seg(TileStore* ts, const TileGeom& tg, hipStream_t s, std::s
   k_hist_seg<D, R><<<tg.ntiles + 64, kB, 0, s>>>(tg, ts->buf[cur], ts->seg_start, ts->seg_cap,
                                                  ts->count[cur], ts->ovf_base, ts->ovf_count + cur,
                                                  h, shx);
+  k_nbmax<<<(tg.ntiles + kB - 1) / kB, kB, 0, s>>>(tg, h, ts->scratch);
   k_scan_tiles<<<1, kB, 0, s>>>(h, tg.ntiles, ts->seg_start2, ts->seg_cap2, ts->grow,
-                                seg_budget(ts->ovf_cap, tg.ntiles));
+                                seg_budget(ts->ovf_cap, tg.ntiles), ts->scratch);
   fill<int>(ts->scratch, tg.ntiles, 0, s);
   k_scatter_seg<D, R><<<tg.ntiles + 64, kB, 0, s>>>(tg, ts->buf[cur], ts->seg_start, ts->seg_cap,
                                                     ts->count[cur], ts->ovf_base,
@@ -932,8 +977,9 @@ int tile_store_get(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* 
 
 template <typename R>
 int tile_store_step_enqueue(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf,
-                            R* jc, R* patch, R alp, int initial, bool j_is_zero, hipStream_t s,
-                            hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg) {
+                            R* jc, double* patch, R alp, int initial, bool j_is_zero,
+                            double* jraw, hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end,
+                            std::string* msg) {
   if (!ts->loaded) return 0;
   if ((int)sizeof(R) != tg.rbytes) {
     *msg = "tile store: step called with the other precision's arrays";
@@ -959,18 +1005,21 @@ int tile_store_step_enqueue(TileStore* ts, const TileGeom& tg, const DevGrid& g,
                               ts->seg_cap, ts->count[cur], ts->count[nxt], ts->ovf_count + cur,
                               ts->ovf_count + nxt, patch, ts->grow, s));
   if (ev_end) TS_CK(hipEventRecord(ev_end, s));
-  {
-    const dim3 fb((g.n1p + 63) / 64, (g.n2p + 3) / 4, g.n3p);
-    k_fold<R><<<fb, kB, 0, s>>>(tg, g, patch, jc, j_is_zero ? 0 : 1);
-  }
+  // the overflowed particles add their current to the patches, then the
+  // patches are summed onto jc
   if (D == 2)
     k_ovf_particles<2, R><<<512, kB, 0, s>>>(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt],
                                              ts->seg_start, ts->seg_cap, ts->ovf_count + cur,
-                                             ts->count[nxt], ts->ovf_count + nxt, jc, ts->grow);
+                                             ts->count[nxt], ts->ovf_count + nxt, patch, ts->grow);
   else
     k_ovf_particles<3, R><<<512, kB, 0, s>>>(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt],
                                              ts->seg_start, ts->seg_cap, ts->ovf_count + cur,
-                                             ts->count[nxt], ts->ovf_count + nxt, jc, ts->grow);
+                                             ts->count[nxt], ts->ovf_count + nxt, patch, ts->grow);
+  {
+    const dim3 fb((g.n1p + 63) / 64, (g.n2p + 3) / 4, g.n3p);
+    k_fold<R><<<fb, kB, 0, s>>>(tg, g, patch, jc, j_is_zero ? 0 : 1);
+    if (jraw) k_fold<double><<<fb, kB, 0, s>>>(tg, g, patch, jraw, 0);
+  }
   TS_CK(hipGetLastError());
   // the overflow count of this step goes to the pinned slot of this step
   TS_CK(hipMemcpyAsync(ts->h_ovf + ts->ovf_slot, ts->ovf_count + nxt, sizeof(int),
@@ -1008,20 +1057,20 @@ int tile_store_step_finish(TileStore* ts, const TileGeom& tg, const DevGrid& g, 
 
 template <typename R>
 int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf, R* jc,
-                    R* patch, R alp, int initial, bool j_is_zero, hipStream_t s,
-                    hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg) {
-  if (int r = tile_store_step_enqueue<R>(ts, tg, g, ebf, jc, patch, alp, initial, j_is_zero, s,
-                                         ev_begin, ev_end, msg))
+                    double* patch, R alp, int initial, bool j_is_zero, double* jraw,
+                    hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg) {
+  if (int r = tile_store_step_enqueue<R>(ts, tg, g, ebf, jc, patch, alp, initial, j_is_zero,
+                                         jraw, s, ev_begin, ev_end, msg))
     return r;
   return tile_store_step_finish(ts, tg, g, s, msg);
 }
-#define ALADYN_STEP_INST(R)                                                                   \
-  template int tile_store_step<R>(TileStore*, const TileGeom&, const DevGrid&, const R*, R*, R*, \
-                                  R, int, bool, hipStream_t, hipEvent_t, hipEvent_t,           \
-                                  std::string*);                                                \
+#define ALADYN_STEP_INST(R)                                                                    \
+  template int tile_store_step<R>(TileStore*, const TileGeom&, const DevGrid&, const R*, R*,   \
+                                  double*, R, int, bool, double*, hipStream_t, hipEvent_t,     \
+                                  hipEvent_t, std::string*);                                   \
   template int tile_store_step_enqueue<R>(TileStore*, const TileGeom&, const DevGrid&,         \
-                                          const R*, R*, R*, R, int, bool, hipStream_t,          \
-                                          hipEvent_t, hipEvent_t, std::string*);
+                                          const R*, R*, double*, R, int, bool, double*,        \
+                                          hipStream_t, hipEvent_t, hipEvent_t, std::string*);
 ALADYN_STEP_INST(float)
 ALADYN_STEP_INST(double)
 #undef ALADYN_STEP_INST
@@ -1038,7 +1087,8 @@ __global__ void k_grid_charge_rec(TileGeom tg, RecPtr U, int64_t n, double qch, 
     double a[3][3] = {{0, 0, 0}, {0, 0, 0}, {1, 0, 0}};
 #pragma unroll
     for (int d = 0; d < D; d++) qweights<double>((double)rec_s<R>(U, q, d), a[d]);
-    const double wq = qch * (double)rec_w<R>(U, q);
+    // wght = charge*wgh in FP32 (grid_part_util.f90:280-286)
+    const double wq = (double)((float)qch * rec_w<R>(U, q));
     const int nk = D == 3 ? 3 : 1;
     for (int k1 = 0; k1 < nk; k1++)
       for (int j1 = 0; j1 < 3; j1++)
@@ -1093,7 +1143,7 @@ __global__ __launch_bounds__(kB) void k_energy_rec(TileGeom tg, DevGrid g, RecPt
       ek[2] += p[2];
       ek[3] += w * (y * p[2] - z * p[1]);
     }
-    ek[5] += qch * w;
+    ek[5] += (double)((float)qch * (float)w);
     ek[6] += w * gam1;
     ek[7] = fmax(ek[7], gam1);
   }

@@ -115,18 +115,22 @@ int tile_store_get(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* 
 // One species-loop body: gather + push + box test + deposit + re-binning.
 // jc(raw) = (j_is_zero ? 0 : jc) + this species' Esirkepov current.  R is
 // float or double and must match the precision the geometry was made for.
+// The per-tile current patches are FP64 in both paths.  jraw, when given,
+// receives this species' raw current alone, summed in FP64 (the
+// continuity diagnostic).
 template <typename R>
 int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf, R* jc,
-                    R* patch, R alp, int initial, bool j_is_zero, hipStream_t s,
-                    hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg);
+                    double* patch, R alp, int initial, bool j_is_zero, double* jraw,
+                    hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg);
 // The two halves of tile_store_step: the device work (kernels and the copy
 // of the overflow count, nothing that waits, so it may be captured into a
 // graph) and the host side (buffer swap, overflow event, re-layout
 // decision), which runs after the work was launched.
 template <typename R>
 int tile_store_step_enqueue(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf,
-                            R* jc, R* patch, R alp, int initial, bool j_is_zero, hipStream_t s,
-                            hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg);
+                            R* jc, double* patch, R alp, int initial, bool j_is_zero,
+                            double* jraw, hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end,
+                            std::string* msg);
 int tile_store_step_finish(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
                            std::string* msg);
 int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,

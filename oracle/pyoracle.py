@@ -291,6 +291,19 @@ class RefHarness:
         self.L.ref_grid_charge(part.shape[0], _p(_f(part)), max(part.shape[0], 1), _p(rho))
         return rho
 
+    @staticmethod
+    def se_coeff(ndim, der_ord, cfl, yx_rat, lib: Path | None = None):
+        """set_field_param's se_coeff(1:2) (grid_field_param.f90:36-99)."""
+        p = Path(lib) if lib else REF_LIB
+        if not p.exists():
+            raise ImportError(f"{p} missing: run `make -C oracle ref` where the reference exists")
+        L = C.CDLL(str(p))
+        L.ref_se_coeff.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _D]
+        L.ref_se_coeff.restype = None
+        out = np.zeros(2)
+        L.ref_se_coeff(ndim, der_ord, cfl, yx_rat, _p(out))
+        return out
+
     def gather_push_deposit(self, part, ebf, nsteps=1):
         jc = np.zeros(self.cfg.current_shape, order="F")
         self.L.ref_gather_push_deposit(part.shape[0], _p(_f(part)), max(part.shape[0], 1),

@@ -15,6 +15,7 @@
 !   init_lpf_momenta, lpf_momenta_and_positions (boris_push.f90:34-168)
 !   esirkepov_2d_curr / esirkepov_3d_curr      (grid_part_connect.f90:956-1334)
 !   set_grid_charge                            (grid_part_util.f90:240-335)
+!   set_field_param                            (grid_field_param.f90:36-99)
 ! Routines in modules that need an MPI library (field_charge_multiply,
 ! reset_all_part_dist, the Maxwell solver) are not reachable here.
 !=============================================================================
@@ -30,6 +31,7 @@ module ref_harness
   use grid_part_connect
   use boris_push
   use grid_part_util
+  use grid_field_param
   implicit none
 
   integer :: h_n1p = 0, h_n2p = 0, h_n3p = 0
@@ -204,6 +206,34 @@ contains
       c_part(n, 1:nd2 + 1) = spec(1)%part(n, 1:nd2 + 1)
     end do
     c_jc = jc
+  end subroutine
+
+  ! set_field_param: the finite-difference coefficients se_coeff(1:2) of the
+  ! Yee x-derivative for der_ord 2, 3 or 4 at the given cfl and cell ratio
+  subroutine ref_se_coeff(c_ndim, c_der_ord, c_cfl, c_yx_rat, c_out) bind(C, name='ref_se_coeff')
+    integer(c_int), value :: c_ndim, c_der_ord
+    real(c_double), value :: c_cfl, c_yx_rat
+    real(c_double), intent(out) :: c_out(2)
+
+    ndim = c_ndim
+    der_ord = c_der_ord
+    cfl = c_cfl
+    yx_rat = c_yx_rat
+    comoving = .false.
+    nx = 8
+    ny = 8
+    nz = 8
+    ny_loc = ny
+    nz_loc = nz
+    if (allocated(ww1)) deallocate (ww1)
+    if (allocated(ww2)) deallocate (ww2)
+    if (allocated(ww0)) deallocate (ww0)
+    if (allocated(wr)) deallocate (wr)
+    if (allocated(wl)) deallocate (wl)
+    if (allocated(var)) deallocate (var)
+    se_coeff = 0.0
+    call set_field_param
+    c_out = se_coeff
   end subroutine
 
   subroutine load_part(np, c_part, ld)

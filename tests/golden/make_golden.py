@@ -1,5 +1,9 @@
 """Writes tests/golden/ref_vectors_*.npz from the reference's own routines.
 
+Also writes tests/golden/ref_se_coeff.json: set_field_param's se_coeff
+(grid_field_param.f90:36-99) for der_ord 2, 3, 4 over a set of (ndim, cfl,
+dy/dx).
+
 Runs the reference Fortran gather (set_part2d/3d_hcell_acc), push
 (init_lpf_momenta + lpf_momenta_and_positions), Esirkepov deposit
 (esirkepov_2d/3d_curr) and set_grid_charge -- compiled from the reference
@@ -64,6 +68,23 @@ def make(name, cfg):
     print(name, {k: np.asarray(v).shape for k, v in out.items()})
 
 
+SE_CASES = [(nd, der, cfl, rat) for nd in (2, 3) for der in (2, 3, 4)
+            for cfl in (0.5, 0.8, 0.9) for rat in (1.0, 2.0, 4.0)]
+
+
+def make_se_coeff():
+    import json
+
+    rows = []
+    for nd, der, cfl, rat in SE_CASES:
+        c = RefHarness.se_coeff(nd, der, cfl, rat)
+        rows.append({"ndim": nd, "der_ord": der, "cfl": cfl, "yx_rat": rat,
+                     "se_coeff": [float(c[0]).hex(), float(c[1]).hex()]})
+    (HERE / "ref_se_coeff.json").write_text(json.dumps(rows, indent=1) + "\n")
+    print("se_coeff", len(rows), "cases")
+
+
 if __name__ == "__main__":
     for name, cfg in configs().items():
         make(name, cfg)
+    make_se_coeff()

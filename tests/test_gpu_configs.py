@@ -73,8 +73,7 @@ def test_two_species_match_oracle(dim, precision, tol):
         eg, jg = gpu.get_fields(), gpu.get_current()
     for ic in range(2):
         _compare(cfg, ps[ic], pg[ic], e, eg, None, None, tol)
-    if precision == PREC_F64:
-        assert rel_maxerr(jg, jc) <= tol
+    assert rel_maxerr(jg, jc) <= tol
 
 
 @pytest.mark.parametrize("precision,tol", [(PREC_F64, 1e-11), (PREC_F32, 1e-4)])
@@ -93,7 +92,7 @@ def test_higher_order_yee_matches_oracle(dim, der_ord, precision, tol):
             p = orc.lpf2_evolve(p, e, jc, initial_time=(it == 0))
             gpu.lpf2_evolve(it, initial_time=(it == 0))
         pg, eg, jg = gpu.get_species(0), gpu.get_fields(), gpu.get_current()
-    _compare(cfg, p, pg, e, eg, jc if precision == PREC_F64 else None, jg, tol)
+    _compare(cfg, p, pg, e, eg, jc, jg, tol)
 
 
 @pytest.mark.parametrize("precision,tol", [(PREC_F64, 1e-12), (PREC_F32, 1e-5)])
@@ -163,7 +162,7 @@ def test_scan_like_2d_runs(nx, ny, ppc, nnc, a0, precision, tol):
             p = orc.lpf2_evolve(p, e, jc, initial_time=(it == 0))
             gpu.lpf2_evolve(it, initial_time=(it == 0))
         pg, eg, jg = gpu.get_species(0), gpu.get_fields(), gpu.get_current()
-    _compare(cfg, p, pg, e, eg, jc if precision == PREC_F64 else None, jg, tol)
+    _compare(cfg, p, pg, e, eg, jc, jg, tol)
 
 
 @pytest.mark.parametrize("precision,tol", [(PREC_F64, 1e-12), (PREC_F32, 1e-5)])
@@ -250,5 +249,4 @@ def test_tnsa_like_three_species_match_oracle(precision, tol):
         eg, jg = gpu.get_fields(), gpu.get_current()
     for ic in range(3):
         _compare(cfg, ps[ic], pg[ic], e, eg, None, None, tol)
-    if precision == PREC_F64:
-        assert rel_maxerr(jg, jc) <= tol
+    assert rel_maxerr(jg, jc) <= tol

@@ -2,12 +2,15 @@
 particles (2x2x2 per cell, Maxwellian momenta 1e-3, linear weight up-ramp
 over the first 128 cells, n/nc = 0.002), a0 = 4 pulse, 500 steps.  The FP32
 path's moments after 500 steps are compared with the FP64 path's on the same
-input: kinetic energy, charge, momentum sums and field energy within 1e-3.
+input, as that study specifies: kinetic energy, charge, momentum sums and
+field energy within 1e-3.  After the last step each path's grid charge
+(set_grid_charge, grid_part_util.f90:240-335) sums to its particles' charge
+(the quadratic weights are a partition of unity).
 
 The FP64 run holds about 250 GB of device memory (two 64-byte record
 buffers of 3.5 slots per particle, 120 GB each), the FP32 run half of that;
-the input is about 30 GB of host memory and the whole test takes a few
-minutes, so it runs only with ALADYN_LARGE=1 set.
+the input is about 30 GB of host memory.  The whole test takes a few
+minutes, about 45 s of it on the GPU.
 """
 import os
 import time
@@ -15,12 +18,7 @@ import time
 import numpy as np
 import pytest
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skipif(os.environ.get("ALADYN_LARGE") != "1",
-                       reason="full-size 1024x256x256 study: set ALADYN_LARGE=1 "
-                              "(about 250 GB of HBM and a few minutes)"),
-]
+pytestmark = pytest.mark.gpu
 
 NSTEPS = 500
 
@@ -57,6 +55,14 @@ def test_large_3d_fp32_against_fp64_full_size():
             wall = time.perf_counter() - t
             kms, fms = gpu.kernel_timing()
             ek, _ = gpu.energy_momenta(0)
+            rho = gpu.grid_charge(0)
+            # FP64 sums over 5.4e8 particles in either order: 1e-10 (one
+            # particle of the 5.4e8 is 2e-9 of the charge)
+            qtol = 1e-10
+            print(f"precision={prec}: grid charge {rho.sum():.9e}, particle charge "
+                  f"{ek[5]:.9e}", flush=True)
+            assert abs(rho.sum() - ek[5]) <= qtol * abs(ek[5])
+            del rho
             out[prec] = (gpu.species_count(0), ek, gpu.field_energy())
             print(f"precision={prec}: {NSTEPS} steps in {wall:.2f} s, "
                   f"{npart * NSTEPS / wall:.3e} particle-steps/s, particle kernel {kms:.2f} ms "

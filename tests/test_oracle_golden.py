@@ -51,3 +51,24 @@ def test_oracle_reproduces_reference_vectors(name):
     orc.esirkepov(p, e, jc)
     assert rel_maxerr(jc, f["esirkepov_jc"]) <= TOL
     assert rel_maxerr(orc.grid_charge(part), f["grid_charge"]) <= TOL
+
+
+def test_se_coeff_reproduces_reference_table():
+    """se_coeff of the oracle against set_field_param's own values
+    (grid_field_param.f90:36-99), written by make_golden.py from the
+    reference compiled here: der_ord 2, 3 and 4 over cfl and dy/dx."""
+    import json
+
+    rows = json.loads((GOLDEN / "ref_se_coeff.json").read_text())
+    assert len(rows) == 54
+    for r in rows:
+        nd, rat = r["ndim"], r["yx_rat"]
+        kw = dict(nz=8, zx_rat=rat) if nd == 3 else {}
+        cfg = LPConfig(ndim=nd, nx=16, ny=8, k0=20.0, yx_rat=rat, cfl=r["cfl"],
+                       der_ord=r["der_ord"], **kw)
+        want = [float.fromhex(v) for v in r["se_coeff"]]
+        got = list(Oracle(cfg).g.se_coeff[:])
+        assert got == want, (r, got)
+    # der_ord 4 is the fourth-order staggered pair (9/8, -1/24) whatever cfl
+    d4 = {tuple(float.fromhex(v) for v in r["se_coeff"]) for r in rows if r["der_ord"] == 4}
+    assert d4 == {(1.125, -1.0 / 24.0)}

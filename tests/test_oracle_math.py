@@ -210,18 +210,22 @@ def test_esirkepov_particle_at_rest_deposits_nothing():
 
 
 # ------------------------------------------------------------- Yee solver
-def _plane_wave_state(cfg, k, t0, amp_e):
+def _plane_wave_state(cfg, k, t0, amp_e, se=(1.0, 0.0)):
     """E^n (at t0) and B^n of an exact lattice eigenmode of the split Yee step.
 
     The split step (B half, E full, B half) is leapfrog with B^{n+1/2};
     its state at t_n holds B^n = B^{n-1/2} - (dt/2) curl_h E^n.  For a mode
     E = e cos(k.x - w t), B^{n-1/2} = bvec cos(k.x - w t_{n-1/2}) with
     e.K = 0, bvec = (K x e)/W (K_d = 2 sin(k_d h_d/2)/h_d, W = 2 sin(w dt/2)/dt).
+    With the 4-point x-derivative of der_ord 3/4,
+    c1 (f(i+1) - f(i)) + c2 (f(i+2) - f(i-1)) over h, the x symbol is
+    K_x = (2/h) (c1 sin(k h/2) + c2 sin(3 k h/2)).
     """
     dt = cfg.dt
     hs = (cfg.dx, cfg.dy, cfg.dz)
     ndim = cfg.ndim
     K = np.array([2 * math.sin(k[d] * hs[d] / 2) / hs[d] if d < ndim else 0.0 for d in range(3)])
+    K[0] = 2 * (se[0] * math.sin(k[0] * hs[0] / 2) + se[1] * math.sin(3 * k[0] * hs[0] / 2)) / hs[0]
     W = math.sqrt(float(np.sum(K * K)))
     w = 2.0 / dt * math.asin(W * dt / 2.0)
     e = amp_e
@@ -274,6 +278,41 @@ def test_yee_advances_a_lattice_eigenmode_exactly(mk, kdir):
           slice(cfg.kz1 - 1 + m, cfg.kz2 - m) if cfg.ndim == 3 else slice(None))
     for c in range(cfg.nfield):
         assert np.max(np.abs(ebf[sl + (c,)] - want[sl + (c,)])) <= 1e-12, c
+
+
+@pytest.mark.parametrize("der_ord", [3, 4])
+@pytest.mark.parametrize("mk,kdir", [(_cfg2, (1.0, 0.6, 0.0)), (_cfg3, (1.0, 0.4, -0.3))])
+def test_higher_order_yee_advances_its_lattice_eigenmode(mk, kdir, der_ord):
+    """der_ord 3 and 4 (rote/rotb with aph2, grid_fields.f90:1405-1552): the
+    4-point x-stencil c1 (f(i+1) - f(i)) + c2 (f(i+2) - f(i-1)) has the x
+    symbol (2/h)(c1 sin(kh/2) + c2 sin(3kh/2)); the plane wave built on it is
+    advanced exactly.  Other offsets (f(i+2) - f(i-2), a dropped c2, c2 on the
+    wrong side) change the symbol and miss by far more than the bound; the
+    der_ord 2 mode is checked to miss, so the test sees the stencil.  The
+    coefficients are the oracle's, pinned to set_field_param itself in
+    test_oracle_vs_reference / test_oracle_golden."""
+    cfg = mk(nx=72, der_ord=der_ord)
+    orc = Oracle(cfg)
+    se = tuple(orc.g.se_coeff[:])
+    assert se[1] != 0.0
+    k = 2 * math.pi * np.array(kdir) / (8.0 * cfg.dx)
+    amp = np.array([0.2, 1.0, 0.5])
+    ebf, *_ = _plane_wave_state(cfg, k, 0.0, amp, se)
+    nsteps = 3
+    jc = orc.zeros_current()
+    for _ in range(nsteps):
+        orc.advance_fields(ebf, jc)
+    want, *_ = _plane_wave_state(cfg, k, nsteps * cfg.dt, amp, se)
+    plain, *_ = _plane_wave_state(cfg, k, nsteps * cfg.dt, amp)
+    # the x-stencil reaches two cells: a wider margin along x
+    m, mx = nsteps + 3, 6 * nsteps + 3
+    sl = (slice(cfg.ix1 - 1 + mx, cfg.ix2 - mx), slice(cfg.jy1 - 1 + m, cfg.jy2 - m),
+          slice(cfg.kz1 - 1 + m, cfg.kz2 - m) if cfg.ndim == 3 else slice(None))
+    miss = 0.0
+    for c in range(cfg.nfield):
+        assert np.max(np.abs(ebf[sl + (c,)] - want[sl + (c,)])) <= 1e-12, c
+        miss = max(miss, np.max(np.abs(ebf[sl + (c,)] - plain[sl + (c,)])))
+    assert miss > 1e-6
 
 
 def _div_b(cfg, ebf):
@@ -368,6 +407,61 @@ def test_jc_fold_conserves_total_current(mk):
     assert np.allclose(jc.sum(axis=(0, 1, 2)), tot, rtol=1e-13, atol=1e-10)
     # ghosts along x and y emptied
     assert np.all(jc[cfg.ix1 - 2, jj, kk] == 0) and np.all(jc[cfg.ix2:cfg.ix2 + 2, jj, kk] == 0)
+
+
+def _fold_moves(cfg):
+    """jc_xyzbd's moves (mpi_curr_interface.f90:551-643, ib* = 0, one rank),
+    written out by hand in 1-based Fortran indices for one interior row per
+    face: (source ghost node, destination edge node).  Lower x and z fold one
+    layer (ix1-1 -> ix1, kz1-1 -> kz1), lower y two (jy1-1, jy1-2 -> jy1),
+    every upper face two (n+1, n+2 -> n).  The x fold runs over the interior
+    rows j = jy1..jy2, k = kz1..kz2 only, the y and z folds over the
+    interior columns, so ghost edges and corners stay where they are."""
+    i1, n1, j1, n2 = cfg.ix1, cfg.ix2, cfg.jy1, cfg.jy2
+    three = cfg.ndim == 3
+    k1, n3 = (cfg.kz1, cfg.kz2) if three else (1, 1)
+    j, k = j1 + 3, (k1 + 2 if three else 1)   # an interior row
+    i = i1 + 5                                  # an interior column
+    moves = [((i1 - 1, j, k), (i1, j, k)),
+             ((n1 + 1, j + 1, k), (n1, j + 1, k)), ((n1 + 2, j + 2, k), (n1, j + 2, k)),
+             ((i, j1 - 1, k), (i, j1, k)), ((i + 1, j1 - 2, k), (i + 1, j1, k)),
+             ((i + 2, n2 + 1, k), (i + 2, n2, k)), ((i + 3, n2 + 2, k), (i + 3, n2, k))]
+    stay = [(i1 - 2, j, k),                        # second lower-x layer: not folded
+            (i1 - 1, j1 - 1, k), (n1 + 1, n2 + 2, k)]   # ghost edges
+    if three:
+        moves += [((i, j, k1 - 1), (i, j, k1)),
+                  ((i + 1, j, n3 + 1), (i + 1, j, n3)), ((i + 2, j + 1, n3 + 2), (i + 2, j + 1, n3))]
+        stay += [(i, j + 1, k1 - 2),               # second lower-z layer: not folded
+                 (i, j1 - 1, k1 - 1), (n1 + 2, j, n3 + 1)]
+    return moves, stay
+
+
+@pytest.mark.parametrize("mk", [_cfg2, _cfg3])
+def test_jc_fold_moves_each_ghost_to_its_edge_node(mk):
+    """Per face and component: a distinct power of two in each ghost node
+    lands on the named edge node, its source is cleared, and the nodes the
+    fold does not cover keep their value.  A fold into the wrong edge cell
+    (ix1+1 instead of ix1), a missed layer or a ghost left uncleared fails."""
+    cfg = mk()
+    orc = Oracle(cfg)
+    moves, stay = _fold_moves(cfg)
+    for comp in range(cfg.curr_ndim):
+        jc = orc.zeros_current()
+        want = orc.zeros_current()
+        v = 1.0
+        for (src, dst) in moves:
+            jc[tuple(a - 1 for a in src) + (comp,)] = v
+            want[tuple(a - 1 for a in dst) + (comp,)] += v
+            v *= 2.0
+        base = 3.0 ** 10
+        for n, node in enumerate(stay):
+            jc[tuple(a - 1 for a in node) + (comp,)] = base * (n + 1)
+            want[tuple(a - 1 for a in node) + (comp,)] = base * (n + 1)
+        other = jc.copy(order="F")
+        orc.jc_xyzbd(jc)
+        assert np.array_equal(jc, want), comp
+        # components do not mix
+        assert np.count_nonzero(other[..., [c for c in range(cfg.curr_ndim) if c != comp]]) == 0
 
 
 def test_mur_abc_absorbs_an_outgoing_pulse():

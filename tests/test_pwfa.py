@@ -17,15 +17,14 @@ the envelope/fluid model), so parity is the LP step's, species by species:
   bunch momentum, field energy) agree with FP64 within 1e-3.
 
 Round-off bars of the full-size run: FP64 continuity <= 1e-12 of max|drho|
-and Gauss-law change <= 1e-9 of max ompe*rho.  In FP32 the bunch's current
-(nb = n0, about 110 particles per cell at the centre, moving 0.65 cells a
-step) is about 50 times its node-to-node difference over sigma_x = 20
-cells, and the FP32 current array carries a relative rounding of the
-current itself, so the residual of the FP32 stage is ~50 times the FP32
-rounding unit (observed 4.6e-5; bar 1e-4), and the Gauss law drifts by that
-residual over the steps (observed 1.8e-4 to 3.6e-4 of max ompe*rho after
-400 steps; bar 1e-3).  The laser workloads, whose currents are not
-dominated by one rigid flow, meet 1e-5 (test_gpu_parity.py).
+and Gauss-law change <= 1e-9 of max ompe*rho; FP32 continuity <= 1e-5 (the
+north-star bar) and Gauss-law change <= 1e-3 after 400 steps.  The stage's
+raw current is summed in FP64 (register and LDS sums per tile, FP64 tile
+patches, the overflowed particles' current added to those patches with FP64
+atomics) and rounded to FP32 once per node; the diagnostic takes it before
+that rounding.  Rounding the bunch's raw current to FP32 once would move
+the residual by about 4e-7 (a CPU restatement of the same bunch, 2,000,000
+particles in a 192x72x72 box); its FP32 fluxes add a few times 1e-7.
 """
 import math
 
@@ -127,8 +126,7 @@ def test_pwfa_small_matches_oracle(precision, tol):
             assert np.max(np.abs(pg[ic][:, d] - ps[ic][:, d])) / span <= tol
         assert rel_maxerr(pg[ic][:, 3:6], ps[ic][:, 3:6]) <= tol
     assert rel_maxerr(eg, e) <= tol
-    if precision == 64:
-        assert rel_maxerr(jg, jc) <= tol
+    assert rel_maxerr(jg, jc) <= tol
 
 
 @pytest.mark.gpu
@@ -147,7 +145,7 @@ def test_pwfa_full_size_400_steps():
     print(f"pwfa full size: {plasma.shape[0]} + {bunch.shape[0]} particles, input "
           f"{time.perf_counter() - t:.1f} s", flush=True)
     out = {}
-    for prec, gtol, ctol in ((64, 1e-9, 1e-12), (32, 1e-3, 1e-4)):
+    for prec, gtol, ctol in ((64, 1e-9, 1e-12), (32, 1e-3, 1e-5)):
         with LPStep(cfg, prec) as gpu:
             gpu.put_species(0, plasma)
             gpu.put_species(1, bunch)

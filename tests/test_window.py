@@ -147,8 +147,7 @@ def test_window_run_matches_oracle(dim, precision, tol):
         assert np.max(np.abs(pg[:, d] - p[:, d])) / span <= tol
     assert rel_maxerr(pg[:, D:2 * D], p[:, D:2 * D]) <= tol
     assert rel_maxerr(eg, e) <= tol
-    if precision == PREC_F64:  # FP32 currents are compared by the one-step tests
-        assert rel_maxerr(jg, jc) <= tol
+    assert rel_maxerr(jg, jc) <= tol
 
 
 @pytest.mark.gpu
@@ -200,5 +199,4 @@ def test_window_table_injection_matches_oracle(dim, precision, tol):
     assert rel_maxerr(pg[:, D:2 * D], p[:, D:2 * D]) <= tol
     assert np.array_equal(pg[:, 2 * D], p[:, 2 * D])
     assert rel_maxerr(eg, e) <= tol
-    if precision == PREC_F64:
-        assert rel_maxerr(jg, jc) <= tol
+    assert rel_maxerr(jg, jc) <= tol
