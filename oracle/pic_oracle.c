@@ -1180,3 +1180,375 @@ int64_t orc_add_particles(const orc_grid *g, const double *xpt, int64_t c0, int6
   }
   return n;
 }
+
+/* ===================== envelope (ponderomotive) particle routines ======== */
+
+/* qqh_*_spline, one axis (grid_part_lib.f90:145-210, 261-330): quadratic
+ * weights at integer nodes (base i = int(xx+1/2) - 1) and at half-integer
+ * nodes (base ih = int(xx) - 1, offset xx - int(xx) - 1/2). */
+static inline void qqh_axis(double xp, double a[3], double ah[3], int *i, int *ih) {
+  double xx = 3.0 + xp; /* shx = 3 */
+  int ix = (int)(xx + 0.5);
+  double sx = xx - (double)ix;
+  double sx2 = sx * sx;
+  a[1] = 0.75 - sx2;
+  a[2] = 0.5 * (0.25 + sx2 + sx);
+  a[0] = 1. - a[1] - a[2];
+  int ihx = (int)xx;
+  sx = xx - (double)ihx - 0.5;
+  sx2 = sx * sx;
+  ah[1] = 0.75 - sx2;
+  ah[2] = 0.5 * (0.25 + sx2 + sx);
+  ah[0] = 1. - ah[1] - ah[2];
+  *i = ix - 1;
+  *ih = ihx - 1;
+}
+
+/* local grid coordinate of part(n, d), d = 1..3 (set_local_*d_positions,
+ * grid_part_lib.f90:458-519, unstretched) */
+static inline double loc_coord(const orc_grid *g, double x, int d) {
+  if (d == 1) return g->dx_inv * (x - g->xmn);
+  if (d == 2) return g->dy_inv * (x - g->ymn);
+  return g->dz_inv * (x - g->zmn);
+}
+
+void orc_env_acc(const orc_grid *g, int64_t np, const double *part, int64_t ld,
+                 const double *ebf, const double *av, double dt_step, double *ebfp) {
+  /* set_env_acc (grid_part_connect.f90:414-615): E, B, F = |a|^2/2 and
+   * grad F at the particle (quadratic shapes at integer and half-integer
+   * nodes), the approximate gamma of the ponderomotive scheme, exit
+   * pt(1:D) = q E - q^2 grad F/(2 gamp), pt(D+1 ..) = q B/gamp,
+   * pt(2D+1) = wgh/gamp.  av(:,:,:,1) = F at integer nodes,
+   * av(:,:,:,2:D+1) = grad F at the staggered points of E. */
+  const double dth = 0.5 * dt_step;
+  double ax1[3], axh1[3], ay1[3], ayh1[3], az1[3], azh1[3];
+  int i, ih, j, jh, k, kh;
+  if (g->ndim == 2) {
+    const int k2 = 1;
+    for (int64_t n = 0; n < np; n++) {
+      double ap[6] = {0, 0, 0, 0, 0, 0};
+      qqh_axis(loc_coord(g, PC(part, n, 1), 1), ax1, axh1, &i, &ih);
+      qqh_axis(loc_coord(g, PC(part, n, 2), 2), ay1, ayh1, &j, &jh);
+      const double u1 = PC(part, n, 3), u2 = PC(part, n, 4);
+      int charge;
+      float wgh;
+      orc_unpack_weight(PC(part, n, 5), &charge, NULL, &wgh);
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = j + j1;
+        double dvol = ay1[j1];
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i + i1;
+          ap[5] = ap[5] + ax1[i1] * dvol * F4(av, i2, j2, k2, 1);
+        }
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = ih + i1;
+          double dvol1 = dvol * axh1[i1];
+          ap[0] = ap[0] + dvol1 * F4(ebf, i2, j2, k2, 1);
+          ap[3] = ap[3] + dvol1 * F4(av, i2, j2, k2, 2);
+        }
+      }
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = jh + j1;
+        double dvol = ayh1[j1];
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i + i1;
+          double dvol1 = dvol * ax1[i1];
+          ap[1] = ap[1] + dvol1 * F4(ebf, i2, j2, k2, 2);
+          ap[4] = ap[4] + dvol1 * F4(av, i2, j2, k2, 3);
+        }
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = ih + i1;
+          ap[2] = ap[2] + axh1[i1] * dvol * F4(ebf, i2, j2, k2, 3);
+        }
+      }
+      const double q = (double)charge;
+      const double gam2 = 1. + u1 * u1 + u2 * u2 + ap[5];
+      for (int c = 0; c < 3; c++) ap[c] = q * ap[c];
+      for (int c = 3; c < 5; c++) ap[c] = 0.5 * q * q * ap[c];
+      const double aa1 = dth * (ap[0] * u1 + ap[1] * u2);
+      const double b1 = dth * (ap[3] * u1 + ap[4] * u2);
+      const double gam = sqrt(gam2);
+      const double dgam = (aa1 * gam - b1) / gam2;
+      const double gam_inv = (gam - dgam) / gam2;
+      for (int c = 2; c < 5; c++) ap[c] = ap[c] * gam_inv;
+      PC(ebfp, n, 1) = ap[0] - ap[3];
+      PC(ebfp, n, 2) = ap[1] - ap[4];
+      PC(ebfp, n, 3) = ap[2];
+      PC(ebfp, n, 5) = (double)wgh * gam_inv;
+    }
+    return;
+  }
+  for (int64_t n = 0; n < np; n++) {
+    double ap[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    qqh_axis(loc_coord(g, PC(part, n, 1), 1), ax1, axh1, &i, &ih);
+    qqh_axis(loc_coord(g, PC(part, n, 2), 2), ay1, ayh1, &j, &jh);
+    qqh_axis(loc_coord(g, PC(part, n, 3), 3), az1, azh1, &k, &kh);
+    const double u[3] = {PC(part, n, 4), PC(part, n, 5), PC(part, n, 6)};
+    int charge;
+    float wgh;
+    orc_unpack_weight(PC(part, n, 7), &charge, NULL, &wgh);
+    for (int k1 = 0; k1 <= 2; k1++) {
+      int k2 = k + k1;
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = j + j1;
+        double dvol = ay1[j1] * az1[k1];
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i1 + i;
+          ap[9] = ap[9] + ax1[i1] * dvol * F4(av, i2, j2, k2, 1);
+        }
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i1 + ih;
+          double dvol1 = dvol * axh1[i1];
+          ap[0] = ap[0] + dvol1 * F4(ebf, i2, j2, k2, 1);
+          ap[6] = ap[6] + dvol1 * F4(av, i2, j2, k2, 2);
+        }
+      }
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = jh + j1;
+        double dvol = ayh1[j1] * az1[k1];
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i + i1;
+          double dvol1 = dvol * ax1[i1];
+          ap[1] = ap[1] + dvol1 * F4(ebf, i2, j2, k2, 2);
+          ap[7] = ap[7] + dvol1 * F4(av, i2, j2, k2, 3);
+        }
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i1 + ih;
+          ap[5] = ap[5] + axh1[i1] * dvol * F4(ebf, i2, j2, k2, 6);
+        }
+      }
+    }
+    for (int k1 = 0; k1 <= 2; k1++) {
+      int k2 = kh + k1;
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = jh + j1;
+        double dvol = ayh1[j1] * azh1[k1];
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i1 + i;
+          ap[3] = ap[3] + ax1[i1] * dvol * F4(ebf, i2, j2, k2, 4);
+        }
+      }
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = j + j1;
+        double dvol = ay1[j1] * azh1[k1];
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = ih + i1;
+          ap[4] = ap[4] + axh1[i1] * dvol * F4(ebf, i2, j2, k2, 5);
+        }
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i1 + i;
+          double dvol1 = dvol * ax1[i1];
+          ap[2] = ap[2] + dvol1 * F4(ebf, i2, j2, k2, 3);
+          ap[8] = ap[8] + dvol1 * F4(av, i2, j2, k2, 4);
+        }
+      }
+    }
+    const double q = (double)charge;
+    const double gam2 = 1. + u[0] * u[0] + u[1] * u[1] + u[2] * u[2] + ap[9];
+    for (int c = 0; c < 6; c++) ap[c] = q * ap[c];
+    for (int c = 6; c < 9; c++) ap[c] = 0.5 * q * q * ap[c];
+    const double aa1 = dth * (ap[0] * u[0] + ap[1] * u[1] + ap[2] * u[2]);
+    const double b1 = dth * (ap[6] * u[0] + ap[7] * u[1] + ap[8] * u[2]);
+    const double gam = sqrt(gam2);
+    const double dgam = (aa1 * gam - b1) / gam2;
+    const double gam_inv = (gam - dgam) / gam2;
+    for (int c = 3; c < 9; c++) ap[c] = ap[c] * gam_inv;
+    for (int c = 0; c < 3; c++) PC(ebfp, n, c + 1) = ap[c] - ap[c + 6];
+    for (int c = 3; c < 6; c++) PC(ebfp, n, c + 1) = ap[c];
+    PC(ebfp, n, 7) = (double)wgh * gam_inv;
+  }
+}
+
+void orc_env_momenta(const orc_grid *g, int64_t np, double *part, int64_t ld, double *ebfp) {
+  /* lpf_env_momenta (boris_push.f90:170-223): Boris rotation with the
+   * particle's gamp fixed, ebfp(1:D) = q(E - q grad F/(2 gamp)),
+   * ebfp(D+1..) = qB/gamp; exit p^{n+1/2} and ebfp(1:D) = x^n. */
+  const double dt_lp = g->dt, alp = 0.5 * dt_lp * g->lorentz_fact;
+  if (g->ndim == 2) {
+    for (int64_t p = 0; p < np; p++) {
+      double pp[2] = {PC(part, p, 3), PC(part, p, 4)}, efp[3], vp[2], vph[2];
+      for (int c = 0; c < 3; c++) efp[c] = alp * PC(ebfp, p, c + 1);
+      vp[0] = pp[0] + efp[0];
+      vp[1] = pp[1] + efp[1];
+      const double bb1 = efp[2];
+      const double b2 = 1. + bb1 * bb1;
+      vph[0] = vp[0] + vp[1] * bb1;
+      vph[1] = vp[1] - vp[0] * bb1;
+      vph[0] = vph[0] / b2;
+      vph[1] = vph[1] / b2;
+      PC(part, p, 3) = 2. * vph[0] - pp[0];
+      PC(part, p, 4) = 2. * vph[1] - pp[1];
+      PC(ebfp, p, 1) = PC(part, p, 1);
+      PC(ebfp, p, 2) = PC(part, p, 2);
+    }
+    return;
+  }
+  for (int64_t p = 0; p < np; p++) {
+    double pp[3], efp[6], vp[3], bb[3], vph[3];
+    for (int c = 0; c < 3; c++) pp[c] = PC(part, p, c + 4);
+    for (int c = 0; c < 6; c++) efp[c] = alp * PC(ebfp, p, c + 1);
+    for (int c = 0; c < 3; c++) vp[c] = efp[c] + pp[c];
+    for (int c = 0; c < 3; c++) bb[c] = efp[c + 3];
+    const double b2 = 1. + (bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2]);
+    const double bv = bb[0] * vp[0] + bb[1] * vp[1] + bb[2] * vp[2];
+    vph[0] = vp[0] + vp[1] * bb[2] - vp[2] * bb[1] + bb[0] * bv;
+    vph[1] = vp[1] + vp[2] * bb[0] - vp[0] * bb[2] + bb[1] * bv;
+    vph[2] = vp[2] + vp[0] * bb[1] - vp[1] * bb[0] + bb[2] * bv;
+    for (int c = 0; c < 3; c++) vph[c] = vph[c] / b2;
+    for (int c = 0; c < 3; c++) PC(part, p, c + 4) = 2. * vph[c] - pp[c];
+    for (int c = 0; c < 3; c++) PC(ebfp, p, c + 1) = PC(part, p, c + 1);
+  }
+}
+
+void orc_env_density(const orc_grid *g, int64_t np, double *ebfp, int64_t ld, double *den) {
+  /* set_env_density (grid_part_connect.f90:884-952): den += wgh/gamp with
+   * the quadratic density shapes at x^n; ebfp(1:D) = x^n on entry, the
+   * local coordinates on exit, ebfp(2D+1) = wgh/gamp. */
+  double ax1[3], ay1[3], az1[3] = {1., 0., 0.};
+  int i, j, k;
+  const int D = g->ndim;
+  const int cw = 2 * D + 1;
+  for (int64_t n = 0; n < np; n++) {
+    for (int d = 1; d <= D; d++) PC(ebfp, n, d) = loc_coord(g, PC(ebfp, n, d), d);
+    const double wghp = PC(ebfp, n, cw);
+    qden_axis(PC(ebfp, n, 1), ax1, &i);
+    qden_axis(PC(ebfp, n, 2), ay1, &j);
+    i = i - 1;
+    j = j - 1;
+    if (D == 2) {
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = j + j1;
+        double dvol = ay1[j1] * wghp;
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i1 + i;
+          double dvol1 = dvol * ax1[i1];
+          F4(den, i2, j2, 1, 1) = F4(den, i2, j2, 1, 1) + dvol1;
+        }
+      }
+    } else {
+      qden_axis(PC(ebfp, n, 3), az1, &k);
+      k = k - 1;
+      for (int k1 = 0; k1 <= 2; k1++) {
+        int k2 = k + k1;
+        for (int j1 = 0; j1 <= 2; j1++) {
+          int j2 = j + j1;
+          double dvol = ay1[j1] * az1[k1] * wghp;
+          for (int i1 = 0; i1 <= 2; i1++) {
+            int i2 = i1 + i;
+            double dvol1 = dvol * ax1[i1];
+            F4(den, i2, j2, k2, 1) = F4(den, i2, j2, k2, 1) + dvol1;
+          }
+        }
+      }
+    }
+  }
+}
+
+void orc_env_grad_interp(const orc_grid *g, int64_t np, const double *part, int64_t ld,
+                         const double *av, double *ebfp) {
+  /* set_env_grad_interp (grid_part_connect.f90:760-882): grad F and F
+   * (F = |a|^2/2 at t^{n+1/2}) at x^n: ebfp(1:D) = grad F, ebfp(D+1) = F. */
+  double ax1[3], axh1[3], ay1[3], ayh1[3], az1[3], azh1[3];
+  int i, ih, j, jh, k, kh;
+  if (g->ndim == 2) {
+    const int k2 = 1;
+    for (int64_t n = 0; n < np; n++) {
+      double ap[4] = {0, 0, 0, 0};
+      qqh_axis(loc_coord(g, PC(part, n, 1), 1), ax1, axh1, &i, &ih);
+      qqh_axis(loc_coord(g, PC(part, n, 2), 2), ay1, ayh1, &j, &jh);
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = j + j1;
+        double dvol = ay1[j1];
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i1 + ih;
+          double dvol1 = dvol * axh1[i1];
+          ap[0] = ap[0] + dvol1 * F4(av, i2, j2, k2, 2);
+          i2 = i1 + i;
+          ap[2] = ap[2] + ax1[i1] * dvol * F4(av, i2, j2, k2, 1);
+        }
+      }
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = jh + j1;
+        double dvol = ayh1[j1];
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i + i1;
+          double dvol1 = dvol * ax1[i1];
+          ap[1] = ap[1] + dvol1 * F4(av, i2, j2, k2, 3);
+        }
+      }
+      for (int c = 0; c < 3; c++) PC(ebfp, n, c + 1) = ap[c];
+    }
+    return;
+  }
+  for (int64_t n = 0; n < np; n++) {
+    double ap[4] = {0, 0, 0, 0};
+    qqh_axis(loc_coord(g, PC(part, n, 1), 1), ax1, axh1, &i, &ih);
+    qqh_axis(loc_coord(g, PC(part, n, 2), 2), ay1, ayh1, &j, &jh);
+    qqh_axis(loc_coord(g, PC(part, n, 3), 3), az1, azh1, &k, &kh);
+    for (int k1 = 0; k1 <= 2; k1++) {
+      int k2 = k + k1;
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = j + j1;
+        double dvol = ay1[j1] * az1[k1];
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i1 + ih;
+          double dvol1 = dvol * axh1[i1];
+          ap[0] = ap[0] + dvol1 * F4(av, i2, j2, k2, 2);
+          i2 = i1 + i;
+          ap[3] = ap[3] + ax1[i1] * dvol * F4(av, i2, j2, k2, 1);
+        }
+      }
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = jh + j1;
+        double dvol = ayh1[j1] * az1[k1];
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i + i1;
+          double dvol1 = dvol * ax1[i1];
+          ap[1] = ap[1] + dvol1 * F4(av, i2, j2, k2, 3);
+        }
+      }
+      k2 = kh + k1;
+      for (int j1 = 0; j1 <= 2; j1++) {
+        int j2 = j + j1;
+        double dvol = ay1[j1] * azh1[k1];
+        for (int i1 = 0; i1 <= 2; i1++) {
+          int i2 = i + i1;
+          double dvol1 = dvol * ax1[i1];
+          ap[2] = ap[2] + dvol1 * F4(av, i2, j2, k2, 4);
+        }
+      }
+    }
+    for (int c = 0; c < 4; c++) PC(ebfp, n, c + 1) = ap[c];
+  }
+}
+
+void orc_env_positions(const orc_grid *g, int64_t np, double *part, int64_t ld, double *ebfp) {
+  /* lpf_env_positions (boris_push.f90:225-295), iform < 2, no comoving
+   * shift: gamp at t^{n+1/2} from p^{n+1/2} and F, corrected by grad F;
+   * x^{n+1} = x^n + dt p/gamp; exit ebfp(1:D) = dt v, ebfp(D+1:2D) = x^n,
+   * ebfp(2D+1) = the weight slot (the Esirkepov deposit's input). */
+  const double dt_lp = g->dt;
+  const int D = g->ndim, ch = 2 * D + 1;
+  for (int64_t p = 0; p < np; p++) {
+    double pp[3] = {0, 0, 0}, vp[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) {
+      pp[d] = PC(part, p, D + d + 1);
+      vp[d] = PC(ebfp, p, d + 1);
+    }
+    const double gam2 = D == 2 ? 1. + (pp[0] * pp[0] + pp[1] * pp[1]) + PC(ebfp, p, 3)
+                               : 1. + (pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]) +
+                                     PC(ebfp, p, 4);
+    const double gam = sqrt(gam2);
+    const double gam3 = gam2 * gam;
+    const double b2 = D == 2 ? 0.25 * (pp[0] * vp[0] + pp[1] * vp[1])
+                             : 0.25 * (pp[0] * vp[0] + pp[1] * vp[1] + pp[2] * vp[2]);
+    double gam_inv = D == 2 ? 1. / gam : 1. / sqrt(gam2);
+    gam_inv = gam_inv * (1. - dt_lp * b2 / gam3);
+    for (int d = 0; d < D; d++) vp[d] = dt_lp * gam_inv * pp[d];
+    for (int d = 0; d < D; d++) PC(ebfp, p, D + d + 1) = PC(part, p, d + 1);
+    PC(ebfp, p, ch) = dt_lp * gam_inv;
+    for (int d = 0; d < D; d++) PC(part, p, d + 1) = PC(part, p, d + 1) + vp[d];
+    for (int d = 0; d < D; d++) PC(ebfp, p, d + 1) = vp[d];
+    PC(ebfp, p, ch) = PC(part, p, ch);
+  }
+}

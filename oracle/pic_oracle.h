@@ -172,6 +172,19 @@ int64_t orc_add_particles(const orc_grid *g, const double *xpt, int64_t c0, int6
                           const double *wghpt, const double *wghyz, int charge,
                           double *part, int64_t ld, int64_t np);
 
+/* The envelope (ponderomotive) model's particle routines, restated from
+ * set_env_acc, lpf_env_momenta, set_env_density, set_env_grad_interp and
+ * lpf_env_positions (grid_part_connect.f90:414-952, boris_push.f90:170-295).
+ * av(n1p, n2p, n3p, 1+curr_ndim): F = |a|^2/2 at integer nodes, then grad F
+ * at the staggered points of E.  ebfp as the reference's stage buffer. */
+void orc_env_acc(const orc_grid *g, int64_t np, const double *part, int64_t ld,
+                 const double *ebf, const double *av, double dt_step, double *ebfp);
+void orc_env_momenta(const orc_grid *g, int64_t np, double *part, int64_t ld, double *ebfp);
+void orc_env_density(const orc_grid *g, int64_t np, double *ebfp, int64_t ld, double *den);
+void orc_env_grad_interp(const orc_grid *g, int64_t np, const double *part, int64_t ld,
+                         const double *av, double *ebfp);
+void orc_env_positions(const orc_grid *g, int64_t np, double *part, int64_t ld, double *ebfp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,11 @@ class Oracle:
                                   C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
             "orc_add_particles": [_G, _D, _I64, _I64, _D, C.c_int, _D, C.c_int, _D, _D, C.c_int,
                                   _D, _I64, _I64],
+            "orc_env_acc": [_G, _I64, _D, _I64, _D, _D, C.c_double, _D],
+            "orc_env_momenta": [_G, _I64, _D, _I64, _D],
+            "orc_env_density": [_G, _I64, _D, _I64, _D],
+            "orc_env_grad_interp": [_G, _I64, _D, _I64, _D, _D],
+            "orc_env_positions": [_G, _I64, _D, _I64, _D],
         }
         for name, at in sig.items():
             fn = getattr(L, name)
@@ -228,6 +233,56 @@ class Oracle:
         assert n == np0 + add
         return out
 
+    # envelope model: av(n1p, n2p, n3p, 1 + curr_ndim) = F, grad F
+    def zeros_envelope(self):
+        c = self.cfg
+        return np.zeros((c.n1p, c.n2p, c.n3p, c.curr_ndim + 1), dtype=np.float64, order="F")
+
+    def env_acc(self, part, ebf, av, ebfp=None):
+        """set_env_acc: returns the stage buffer (q E - q^2 grad F/2gamp,
+        q B/gamp, wgh/gamp)."""
+        ebfp = self.zeros_stage(part.shape[0]) if ebfp is None else ebfp
+        self.L.orc_env_acc(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part),
+                           _p(_f(ebf)), _p(_f(av)), self.cfg.dt, _p(ebfp))
+        return ebfp
+
+    def env_momenta(self, part, ebfp):
+        self.L.orc_env_momenta(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part),
+                               _p(ebfp))
+
+    def env_density(self, ebfp, np_, den=None):
+        """set_env_density: adds wgh/gamp shapes to den (n1p, n2p, n3p)."""
+        c = self.cfg
+        den = np.zeros((c.n1p, c.n2p, c.n3p), order="F") if den is None else den
+        self.L.orc_env_density(C.byref(self.g), np_, _p(ebfp), max(ebfp.shape[0], 1), _p(_f(den)))
+        return den
+
+    def env_grad_interp(self, part, av, ebfp):
+        self.L.orc_env_grad_interp(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part),
+                                   _p(_f(av)), _p(ebfp))
+
+    def env_positions(self, part, ebfp):
+        self.L.orc_env_positions(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part),
+                                 _p(ebfp))
+
+    def env_particle_step(self, part, ebf, av0, av1, jc):
+        """The particle side of env_lpf2_evolve (env_evolve_in_time.f90:196-
+        361) for one species: set_env_acc with (F, grad F) av0 at t^n,
+        lpf_env_momenta, set_env_density, then set_env_grad_interp with av1 at
+        t^{n+1/2}, lpf_env_positions, cell_part_dist and the Esirkepov
+        deposit into jc.  Returns (surviving particles, density)."""
+        p = part.copy(order="F")
+        ebfp = self.env_acc(p, ebf, av0)
+        self.env_momenta(p, ebfp)
+        den = self.env_density(ebfp.copy(order="F"), p.shape[0])
+        self.env_grad_interp(p, av1, ebfp)
+        self.env_positions(p, ebfp)
+        n = self.part_dist(p, ebfp)
+        p = np.asfortranarray(p[:n])
+        e = np.asfortranarray(ebfp[:max(n, 1)])
+        self.esirkepov(p, e, jc, n)
+        return p, den
+
     def grid_charge(self, part):
         rho = np.zeros((self.cfg.n1p, self.cfg.n2p, self.cfg.n3p), dtype=np.float64, order="F")
         self.L.orc_grid_charge(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part), _p(rho))
@@ -290,6 +345,45 @@ class RefHarness:
         rho = np.zeros((self.cfg.n1p, self.cfg.n2p, self.cfg.n3p), order="F")
         self.L.ref_grid_charge(part.shape[0], _p(_f(part)), max(part.shape[0], 1), _p(rho))
         return rho
+
+    def _env_sigs(self):
+        L = self.L
+        L.ref_env_acc.argtypes = [_I64, _D, _I64, _D, _D, C.c_double, _D]
+        L.ref_env_momenta.argtypes = [_I64, _D, _I64, _D]
+        L.ref_env_density.argtypes = [_I64, _I64, _D, _D]
+        L.ref_env_grad_interp.argtypes = [_I64, _D, _I64, _D, _D]
+        L.ref_env_positions.argtypes = [_I64, _D, _I64, _D]
+        for f in (L.ref_env_acc, L.ref_env_momenta, L.ref_env_density, L.ref_env_grad_interp,
+                  L.ref_env_positions):
+            f.restype = None
+
+    def env_acc(self, part, ebf, av, ebfp=None):
+        self._env_sigs()
+        n = part.shape[0]
+        ebfp = np.zeros((max(n, 1), self.cfg.ncol), order="F") if ebfp is None else ebfp
+        self.L.ref_env_acc(n, _p(_f(part)), max(n, 1), _p(_f(ebf)), _p(_f(av)), self.cfg.dt,
+                           _p(ebfp))
+        return ebfp
+
+    def env_momenta(self, part, ebfp):
+        self._env_sigs()
+        self.L.ref_env_momenta(part.shape[0], _p(_f(part)), max(part.shape[0], 1), _p(ebfp))
+
+    def env_density(self, ebfp, np_, den=None):
+        self._env_sigs()
+        c = self.cfg
+        den = np.zeros((c.n1p, c.n2p, c.n3p), order="F") if den is None else den
+        self.L.ref_env_density(np_, max(ebfp.shape[0], 1), _p(ebfp), _p(_f(den)))
+        return den
+
+    def env_grad_interp(self, part, av, ebfp):
+        self._env_sigs()
+        self.L.ref_env_grad_interp(part.shape[0], _p(_f(part)), max(part.shape[0], 1),
+                                   _p(_f(av)), _p(ebfp))
+
+    def env_positions(self, part, ebfp):
+        self._env_sigs()
+        self.L.ref_env_positions(part.shape[0], _p(_f(part)), max(part.shape[0], 1), _p(ebfp))
 
     @staticmethod
     def se_coeff(ndim, der_ord, cfl, yx_rat, lib: Path | None = None):

@@ -16,6 +16,9 @@
 !   esirkepov_2d_curr / esirkepov_3d_curr      (grid_part_connect.f90:956-1334)
 !   set_grid_charge                            (grid_part_util.f90:240-335)
 !   set_field_param                            (grid_field_param.f90:36-99)
+!   set_env_acc, set_env_grad_interp, set_env_density
+!                                              (grid_part_connect.f90:414-952)
+!   lpf_env_momenta, lpf_env_positions         (boris_push.f90:170-295)
 ! Routines in modules that need an MPI library (field_charge_multiply,
 ! reset_all_part_dist, the Maxwell solver) are not reachable here.
 !=============================================================================
@@ -234,6 +237,113 @@ contains
     se_coeff = 0.0
     call set_field_param
     c_out = se_coeff
+  end subroutine
+
+  ! set_env_acc(ebf, av, spec(1), ebfp, np, dt): ebfp_out(1:2D+1) as it exits
+  subroutine ref_env_acc(np, c_part, ld, c_ebf, c_av, dt_step, c_ebfp) bind(C, name='ref_env_acc')
+    integer(c_int64_t), value :: np, ld
+    real(c_double), value :: dt_step
+    real(c_double), intent(in) :: c_part(ld, nd2 + 1)
+    real(c_double), intent(in) :: c_ebf(h_n1p, h_n2p, h_n3p, nfield)
+    real(c_double), intent(in) :: c_av(h_n1p, h_n2p, h_n3p, curr_ndim + 1)
+    real(c_double), intent(inout) :: c_ebfp(ld, nd2 + 1)
+    real(dp), allocatable :: av(:, :, :, :)
+    integer :: n
+
+    call load_part(int(np), c_part, ld)
+    do n = 1, int(np)
+      ebfp(n, 1:nd2 + 1) = c_ebfp(n, 1:nd2 + 1)
+    end do
+    if (allocated(ebf)) deallocate (ebf)
+    allocate (ebf(h_n1p, h_n2p, h_n3p, nfield))
+    ebf = c_ebf
+    allocate (av(h_n1p, h_n2p, h_n3p, curr_ndim + 1))
+    av = c_av
+    call set_env_acc(ebf, av, spec(1), ebfp, int(np), dt_step)
+    do n = 1, int(np)
+      c_ebfp(n, 1:nd2 + 1) = ebfp(n, 1:nd2 + 1)
+    end do
+    deallocate (av)
+  end subroutine
+
+  ! lpf_env_momenta(spec(1), ebfp, np, 1): part and ebfp in/out
+  subroutine ref_env_momenta(np, c_part, ld, c_ebfp) bind(C, name='ref_env_momenta')
+    integer(c_int64_t), value :: np, ld
+    real(c_double), intent(inout) :: c_part(ld, nd2 + 1)
+    real(c_double), intent(inout) :: c_ebfp(ld, nd2 + 1)
+    integer :: n
+
+    call load_part(int(np), c_part, ld)
+    do n = 1, int(np)
+      ebfp(n, 1:nd2 + 1) = c_ebfp(n, 1:nd2 + 1)
+    end do
+    call lpf_env_momenta(spec(1), ebfp, int(np), 1)
+    do n = 1, int(np)
+      c_part(n, 1:nd2 + 1) = spec(1)%part(n, 1:nd2 + 1)
+      c_ebfp(n, 1:nd2 + 1) = ebfp(n, 1:nd2 + 1)
+    end do
+  end subroutine
+
+  ! set_env_density(ebfp, den, np, 1): den(:,:,:) += wgh/gamp shapes
+  subroutine ref_env_density(np, ld, c_ebfp, c_den) bind(C, name='ref_env_density')
+    integer(c_int64_t), value :: np, ld
+    real(c_double), intent(inout) :: c_ebfp(ld, nd2 + 1)
+    real(c_double), intent(inout) :: c_den(h_n1p, h_n2p, h_n3p)
+    real(dp), allocatable :: efp(:, :), den(:, :, :, :)
+    integer :: n
+
+    allocate (efp(max(int(np), 1), nd2 + 1))
+    do n = 1, int(np)
+      efp(n, 1:nd2 + 1) = c_ebfp(n, 1:nd2 + 1)
+    end do
+    allocate (den(h_n1p, h_n2p, h_n3p, 1))
+    den(:, :, :, 1) = c_den
+    call set_env_density(efp, den, int(np), 1)
+    c_den = den(:, :, :, 1)
+    do n = 1, int(np)
+      c_ebfp(n, 1:nd2 + 1) = efp(n, 1:nd2 + 1)
+    end do
+    deallocate (efp, den)
+  end subroutine
+
+  ! set_env_grad_interp(av, spec(1), ebfp, np, curr_ndim)
+  subroutine ref_env_grad_interp(np, c_part, ld, c_av, c_ebfp) bind(C, name='ref_env_grad_interp')
+    integer(c_int64_t), value :: np, ld
+    real(c_double), intent(in) :: c_part(ld, nd2 + 1)
+    real(c_double), intent(in) :: c_av(h_n1p, h_n2p, h_n3p, curr_ndim + 1)
+    real(c_double), intent(inout) :: c_ebfp(ld, nd2 + 1)
+    real(dp), allocatable :: av(:, :, :, :)
+    integer :: n
+
+    call load_part(int(np), c_part, ld)
+    do n = 1, int(np)
+      ebfp(n, 1:nd2 + 1) = c_ebfp(n, 1:nd2 + 1)
+    end do
+    allocate (av(h_n1p, h_n2p, h_n3p, curr_ndim + 1))
+    av = c_av
+    call set_env_grad_interp(av, spec(1), ebfp, int(np), curr_ndim)
+    do n = 1, int(np)
+      c_ebfp(n, 1:nd2 + 1) = ebfp(n, 1:nd2 + 1)
+    end do
+    deallocate (av)
+  end subroutine
+
+  ! lpf_env_positions(spec(1), ebfp, np): part and ebfp in/out
+  subroutine ref_env_positions(np, c_part, ld, c_ebfp) bind(C, name='ref_env_positions')
+    integer(c_int64_t), value :: np, ld
+    real(c_double), intent(inout) :: c_part(ld, nd2 + 1)
+    real(c_double), intent(inout) :: c_ebfp(ld, nd2 + 1)
+    integer :: n
+
+    call load_part(int(np), c_part, ld)
+    do n = 1, int(np)
+      ebfp(n, 1:nd2 + 1) = c_ebfp(n, 1:nd2 + 1)
+    end do
+    call lpf_env_positions(spec(1), ebfp, int(np))
+    do n = 1, int(np)
+      c_part(n, 1:nd2 + 1) = spec(1)%part(n, 1:nd2 + 1)
+      c_ebfp(n, 1:nd2 + 1) = ebfp(n, 1:nd2 + 1)
+    end do
   end subroutine
 
   subroutine load_part(np, c_part, ld)

@@ -84,7 +84,25 @@ def make_se_coeff():
     print("se_coeff", len(rows), "cases")
 
 
+def make_envelope():
+    """ref_env_{2d,3d}.npz: the envelope particle routines of the reference
+    (set_env_acc, lpf_env_momenta, set_env_density, set_env_grad_interp,
+    lpf_env_positions) on the inputs of tests/test_oracle_envelope.py."""
+    from cases import small_2d, small_3d
+    from test_oracle_envelope import _chain, _inputs
+
+    for name, mk in (("2d", small_2d), ("3d", small_3d)):
+        cfg, _, _ = mk()
+        part, ebf, av0 = _inputs(cfg, n=800)
+        _, _, av1 = _inputs(cfg, seed=9, n=800)
+        out = _chain(RefHarness(cfg), cfg, part, ebf, av0, av1)
+        np.savez_compressed(HERE / f"ref_env_{name}.npz", part=part, ebf=ebf, av0=av0, av1=av1,
+                            dt=cfg.dt, **{f"out_{k}": v for k, v in out.items()})
+        print("envelope", name, {k: v.shape for k, v in out.items()})
+
+
 if __name__ == "__main__":
     for name, cfg in configs().items():
         make(name, cfg)
     make_se_coeff()
+    make_envelope()
