@@ -58,6 +58,9 @@ class LPConfig:
         # ncrit = pi/(rc0*lam0^2) in 1e21 cm^-3 (set_init_param.f90:272);
         # n0_ref in 1e18 cm^-3 so that n_over_nc = 1e-3*n0_ref/ncrit (:279).
         ncrit = PI / (RC0 * self.lam0 * self.lam0)
+        # laser frequency in units c/l0 (set_init_param.f90:291): the
+        # envelope solver's om0
+        self.oml = 2.0 * PI / self.lam0
         self.n0_ref = self.n_over_nc * ncrit * 1.0e3
         # ompe = omega_p^2 at the reference density 1e18 cm^-3 = 1e6 um^-3
         # (set_init_param.f90:254-255, phys_param.f90:57); the plasma density

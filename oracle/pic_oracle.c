@@ -1552,3 +1552,448 @@ void orc_env_positions(const orc_grid *g, int64_t np, double *part, int64_t ld, 
     PC(ebfp, p, ch) = PC(part, p, ch);
   }
 }
+
+/* ------------------------------------------------------------------------
+ * Envelope model, grid side (one rank, all faces physical, unstretched).
+ * ---------------------------------------------------------------------- */
+
+void orc_env_setup(const orc_grid *g, double cfl, double om0, orc_env *e) {
+  /* set_field_param (grid_field_param.f90:36-99): nu, hord_der2 (der_ord 3:
+   * opt_der2 = -(1-nu^2)/12; der_ord 4: -1/12), opt_der1 (der_ord 2: 1;
+   * der_ord 3: (4-nu^2)/3).  hord_der2 is left 0 and opt_der1 1 where the
+   * reference does not set them (der_ord 2 / 4): neither is read there. */
+  double nu = cfl;
+  const double yx_rat = g->dy / g->dx;
+  if (g->ndim > 1) nu = cfl * yx_rat / sqrt(yx_rat * yx_rat + (double)(float)g->ndim - 1.);
+  e->om0 = om0;
+  e->hord_der2 = 0.0;
+  e->opt_der1 = 1.0;
+  if (g->der_ord == 3) {
+    e->hord_der2 = -(1. - nu * nu) / 12.;
+    e->opt_der1 = (4. - nu * nu) / 3.;
+  } else if (g->der_ord == 4) {
+    e->hord_der2 = -1. / 12.;
+  }
+}
+
+void orc_env_grad(const orc_grid *g, double *av) {
+  /* env_grad (grid_fields.f90:426-586) with every face physical (xl_bd ..
+   * zr_bd true on one rank) and loc_yg = loc_zg = 1: av(.,1) = F on the
+   * interior; exit av(.,2:1+D) = staggered grad F on the interior, with the
+   * one-sided forms at the faces, and F extrapolated into the z ghost
+   * planes kz1-1, kz2+1 (3-D). */
+  const int ix1 = g->ix1, ix2 = g->ix2, jy1 = g->jy1, jy2 = g->jy2, kz1 = g->kz1, kz2 = g->kz2;
+  const double a_hcd = 13. / 12., b_hcd = -1. / 24.;
+  double ax1, ay1, az1, ax2, ay2, az2;
+  if (g->der_ord < 4) {
+    ax1 = g->dx_inv;
+    ay1 = g->dy_inv;
+    az1 = g->dz_inv;
+    ax2 = 0.;
+    ay2 = 0.;
+    az2 = 0.;
+  } else {
+    ax1 = g->dx_inv * a_hcd;
+    ay1 = g->dy_inv * a_hcd;
+    az1 = g->dz_inv * a_hcd;
+    ax2 = g->dx_inv * b_hcd;
+    ay2 = g->dy_inv * b_hcd;
+    az2 = g->dz_inv * b_hcd;
+  }
+  int i01 = ix1, i02 = ix2, j01 = jy1, j02 = jy2, k01 = kz1, k02 = kz2;
+  /* x: left face (xl_bd) */
+  for (int k = kz1; k <= kz2; k++)
+    for (int j = jy1; j <= jy2; j++)
+      F4(av, ix1, j, k, 2) = g->dx_inv * (F4(av, ix1 + 1, j, k, 1) - F4(av, ix1, j, k, 1));
+  i01 = ix1 + 1;
+  /* x: right face (xr_bd) */
+  for (int k = kz1; k <= kz2; k++)
+    for (int j = jy1; j <= jy2; j++) {
+      const int i = ix2 - 1;
+      F4(av, i, j, k, 2) = g->dx_inv * (F4(av, i + 1, j, k, 1) - F4(av, i, j, k, 1));
+      F4(av, i + 1, j, k, 2) = g->dx_inv * (2. * F4(av, i, j, k, 1) - 3. * F4(av, i - 1, j, k, 1) +
+                                            F4(av, i - 2, j, k, 1));
+    }
+  i02 = ix2 - 2;
+  for (int k = kz1; k <= kz2; k++)
+    for (int j = jy1; j <= jy2; j++)
+      for (int i = i01; i <= i02; i++)
+        F4(av, i, j, k, 2) = ax1 * (F4(av, i + 1, j, k, 1) - F4(av, i, j, k, 1));
+  if (g->der_ord == 4)
+    for (int k = kz1; k <= kz2; k++)
+      for (int j = jy1; j <= jy2; j++)
+        for (int i = i01; i <= i02; i++)
+          F4(av, i, j, k, 2) = F4(av, i, j, k, 2) +
+                               ax2 * (F4(av, i + 2, j, k, 1) - F4(av, i + 1, j, k, 1) +
+                                      F4(av, i, j, k, 1) - F4(av, i - 1, j, k, 1));
+  /* y: right face (yr_bd) first, as the reference orders them */
+  for (int k = kz1; k <= kz2; k++) {
+    int j = jy2;
+    double shy = 1.0 * g->dy_inv;
+    for (int i = ix1; i <= ix2; i++)
+      F4(av, i, j, k, 3) = shy * (2. * F4(av, i, j, k, 1) - 3. * F4(av, i, j - 1, k, 1) +
+                                  F4(av, i, j - 2, k, 1));
+    j = jy2 - 1;
+    shy = 1.0 * g->dy_inv;
+    for (int i = ix1; i <= ix2; i++)
+      F4(av, i, j, k, 3) = shy * (F4(av, i, j + 1, k, 1) - F4(av, i, j, k, 1));
+  }
+  j02 = jy2 - 2;
+  {
+    const int j = jy1;
+    const double shy = 1.0 * g->dy_inv;
+    for (int k = kz1; k <= kz2; k++)
+      for (int i = ix1; i <= ix2; i++)
+        F4(av, i, j, k, 3) = shy * (F4(av, i, j + 1, k, 1) - F4(av, i, j, k, 1));
+  }
+  j01 = jy1 + 1;
+  for (int k = kz1; k <= kz2; k++)
+    for (int j = j01; j <= j02; j++) {
+      const double shy = 1.0 * ay1;
+      for (int i = ix1; i <= ix2; i++)
+        F4(av, i, j, k, 3) = shy * (F4(av, i, j + 1, k, 1) - F4(av, i, j, k, 1));
+    }
+  if (g->der_ord == 4)
+    for (int k = kz1; k <= kz2; k++)
+      for (int j = j01; j <= j02; j++) {
+        const double shp = 1.0 * ay2, shm = 1.0 * ay2;
+        for (int i = ix1; i <= ix2; i++)
+          F4(av, i, j, k, 3) = F4(av, i, j, k, 3) +
+                               shp * (F4(av, i, j + 2, k, 1) - F4(av, i, j + 1, k, 1)) +
+                               shm * (F4(av, i, j, k, 1) - F4(av, i, j - 1, k, 1));
+      }
+  if (g->ndim == 2) return;
+  /* z: right face (zr_bd) writes the ghost plane kz2+1 of F first */
+  {
+    const int k = kz2;
+    const double shz = 1.0 * g->dz_inv;
+    for (int j = jy1; j <= jy2; j++)
+      for (int i = ix1; i <= ix2; i++) {
+        F4(av, i, j, k + 1, 1) = 2. * F4(av, i, j, k, 1) - F4(av, i, j, k - 1, 1);
+        F4(av, i, j, k, 4) = shz * (F4(av, i, j, k + 1, 1) - F4(av, i, j, k, 1));
+      }
+  }
+  k02 = kz2 - 1;
+  {
+    const int k = kz1;
+    const double shz = 1.0 * g->dz_inv;
+    for (int j = jy1; j <= jy2; j++)
+      for (int i = ix1; i <= ix2; i++) {
+        F4(av, i, j, k - 1, 1) = 2. * F4(av, i, j, k, 1) - F4(av, i, j, k + 1, 1);
+        F4(av, i, j, k, 4) = shz * (F4(av, i, j, k + 1, 1) - F4(av, i, j, k, 1));
+      }
+  }
+  k01 = kz1 + 1;
+  for (int k = k01; k <= k02; k++) {
+    const double shz = 1.0 * az1;
+    for (int j = jy1; j <= jy2; j++)
+      for (int i = ix1; i <= ix2; i++)
+        F4(av, i, j, k, 4) = shz * (F4(av, i, j, k + 1, 1) - F4(av, i, j, k, 1));
+  }
+  if (g->der_ord == 4)
+    for (int k = k01; k <= k02; k++) {
+      const double shp = 1.0 * az2, shm = 1.0 * az2;
+      for (int j = jy1; j <= jy2; j++)
+        for (int i = ix1; i <= ix2; i++)
+          F4(av, i, j, k, 4) = F4(av, i, j, k, 4) +
+                               shp * (F4(av, i, j, k + 2, 1) - F4(av, i, j, k + 1, 1)) +
+                               shm * (F4(av, i, j, k, 1) - F4(av, i, j, k - 1, 1));
+    }
+}
+
+void orc_env_amp_prepare(const orc_grid *g, const double *env, double *av) {
+  /* env_amp_prepare (env_evolve_in_time.f90:145-170): av(.,1) = |A^n|^2/2
+   * on the interior, then env_grad. */
+  for (int k = g->kz1; k <= g->kz2; k++)
+    for (int j = g->jy1; j <= g->jy2; j++)
+      for (int i = g->ix1; i <= g->ix2; i++)
+        F4(av, i, j, k, 1) = 0.5 * (F4(env, i, j, k, 1) * F4(env, i, j, k, 1) +
+                                    F4(env, i, j, k, 2) * F4(env, i, j, k, 2));
+  orc_env_grad(g, av);
+}
+
+void orc_env_fields_average(const orc_grid *g, const double *env, double *av) {
+  /* env_fields_average (env_evolve_in_time.f90:117-143): av(.,1) =
+   * |(A^{n+1} + A^n)/2|^2/2 = ((A1+A3)^2 + (A2+A4)^2)/8 on the interior,
+   * then env_grad. */
+  const double frac = 1.0 / 8.;
+  for (int k = g->kz1; k <= g->kz2; k++)
+    for (int j = g->jy1; j <= g->jy2; j++)
+      for (int i = g->ix1; i <= g->ix2; i++) {
+        const double r = F4(env, i, j, k, 1) + F4(env, i, j, k, 3);
+        const double m = F4(env, i, j, k, 2) + F4(env, i, j, k, 4);
+        F4(av, i, j, k, 1) = frac * (r * r + m * m);
+      }
+  orc_env_grad(g, av);
+}
+
+void orc_env_bds(const orc_grid *g, double *ef, int ptrght, int ptlft, int c1, int c2) {
+  /* env_bds (grid_fields.f90:864-1067), open boundaries (ibx = iby = ibz =
+   * 0), unstretched (alpha = 1): x ghosts copy the edge value (the one-point
+   * stencil COEFF_0); y and z ghosts take the quadratic extrapolation
+   * 3 f(1) - 3 f(2) + f(3) written as alpha*(f(3)-f(2)) - 2 f(2) + 3 f(1),
+   * the same value in every ghost layer.  The y pass covers the x ghosts
+   * just written, the z pass the x and y ghosts. */
+  int i1 = g->ix1, i2 = g->ix2, j1 = g->jy1, j2 = g->jy2, k1 = g->kz1, k2 = g->kz2;
+  for (int ic = c1; ic <= c2; ic++)
+    for (int k = k1; k <= k2; k++)
+      for (int j = j1; j <= j2; j++)
+        for (int point = ptlft; point >= 1; point--) F4(ef, i1 - point, j, k, ic) = F4(ef, i1, j, k, ic);
+  i1 = i1 - ptlft;
+  for (int ic = c1; ic <= c2; ic++)
+    for (int k = k1; k <= k2; k++)
+      for (int j = j1; j <= j2; j++)
+        for (int point = 1; point <= ptrght; point++) F4(ef, i2 + point, j, k, ic) = F4(ef, i2, j, k, ic);
+  i2 = i2 + ptrght;
+  const double alpha = 1.0 / 1.0;
+  for (int j = j1 - ptlft; j <= j1 - 1; j++)
+    for (int ic = c1; ic <= c2; ic++)
+      for (int k = k1; k <= k2; k++)
+        for (int i = i1; i <= i2; i++)
+          F4(ef, i, j, k, ic) = alpha * (F4(ef, i, j1 + 2, k, ic) - F4(ef, i, j1 + 1, k, ic)) -
+                                2 * F4(ef, i, j1 + 1, k, ic) + 3 * F4(ef, i, j1, k, ic);
+  j1 = j1 - ptlft;
+  for (int j = j2 + 1; j <= j2 + ptrght; j++)
+    for (int ic = c1; ic <= c2; ic++)
+      for (int k = k1; k <= k2; k++)
+        for (int i = i1; i <= i2; i++)
+          F4(ef, i, j, k, ic) = alpha * (F4(ef, i, j2 - 2, k, ic) - F4(ef, i, j2 - 1, k, ic)) -
+                                2 * F4(ef, i, j2 - 1, k, ic) + 3 * F4(ef, i, j2, k, ic);
+  j2 = j2 + ptrght;
+  if (g->ndim < 3) return;
+  for (int k = k1 - ptlft; k <= k1 - 1; k++)
+    for (int ic = c1; ic <= c2; ic++)
+      for (int j = j1; j <= j2; j++)
+        for (int i = i1; i <= i2; i++)
+          F4(ef, i, j, k, ic) = alpha * (F4(ef, i, j, k1 + 2, ic) - F4(ef, i, j, k1 + 1, ic)) -
+                                2 * F4(ef, i, j, k1 + 1, ic) + 3 * F4(ef, i, j, k1, ic);
+  for (int k = k2 + 1; k <= k2 + ptrght; k++)
+    for (int ic = c1; ic <= c2; ic++)
+      for (int j = j1; j <= j2; j++)
+        for (int i = i1; i <= i2; i++)
+          F4(ef, i, j, k, ic) = alpha * (F4(ef, i, j, k2 - 2, ic) - F4(ef, i, j, k2 - 1, ic)) -
+                                2 * F4(ef, i, j, k2 - 1, ic) + 3 * F4(ef, i, j, k2, ic);
+}
+
+static void orc_pp_lapl(const orc_grid *g, const double *av, double *source, int ic1, int ic2) {
+  /* pp_lapl (grid_fields.f90:326-424), unstretched: the transverse
+   * Laplacian added to source, y then z, each with the der_ord 4
+   * wide-stencil term after the compact one. */
+  double cf[2] = {1., 0.};
+  if (g->der_ord == 4) {
+    cf[0] = 4. / 3.;
+    cf[1] = -1. / 12.;
+  }
+  const double dy2_inv = g->dy_inv * g->dy_inv, dz2_inv = g->dz_inv * g->dz_inv;
+  const double dy4_inv[2] = {cf[0] * dy2_inv, cf[1] * dy2_inv};
+  const double dz4_inv[2] = {cf[0] * dz2_inv, cf[1] * dz2_inv};
+  for (int ic = ic1; ic <= ic2; ic++)
+    for (int k = g->kz1; k <= g->kz2; k++)
+      for (int j = g->jy1; j <= g->jy2; j++) {
+        const double shy = dy4_inv[0] * 1.0, sphy = 1.0, smhy = 1.0;
+        for (int i = g->ix1; i <= g->ix2; i++)
+          F4(source, i, j, k, ic) =
+              F4(source, i, j, k, ic) +
+              shy * (sphy * (F4(av, i, j + 1, k, ic) - F4(av, i, j, k, ic)) -
+                     smhy * (F4(av, i, j, k, ic) - F4(av, i, j - 1, k, ic)));
+      }
+  if (g->der_ord == 4)
+    for (int ic = ic1; ic <= ic2; ic++)
+      for (int k = g->kz1; k <= g->kz2; k++)
+        for (int j = g->jy1; j <= g->jy2; j++) {
+          const double shy = dy4_inv[1] * 1.0, sphy = 1.0, smhy = 1.0;
+          for (int i = g->ix1; i <= g->ix2; i++)
+            F4(source, i, j, k, ic) =
+                F4(source, i, j, k, ic) +
+                shy * (sphy * (F4(av, i, j + 2, k, ic) - F4(av, i, j, k, ic)) -
+                       smhy * (F4(av, i, j, k, ic) - F4(av, i, j - 2, k, ic)));
+        }
+  if (g->ndim < 3) return;
+  for (int ic = ic1; ic <= ic2; ic++)
+    for (int k = g->kz1; k <= g->kz2; k++) {
+      const double shz = dz4_inv[0] * 1.0, sphz = 1.0, smhz = 1.0;
+      for (int j = g->jy1; j <= g->jy2; j++)
+        for (int i = g->ix1; i <= g->ix2; i++)
+          F4(source, i, j, k, ic) =
+              F4(source, i, j, k, ic) +
+              shz * (sphz * (F4(av, i, j, k + 1, ic) - F4(av, i, j, k, ic)) -
+                     smhz * (F4(av, i, j, k, ic) - F4(av, i, j, k - 1, ic)));
+    }
+  if (g->der_ord == 4)
+    for (int ic = ic1; ic <= ic2; ic++)
+      for (int k = g->kz1; k <= g->kz2; k++) {
+        const double shz = dz4_inv[1] * 1.0, sphz = 1.0, smhz = 1.0;
+        for (int j = g->jy1; j <= g->jy2; j++)
+          for (int i = g->ix1; i <= g->ix2; i++)
+            F4(source, i, j, k, ic) =
+                F4(source, i, j, k, ic) +
+                shz * (sphz * (F4(av, i, j, k + 2, ic) - F4(av, i, j, k, ic)) -
+                       smhz * (F4(av, i, j, k, ic) - F4(av, i, j, k - 2, ic)));
+      }
+}
+
+void orc_potential_lapl(const orc_grid *g, const orc_env *e, const double *apf, double *curr,
+                        int ic1, int ic2) {
+  /* potential_lapl (grid_fields.f90:1356-1403): curr += Laplacian(apf),
+   * x part with cf = (1 - 4 hord_der2, hord_der2) for der_ord > 2, then
+   * pp_lapl for the transverse part. */
+  const double dx2 = g->dx_inv * g->dx_inv;
+  double cf[2] = {1., 0.0};
+  if (g->der_ord > 2) {
+    cf[0] = 1. - 4. * e->hord_der2;
+    cf[1] = e->hord_der2;
+  }
+  const double dx4[2] = {cf[0] * dx2, cf[1] * dx2};
+  for (int ic = ic1; ic <= ic2; ic++)
+    for (int k = g->kz1; k <= g->kz2; k++)
+      for (int j = g->jy1; j <= g->jy2; j++)
+        for (int i = g->ix1; i <= g->ix2; i++)
+          F4(curr, i, j, k, ic) =
+              F4(curr, i, j, k, ic) +
+              dx4[0] * (F4(apf, i + 1, j, k, ic) + F4(apf, i - 1, j, k, ic) - 2. * F4(apf, i, j, k, ic));
+  if (g->der_ord > 2)
+    for (int ic = ic1; ic <= ic2; ic++)
+      for (int k = g->kz1; k <= g->kz2; k++)
+        for (int j = g->jy1; j <= g->jy2; j++)
+          for (int i = g->ix1; i <= g->ix2; i++)
+            F4(curr, i, j, k, ic) =
+                F4(curr, i, j, k, ic) +
+                dx4[1] * (F4(apf, i + 2, j, k, ic) + F4(apf, i - 2, j, k, ic) - 2. * F4(apf, i, j, k, ic));
+  if (g->ndim > 1) orc_pp_lapl(g, apf, curr, ic1, ic2);
+}
+
+void orc_env_maxw_solve(const orc_grid *g, const orc_env *e, double *curr, double *evf,
+                        double dtl) {
+  /* env_maxw_solve (grid_fields.f90:588-694): on entry curr(1:2) =
+   * -ompe chi A^n; S(A) = curr + Laplacian A^n + the centred 2 om0 D_x term
+   * (first_ader); F = dt^2 S + 2 A^n - A^{n-1} -/+ kfact A^{n-1}_{I/R};
+   * A^{n+1} = (F -/+ kfact F_{I/R})/(1 + kfact^2), A^{n-1} <- A^n.
+   * evf(1:2) = A^n, evf(3:4) = A^{n-1}. */
+  const double dt2 = dtl * dtl, om0 = e->om0;
+  const double kfact = om0 * dtl, k2_fact = 1. / (1. + kfact * kfact), skfact = om0;
+  const double dx1_inv = skfact * g->dx_inv;
+  double aph_opt[2] = {1., 0.};
+  if (g->der_ord == 3) {
+    aph_opt[0] = dx1_inv * e->opt_der1;
+    aph_opt[1] = dx1_inv * 0.5 * (1. - e->opt_der1);
+  }
+  orc_potential_lapl(g, e, evf, curr, 1, 2);
+  /* first_ader */
+  for (int k = g->kz1; k <= g->kz2; k++)
+    for (int j = g->jy1; j <= g->jy2; j++)
+      for (int i = g->ix1; i <= g->ix2; i++) {
+        if (g->der_ord < 3) {
+          F4(curr, i, j, k, 1) = F4(curr, i, j, k, 1) -
+                                 dx1_inv * (F4(evf, i + 1, j, k, 2) - F4(evf, i - 1, j, k, 2));
+          F4(curr, i, j, k, 2) = F4(curr, i, j, k, 2) +
+                                 dx1_inv * (F4(evf, i + 1, j, k, 1) - F4(evf, i - 1, j, k, 1));
+        } else {
+          F4(curr, i, j, k, 1) = F4(curr, i, j, k, 1) -
+                                 aph_opt[0] * (F4(evf, i + 1, j, k, 2) - F4(evf, i - 1, j, k, 2)) -
+                                 aph_opt[1] * (F4(evf, i + 2, j, k, 2) - F4(evf, i - 2, j, k, 2));
+          F4(curr, i, j, k, 2) = F4(curr, i, j, k, 2) +
+                                 aph_opt[0] * (F4(evf, i + 1, j, k, 1) - F4(evf, i - 1, j, k, 1)) +
+                                 aph_opt[1] * (F4(evf, i + 2, j, k, 1) - F4(evf, i - 2, j, k, 1));
+        }
+      }
+  for (int k = g->kz1; k <= g->kz2; k++)
+    for (int j = g->jy1; j <= g->jy2; j++)
+      for (int i = g->ix1; i <= g->ix2; i++) {
+        F4(curr, i, j, k, 1) = dt2 * F4(curr, i, j, k, 1) + 2. * F4(evf, i, j, k, 1) -
+                               F4(evf, i, j, k, 3) + kfact * F4(evf, i, j, k, 4);
+        F4(curr, i, j, k, 2) = dt2 * F4(curr, i, j, k, 2) + 2. * F4(evf, i, j, k, 2) -
+                               F4(evf, i, j, k, 4) - kfact * F4(evf, i, j, k, 3);
+      }
+  for (int k = g->kz1; k <= g->kz2; k++)
+    for (int j = g->jy1; j <= g->jy2; j++)
+      for (int i = g->ix1; i <= g->ix2; i++) {
+        F4(evf, i, j, k, 3) = F4(evf, i, j, k, 1);
+        F4(evf, i, j, k, 4) = F4(evf, i, j, k, 2);
+        F4(evf, i, j, k, 1) = k2_fact * (F4(curr, i, j, k, 1) - kfact * F4(curr, i, j, k, 2));
+        F4(evf, i, j, k, 2) = k2_fact * (F4(curr, i, j, k, 2) + kfact * F4(curr, i, j, k, 1));
+      }
+}
+
+void orc_advance_lpf_envelope(const orc_grid *g, const orc_env *e, double *curr, double *evf) {
+  /* advance_lpf_envelope (curr_and_fields_util.f90:262-299) outside the
+   * comoving frame (ib = 2: env_maxw_solve): env_bds(evf, 2, 2) on A^n,
+   * curr(1:2) = -ompe curr(3) A^n on the interior, the solve. */
+  orc_env_bds(g, evf, 2, 2, 1, 2);
+  for (int k = g->kz1; k <= g->kz2; k++)
+    for (int j = g->jy1; j <= g->jy2; j++)
+      for (int i = g->ix1; i <= g->ix2; i++) {
+        F4(curr, i, j, k, 1) = -g->ompe * F4(curr, i, j, k, 3) * F4(evf, i, j, k, 1);
+        F4(curr, i, j, k, 2) = -g->ompe * F4(curr, i, j, k, 3) * F4(evf, i, j, k, 2);
+      }
+  orc_env_maxw_solve(g, e, curr, evf, g->dt);
+}
+
+void orc_den_zyxbd(const orc_grid *g, double *rho, int ik) {
+  /* den_zyxbd (mpi_curr_interface.f90:646-720), open boundaries, one rank:
+   * per face the first ghost layer is added to the second interior layer,
+   * which is then copied onto the first (z, then y, then x). */
+  const int i1 = g->ix1, i2 = g->ix2, j1 = g->jy1, j2 = g->jy2, k1 = g->kz1, k2 = g->kz2;
+  if (g->ndim > 2) {
+    for (int iy = j1; iy <= j2; iy++)
+      for (int ix = i1; ix <= i2; ix++) {
+        F4(rho, ix, iy, k1 + 1, ik) = F4(rho, ix, iy, k1 + 1, ik) + F4(rho, ix, iy, k1 - 1, ik);
+        F4(rho, ix, iy, k1, ik) = F4(rho, ix, iy, k1 + 1, ik);
+      }
+    for (int iy = j1; iy <= j2; iy++)
+      for (int ix = i1; ix <= i2; ix++) {
+        F4(rho, ix, iy, k2 - 1, ik) = F4(rho, ix, iy, k2 - 1, ik) + F4(rho, ix, iy, k2 + 1, ik);
+        F4(rho, ix, iy, k2, ik) = F4(rho, ix, iy, k2 - 1, ik);
+      }
+  }
+  for (int k = k1; k <= k2; k++)
+    for (int i = i1; i <= i2; i++) {
+      F4(rho, i, j1 + 1, k, ik) = F4(rho, i, j1 + 1, k, ik) + F4(rho, i, j1 - 1, k, ik);
+      F4(rho, i, j1, k, ik) = F4(rho, i, j1 + 1, k, ik);
+    }
+  for (int k = k1; k <= k2; k++)
+    for (int i = i1; i <= i2; i++) {
+      F4(rho, i, j2 - 1, k, ik) = F4(rho, i, j2 - 1, k, ik) + F4(rho, i, j2 + 1, k, ik);
+      F4(rho, i, j2, k, ik) = F4(rho, i, j2 - 1, k, ik);
+    }
+  for (int k = k1; k <= k2; k++)
+    for (int j = j1; j <= j2; j++) {
+      F4(rho, i1 + 1, j, k, ik) = F4(rho, i1 + 1, j, k, ik) + F4(rho, i1 - 1, j, k, ik);
+      F4(rho, i1, j, k, ik) = F4(rho, i1 + 1, j, k, ik);
+    }
+  for (int k = k1; k <= k2; k++)
+    for (int j = j1; j <= j2; j++) {
+      F4(rho, i2 - 1, j, k, ik) = F4(rho, i2 - 1, j, k, ik) + F4(rho, i2 + 1, j, k, ik);
+      F4(rho, i2, j, k, ik) = F4(rho, i2 - 1, j, k, ik);
+    }
+}
+
+int64_t orc_env_lpf2_evolve(const orc_grid *g, const orc_env *e, int64_t np, double *part,
+                            int64_t ld, double *ebfp, double *ebf, double *env, double *jc) {
+  /* env_lpf2_evolve (env_evolve_in_time.f90:196-361) for one species, one
+   * rank, no ionization, one colour, no fluid: jc is the reference's
+   * jc(n1p, n2p, n3p, curr_ndim + 1), reused as it reuses it (|A|^2/2 and
+   * its gradient, the density source, the solver's scratch, the current).
+   * Returns the new particle count. */
+  const int D = g->curr_ndim;
+  const int64_t nn = (int64_t)g->n1p * g->n2p * g->n3p;
+  double *c1 = jc, *c3 = jc + 2 * nn;
+  memset(jc, 0, sizeof(double) * nn * (D + 1));
+  orc_env_amp_prepare(g, env, jc);
+  memset(ebfp, 0, sizeof(double) * ld * (2 * D + 1));
+  orc_env_acc(g, np, part, ld, ebf, jc, g->dt, ebfp);
+  orc_env_momenta(g, np, part, ld, ebfp);
+  memset(c1, 0, sizeof(double) * nn);
+  orc_env_density(g, np, ebfp, ld, c1);
+  orc_den_zyxbd(g, jc, 1);
+  memcpy(c3, c1, sizeof(double) * nn);
+  orc_advance_lpf_envelope(g, e, jc, env);
+  orc_env_fields_average(g, env, jc);
+  orc_env_grad_interp(g, np, part, ld, jc, ebfp);
+  orc_env_positions(g, np, part, ld, ebfp);
+  np = orc_part_dist(g, np, part, ld, ebfp);
+  memset(jc, 0, sizeof(double) * nn * (D + 1));
+  orc_esirkepov(g, np, part, ld, ebfp, jc);
+  orc_curr_collect(g, jc);
+  orc_advance_fields(g, ebf, jc);
+  return np;
+}

@@ -185,6 +185,38 @@ void orc_env_grad_interp(const orc_grid *g, int64_t np, const double *part, int6
                          const double *av, double *ebfp);
 void orc_env_positions(const orc_grid *g, int64_t np, double *part, int64_t ld, double *ebfp);
 
+/* The envelope model's grid side (one rank, open faces, unstretched). */
+typedef struct orc_env {
+  double om0;       /* oml: the laser frequency in the code's units */
+  double hord_der2; /* set_field_param: der_ord 3 -(1-nu^2)/12, der_ord 4 -1/12 */
+  double opt_der1;  /* set_field_param: der_ord 3 (4-nu^2)/3, else 1 */
+} orc_env;
+/* set_field_param's envelope constants (grid_field_param.f90:36-99). */
+void orc_env_setup(const orc_grid *g, double cfl, double om0, orc_env *e);
+/* env_grad (grid_fields.f90:426-586): av(.,1) = F -> av(.,2:1+D) = grad F. */
+void orc_env_grad(const orc_grid *g, double *av);
+/* env_amp_prepare / env_fields_average (env_evolve_in_time.f90:117-170):
+ * av(.,1) = |A^n|^2/2, resp. |(A^{n+1}+A^n)/2|^2/2, then env_grad.
+ * env(n1p, n2p, n3p, 4) = A_R^n, A_I^n, A_R^{n-1}, A_I^{n-1}. */
+void orc_env_amp_prepare(const orc_grid *g, const double *env, double *av);
+void orc_env_fields_average(const orc_grid *g, const double *env, double *av);
+/* env_bds (grid_fields.f90:864-1067), open faces: components c1..c2. */
+void orc_env_bds(const orc_grid *g, double *ef, int ptrght, int ptlft, int c1, int c2);
+/* potential_lapl (grid_fields.f90:1356-1403) with pp_lapl (:326-424). */
+void orc_potential_lapl(const orc_grid *g, const orc_env *e, const double *apf, double *curr,
+                        int ic1, int ic2);
+/* env_maxw_solve (grid_fields.f90:588-694). */
+void orc_env_maxw_solve(const orc_grid *g, const orc_env *e, double *curr, double *evf,
+                        double dtl);
+/* advance_lpf_envelope (curr_and_fields_util.f90:262-299), ib = 2. */
+void orc_advance_lpf_envelope(const orc_grid *g, const orc_env *e, double *curr, double *evf);
+/* den_zyxbd (mpi_curr_interface.f90:646-720), open faces, component ik. */
+void orc_den_zyxbd(const orc_grid *g, double *rho, int ik);
+/* env_lpf2_evolve (env_evolve_in_time.f90:196-361), one species; jc has
+ * curr_ndim + 1 components.  Returns the new particle count. */
+int64_t orc_env_lpf2_evolve(const orc_grid *g, const orc_env *e, int64_t np, double *part,
+                            int64_t ld, double *ebfp, double *ebf, double *env, double *jc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,16 @@ class OrcWindow(C.Structure):
     _fields_ = [("xlapse", C.c_double), ("wi1", C.c_int)]
 
 
+class OrcEnv(C.Structure):
+    """The envelope model's constants (om0 and set_field_param's hord_der2,
+    opt_der1)."""
+
+    _fields_ = [("om0", C.c_double), ("hord_der2", C.c_double), ("opt_der1", C.c_double)]
+
+
+_E = C.POINTER(OrcEnv)
+
+
 class Oracle:
     """The C restatement of the LP step on the reference layouts."""
 
@@ -99,12 +109,22 @@ class Oracle:
             "orc_env_density": [_G, _I64, _D, _I64, _D],
             "orc_env_grad_interp": [_G, _I64, _D, _I64, _D, _D],
             "orc_env_positions": [_G, _I64, _D, _I64, _D],
+            "orc_env_setup": [_G, C.c_double, C.c_double, _E],
+            "orc_env_grad": [_G, _D],
+            "orc_env_amp_prepare": [_G, _D, _D],
+            "orc_env_fields_average": [_G, _D, _D],
+            "orc_env_bds": [_G, _D, C.c_int, C.c_int, C.c_int, C.c_int],
+            "orc_potential_lapl": [_G, _E, _D, _D, C.c_int, C.c_int],
+            "orc_env_maxw_solve": [_G, _E, _D, _D, C.c_double],
+            "orc_advance_lpf_envelope": [_G, _E, _D, _D],
+            "orc_den_zyxbd": [_G, _D, C.c_int],
+            "orc_env_lpf2_evolve": [_G, _E, _I64, _D, _I64, _D, _D, _D, _D],
         }
         for name, at in sig.items():
             fn = getattr(L, name)
             fn.argtypes = at
             fn.restype = _I64 if name in ("orc_part_dist", "orc_particle_stage", "orc_lpf2_evolve",
-                                          "orc_esirkepov_skipped") else None
+                                          "orc_esirkepov_skipped", "orc_env_lpf2_evolve") else None
             if name == "orc_window_xshift":
                 fn.restype = C.c_int
             if name in ("orc_inject_select", "orc_add_particles"):
@@ -121,6 +141,12 @@ class Oracle:
         g.lorentz_fact = cfg.lorentz_fact[0] if lorentz_fact is None else lorentz_fact
         L.orc_grid_setup(C.byref(g), cfg.cfl)
         self.g = g
+        self.env = OrcEnv()
+        L.orc_env_setup(C.byref(g), cfg.cfl, getattr(cfg, "oml", 0.0), C.byref(self.env))
+
+    def set_om0(self, om0: float):
+        """The envelope's laser frequency (oml) for the envelope solver."""
+        self.L.orc_env_setup(C.byref(self.g), self.cfg.cfl, om0, C.byref(self.env))
 
     # array factories in the reference layouts
     def zeros_fields(self):
@@ -283,6 +309,48 @@ class Oracle:
         self.esirkepov(p, e, jc, n)
         return p, den
 
+    # envelope model, grid side: env(n1p, n2p, n3p, 4) = A_R^n, A_I^n,
+    # A_R^{n-1}, A_I^{n-1}; jc in envelope mode has curr_ndim + 1 components
+    def zeros_env_fields(self):
+        c = self.cfg
+        return np.zeros((c.n1p, c.n2p, c.n3p, 4), dtype=np.float64, order="F")
+
+    def env_grad(self, av):
+        self.L.orc_env_grad(C.byref(self.g), _p(_f(av)))
+
+    def env_amp_prepare(self, env, av):
+        self.L.orc_env_amp_prepare(C.byref(self.g), _p(_f(env)), _p(_f(av)))
+
+    def env_fields_average(self, env, av):
+        self.L.orc_env_fields_average(C.byref(self.g), _p(_f(env)), _p(_f(av)))
+
+    def env_bds(self, ef, ptrght=2, ptlft=2, c1=1, c2=2):
+        self.L.orc_env_bds(C.byref(self.g), _p(_f(ef)), ptrght, ptlft, c1, c2)
+
+    def potential_lapl(self, apf, curr, ic1=1, ic2=2):
+        self.L.orc_potential_lapl(C.byref(self.g), C.byref(self.env), _p(_f(apf)), _p(_f(curr)),
+                                  ic1, ic2)
+
+    def env_maxw_solve(self, curr, env, dtl=None):
+        self.L.orc_env_maxw_solve(C.byref(self.g), C.byref(self.env), _p(_f(curr)), _p(_f(env)),
+                                  self.cfg.dt if dtl is None else dtl)
+
+    def advance_lpf_envelope(self, curr, env):
+        self.L.orc_advance_lpf_envelope(C.byref(self.g), C.byref(self.env), _p(_f(curr)),
+                                        _p(_f(env)))
+
+    def den_zyxbd(self, rho, ik=1):
+        self.L.orc_den_zyxbd(C.byref(self.g), _p(_f(rho)), ik)
+
+    def env_lpf2_evolve(self, part, ebf, env, jc):
+        """One env_lpf2_evolve; returns the surviving particles (new array)."""
+        p = part.copy(order="F")
+        ebfp = self.zeros_stage(p.shape[0])
+        n = self.L.orc_env_lpf2_evolve(C.byref(self.g), C.byref(self.env), p.shape[0], _p(p),
+                                       self._ld(p), _p(ebfp), _p(_f(ebf)), _p(_f(env)),
+                                       _p(_f(jc)))
+        return np.asfortranarray(p[:n])
+
     def grid_charge(self, part):
         rho = np.zeros((self.cfg.n1p, self.cfg.n2p, self.cfg.n3p), dtype=np.float64, order="F")
         self.L.orc_grid_charge(C.byref(self.g), part.shape[0], _p(_f(part)), self._ld(part), _p(rho))
@@ -396,6 +464,19 @@ class RefHarness:
         L.ref_se_coeff.restype = None
         out = np.zeros(2)
         L.ref_se_coeff(ndim, der_ord, cfl, yx_rat, _p(out))
+        return out
+
+    @staticmethod
+    def field_param(ndim, der_ord, cfl, yx_rat, lib: Path | None = None):
+        """set_field_param's (hord_der2, opt_der1) (grid_field_param.f90:36-99)."""
+        p = Path(lib) if lib else REF_LIB
+        if not p.exists():
+            raise ImportError(f"{p} missing: run `make -C oracle ref` where the reference exists")
+        L = C.CDLL(str(p))
+        L.ref_field_param.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _D]
+        L.ref_field_param.restype = None
+        out = np.zeros(2)
+        L.ref_field_param(ndim, der_ord, cfl, yx_rat, _p(out))
         return out
 
     def gather_push_deposit(self, part, ebf, nsteps=1):

@@ -239,6 +239,21 @@ contains
     c_out = se_coeff
   end subroutine
 
+  ! set_field_param's envelope-solver constants besides se_coeff:
+  ! c_out = (hord_der2, opt_der1), read after the call for der_ord 3 or 4
+  subroutine ref_field_param(c_ndim, c_der_ord, c_cfl, c_yx_rat, c_out) bind(C, name='ref_field_param')
+    integer(c_int), value :: c_ndim, c_der_ord
+    real(c_double), value :: c_cfl, c_yx_rat
+    real(c_double), intent(out) :: c_out(2)
+    real(c_double) :: se(2)
+
+    hord_der2 = 0.0
+    opt_der1 = 1.0
+    call ref_se_coeff(c_ndim, c_der_ord, c_cfl, c_yx_rat, se)
+    c_out(1) = hord_der2
+    c_out(2) = opt_der1
+  end subroutine
+
   ! set_env_acc(ebf, av, spec(1), ebfp, np, dt): ebfp_out(1:2D+1) as it exits
   subroutine ref_env_acc(np, c_part, ld, c_ebf, c_av, dt_step, c_ebfp) bind(C, name='ref_env_acc')
     integer(c_int64_t), value :: np, ld

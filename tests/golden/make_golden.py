@@ -1,6 +1,7 @@
 """Writes tests/golden/ref_vectors_*.npz from the reference's own routines.
 
-Also writes tests/golden/ref_se_coeff.json: set_field_param's se_coeff
+Also writes tests/golden/ref_se_coeff.json: set_field_param's se_coeff and
+its envelope-solver constants hord_der2 and opt_der1
 (grid_field_param.f90:36-99) for der_ord 2, 3, 4 over a set of (ndim, cfl,
 dy/dx).
 
@@ -78,8 +79,10 @@ def make_se_coeff():
     rows = []
     for nd, der, cfl, rat in SE_CASES:
         c = RefHarness.se_coeff(nd, der, cfl, rat)
+        e = RefHarness.field_param(nd, der, cfl, rat)
         rows.append({"ndim": nd, "der_ord": der, "cfl": cfl, "yx_rat": rat,
-                     "se_coeff": [float(c[0]).hex(), float(c[1]).hex()]})
+                     "se_coeff": [float(c[0]).hex(), float(c[1]).hex()],
+                     "hord_der2_opt_der1": [float(e[0]).hex(), float(e[1]).hex()]})
     (HERE / "ref_se_coeff.json").write_text(json.dumps(rows, indent=1) + "\n")
     print("se_coeff", len(rows), "cases")
 

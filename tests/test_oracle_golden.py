@@ -54,9 +54,9 @@ def test_oracle_reproduces_reference_vectors(name):
 
 
 def test_se_coeff_reproduces_reference_table():
-    """se_coeff of the oracle against set_field_param's own values
-    (grid_field_param.f90:36-99), written by make_golden.py from the
-    reference compiled here: der_ord 2, 3 and 4 over cfl and dy/dx."""
+    """se_coeff, hord_der2 and opt_der1 of the oracle against set_field_param's
+    own values (grid_field_param.f90:36-99), written by make_golden.py from
+    the reference compiled here: der_ord 2, 3 and 4 over cfl and dy/dx."""
     import json
 
     rows = json.loads((GOLDEN / "ref_se_coeff.json").read_text())
@@ -67,8 +67,15 @@ def test_se_coeff_reproduces_reference_table():
         cfg = LPConfig(ndim=nd, nx=16, ny=8, k0=20.0, yx_rat=rat, cfl=r["cfl"],
                        der_ord=r["der_ord"], **kw)
         want = [float.fromhex(v) for v in r["se_coeff"]]
-        got = list(Oracle(cfg).g.se_coeff[:])
+        orc = Oracle(cfg)
+        got = list(orc.g.se_coeff[:])
         assert got == want, (r, got)
+        # the envelope solver's constants, where the reference sets them
+        if r["der_ord"] > 2:
+            hw, ow = (float.fromhex(v) for v in r["hord_der2_opt_der1"])
+            assert orc.env.hord_der2 == hw, (r, orc.env.hord_der2)
+            if r["der_ord"] == 3:
+                assert orc.env.opt_der1 == ow, (r, orc.env.opt_der1)
     # der_ord 4 is the fourth-order staggered pair (9/8, -1/24) whatever cfl
     d4 = {tuple(float.fromhex(v) for v in r["se_coeff"]) for r in rows if r["der_ord"] == 4}
     assert d4 == {(1.125, -1.0 / 24.0)}
