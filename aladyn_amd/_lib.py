@@ -10,7 +10,9 @@ import ctypes as C
 import os
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libaladyn_gpu.so"
+# ALADYN_LIB names another build of the same library (a variant under
+# comparison); it must exist like the default one
+LIB_PATH = Path(os.environ.get("ALADYN_LIB") or Path(__file__).resolve().parent / "libaladyn_gpu.so")
 
 PREC_F64 = 64
 PREC_F32 = 32

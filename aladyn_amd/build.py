@@ -43,17 +43,21 @@ def flags() -> list[str]:
     ]
 
 
-def build(verbose: bool = False, jobs: int = 4) -> Path:
-    """Compiles every source to an object (in parallel) and links the library."""
+def build(verbose: bool = False, jobs: int = 4, csrc: Path | None = None,
+          out: Path | None = None) -> Path:
+    """Compiles every source to an object (in parallel) and links the library
+    (by default the package's sources into aladyn_amd/libaladyn_gpu.so)."""
     cc = hipcc()
-    objdir = PKG / "_build"
+    src_dir = Path(csrc) if csrc else CSRC
+    lib = Path(out) if out else LIB
+    objdir = lib.parent / "_build"
     objdir.mkdir(exist_ok=True)
     procs = []
     objs = []
     for src in SOURCES:
         obj = objdir / (src + ".o")
         objs.append(obj)
-        cmd = [cc, *flags(), "-c", str(CSRC / src), "-o", str(obj)]
+        cmd = [cc, *flags(), "-c", str(src_dir / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -70,13 +74,13 @@ def build(verbose: bool = False, jobs: int = 4) -> Path:
     if failed:
         msg = "\n".join(f"--- {s}\n{t}" for s, t in failed)
         raise RuntimeError(f"HIP compilation failed:\n{msg}")
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = lib.with_suffix(".so.tmp")
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
