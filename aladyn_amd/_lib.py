@@ -72,6 +72,7 @@ SIGNATURES = {
     "aladyn_energy_momenta": [_P, C.c_int32, _D, _D],
     "aladyn_field_energy": [_P, _D],
     "aladyn_grid_charge": [_P, C.c_int32, _D],
+    "aladyn_grid_charge_noalloc": [_P, C.c_int32, _D],
     "aladyn_continuity_residual": [_P, _D],
     "aladyn_set_diagnostics": [_P, C.c_int32],
     "aladyn_synchronize": [_P],

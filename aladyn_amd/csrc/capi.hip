@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -137,6 +138,7 @@ static int fail(aladyn_ctx* c, int code, const char* fmt, ...) {
   } while (0)
 
 static bool f64(const aladyn_ctx* c) { return c->cfg.precision == ALADYN_PREC_F64; }
+static int noalloc_scratch_init(aladyn_ctx* c);
 
 // ----------------------------------------------------------------- set-up
 static int setup_grid(aladyn_ctx* c) {
@@ -232,6 +234,7 @@ extern "C" int aladyn_create(const aladyn_config* cfg, aladyn_ctx** out) {
   setup_grid(c);
   *out = c;
   HIPCK(c, hipGetDevice(&c->dev));
+  if (int r = noalloc_scratch_init(c)) return r;
   HIPCK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   if (c->tpart.create() || c->tfield.create())
     return fail(c, ALADYN_EHIP, "event creation failed");
@@ -818,6 +821,57 @@ extern "C" int aladyn_grid_charge(aladyn_ctx* c, int32_t ic, double* rho) {
   HIPCK(c, hipMemcpyAsync(rho, c->rho1, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCK(c, hipStreamSynchronize(c->stream));
   c->have_resid = false;  // rho1 was reused
+  return ALADYN_OK;
+}
+
+// ------------------------------------------- allocation-free grid charge
+// Device staging of aladyn_grid_charge_noalloc: one array per GPU, in the
+// code object's data (no allocation call ever makes it).  Its host-side name
+// is not a device address: host code uses only the address that
+// hipGetSymbolAddress returns for the current device (noalloc_scratch),
+// looked up once per device at aladyn_create.  Every call on one GPU shares
+// the array: calls that overlap on one GPU (several host threads or
+// streams) race, and it keeps the last call's contents.
+static constexpr int64_t kNoAllocCells = int64_t(1) << 23;  // 64 MiB of FP64
+static __device__ __align__(256) double scratch[kNoAllocCells];
+static constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Looks up this device's copy of the array (the device must be current).
+static int noalloc_scratch_init(aladyn_ctx* c) {
+  if (c->dev < 0 || c->dev >= kMaxDevices)
+    return fail(c, ALADYN_EINVAL, "device %d beyond the %d the library tracks", c->dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[c->dev]) return 0;
+  void* p = nullptr;
+  HIPCK(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[c->dev] = static_cast<double*>(p);
+  return 0;
+}
+
+static double* noalloc_scratch(const aladyn_ctx* c) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[c->dev];
+}
+
+extern "C" int aladyn_grid_charge_noalloc(aladyn_ctx* c, int32_t ic, double* rho) {
+  if (int r = check_species(c, ic)) return r;
+  if (!rho) return fail(c, ALADYN_EINVAL, "grid_charge_noalloc: NULL rho");
+  const int64_t n = c->g.ncell;
+  if (n > kNoAllocCells)
+    return fail(c, ALADYN_EINVAL,
+                "grid_charge_noalloc: %lld grid nodes exceed its fixed device buffer of %lld "
+                "(use aladyn_grid_charge)",
+                (long long)n, (long long)kNoAllocCells);
+  double* d = noalloc_scratch(c);
+  if (!d) return fail(c, ALADYN_ESTATE, "grid_charge_noalloc: device buffer not set up");
+  // the whole used range is overwritten by the caller's rho: nothing of an
+  // earlier call survives into this one
+  HIPCK(c, hipMemcpyAsync(d, rho, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (int r = grid_charge_dev(c, ic, d)) return r;
+  HIPCK(c, hipMemcpyAsync(rho, d, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
   return ALADYN_OK;
 }
 

@@ -49,6 +49,7 @@ module aladyn_gpu
   public :: aladyn_lp_window_xshift, aladyn_inject_species, aladyn_box
   public :: aladyn_set_injection, aladyn_injection_state, aladyn_set_box_origin
   public :: aladyn_energy_momenta, aladyn_field_energy, aladyn_grid_charge
+  public :: aladyn_grid_charge_noalloc
   public :: aladyn_continuity_residual, aladyn_set_diagnostics
   public :: aladyn_synchronize, aladyn_kernel_timing
 
@@ -219,6 +220,16 @@ module aladyn_gpu
     end function
 
     integer(c_int) function aladyn_grid_charge(ctx, ic, rho) bind(C, name="aladyn_grid_charge")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ic
+      real(c_double), intent(inout) :: rho(*)
+    end function
+
+    ! no counterpart in the reference: aladyn_grid_charge for callers that
+    ! cannot allocate device memory (a fixed 2**23-node device array per GPU)
+    integer(c_int) function aladyn_grid_charge_noalloc(ctx, ic, rho) &
+        bind(C, name="aladyn_grid_charge_noalloc")
       import :: c_int, c_ptr, c_int32_t, c_double
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: ic

@@ -195,6 +195,15 @@ int aladyn_field_energy(aladyn_ctx *ctx, double out[6]);
 /* set_grid_charge for species ic (grid_part_util.f90:240-335), accumulated in
  * FP64 on the device and copied into rho(n1p, n2p, n3p) (added to it). */
 int aladyn_grid_charge(aladyn_ctx *ctx, int32_t ic, double *rho);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the same as
+ * aladyn_grid_charge, but its device staging of rho is a fixed array of
+ * 2^23 FP64 values per GPU held by the library (64 MiB, set up at
+ * aladyn_create), so the call allocates nothing (aladyn_grid_charge
+ * allocates its diagnostic buffers on first use).  Grids of more than 2^23
+ * nodes (n1p*n2p*n3p) give ALADYN_EINVAL.  Calls that overlap on one GPU
+ * share that array and race. */
+int aladyn_grid_charge_noalloc(aladyn_ctx *ctx, int32_t ic, double *rho);
 /* Discrete continuity residual of the last particle_stage on the device in
  * FP64: r = (rho^{n+1} - rho^n) + (1/curr_ndim) sum_d D_d J_d(raw) on the
  * interior, with rho from the particle positions before and after the stage
