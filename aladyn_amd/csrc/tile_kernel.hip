@@ -14,10 +14,11 @@
 //      nearest node moved on at most one axis adds the part of its Esirkepov
 //      current on its old cell's node block to register sums of that cell
 //      (flushed into the LDS patch when the thread's slice moves to the next
-//      cell) and the layer beyond the block, if any, to the LDS patch; one
-//      whose node moved on two or three axes is queued per wave (64
-//      entries) and deposited through the union window, one per lane, when
-//      the next round's entries would not fit; survivors are written to the
+//      cell); the layer beyond the block of one whose node moved on one axis
+//      waits in a per-wave edge queue (64 entries) and is added, one particle
+//      per lane, when the next round's entries would not fit; one whose node
+//      moved on two or three axes is queued likewise and deposited through
+//      the union window; survivors are written to the
 //      other buffer: stayers into slots reserved per chunk, movers into
 //      their new tile with wave-aggregated atomics, overflow into the global
 //      list;
@@ -25,6 +26,7 @@
 // LDS accumulation is FP64 because LDS FP32 atomic adds are far slower than
 // FP64 ones on gfx950; there are no global float atomics.
 #include "tile_device.hpp"
+
 
 namespace aladyn {
 
@@ -190,8 +192,9 @@ struct CellAcc {
   // component) and the outer node's weight wo (the transverse components);
   // the other axes are inside the block.
   template <int JX, int JY, int PVOL>
-  __device__ inline void edge(double* j1, const int sg[3], const R a0[3][3], const R w1[3][3],
-                              const R fl[3][2], const R wo[3], const R fo[3]) const {
+  __device__ static inline void edge(double* j1, const int sg[3], const R a0[3][3],
+                                     const R w1[3][3], const R fl[3][2], const R wo[3],
+                                     const R fo[3]) {
     const R half = R(0.5);
     if constexpr (D == 2) {
       if (sg[0] != 0) {
@@ -307,6 +310,8 @@ struct CellAcc {
 //   Jl  double [D][PVOL]    current patch, nodes T0-2 .. T0+T+1 per axis
 //   Q   per wave: R s[6][QCAP], float wq[QCAP], int meta[QCAP]: the queue of
 //       particles whose node moved on two or three axes
+//   E   per wave: the same for particles whose node moved on one axis (the
+//       layer of their current outside the old cell's node block)
 //   M   per wave: the record words of MCAP movers (word-major), their
 //       destination tiles and ids
 //   offs, fillc int [NCL+1], [NCL]; misc int [4]; wsum int [NW]; perm u16 [kChunk]
@@ -324,7 +329,7 @@ struct TileShape {
   static constexpr int MW_BYTES = MCAP * (8 * (int)sizeof(R) + 8);
   static constexpr int M_BYTES = NW * MW_BYTES;
   static constexpr int I_BYTES = 4 * (2 * NCL + 5 + NW);
-  static constexpr int BYTES = F_BYTES + J_BYTES + Q_BYTES + M_BYTES + I_BYTES + 2 * kChunk;
+  static constexpr int BYTES = F_BYTES + J_BYTES + 2 * Q_BYTES + M_BYTES + I_BYTES + 2 * kChunk;
   static_assert(NT % 64 == 0 && NT >= NCL, "threads: whole wavefronts, at least one per cell");
   static_assert(kChunk % NT == 0, "a chunk is a whole number of records per thread");
 };
@@ -370,6 +375,87 @@ __device__ inline void drain_queue(const WaveQueue<R, 64>& qu, int n, double* Jl
     double* ub = Jl + (cx + 1 + sh[0]) +
                  S::JX * ((cy + 1 + sh[1]) + S::JY * (D == 3 ? cz + 1 + sh[2] : 0));
     deposit_general<D, S::JX, S::JY, S::PVOL, R>(ub, s0, s1, o0, o1, wq);
+  }
+}
+
+// One axis on which the nearest node stayed: quadratic weights of the old
+// and new offsets and the two in-block fluxes, as axis_window gives them for
+// sg = 0.
+template <typename R>
+__device__ inline void axis_kept(R s0, R s1, R wq, R a0[3], R w1[3], R fl[2]) {
+  const R half = R(0.5), one = R(1);
+  qweights<R>(s0, a0);
+  qweights<R>(s1, w1);
+  const R h = half * wq * (s1 - s0);
+  fl[0] = h * ((one - s0) - s1);
+  fl[1] = h * ((one + s0) + s1);
+}
+
+// Drains the n <= 64 queued single-axis movers, one per lane: the layer of
+// each one's Esirkepov current just outside its old cell's node block (the
+// rest went into the register sums), the terms CellAcc::edge adds.  The
+// moved axis a and the other two b < c are per-lane values (LDS strides and
+// component offsets), so every lane runs the same instructions whichever
+// axis its particle crossed on:
+//   J_a, outer flux:  fo * (P0_b[m] a0_c[l] + P1_b[m] w1_c[l]), P0 = a0 + w1/2,
+//                     P1 = w1 + a0/2 (the transverse factor of J_a)
+//   J_b, outer node:  fl_b[f] * (wo * (a0_c[l]/2 + w1_c[l]))
+//   J_c, outer node:  fl_c[f] * (wo * (a0_b[l]/2 + w1_b[l]))
+// (in 2-D: fo * (a0_b[m] + w1_b[m]) and fl_b[f] * wo).
+template <int D, int TX, int TY, int TZ, int NT, typename R>
+__device__ inline void drain_edges(const WaveQueue<R, 64>& qu, int n, double* Jl) {
+  using S = TileShape<D, TX, TY, TZ, NT, R>;
+  constexpr int JX = S::JX, JXY = S::JX * S::JY, PVOL = S::PVOL;
+  const int lane = (int)__lane_id();
+  if (lane < n) {
+    const int e = lane;
+    const R wq = (R)qu.wq[e];
+    const int meta = qu.meta[e];
+    const int lc = meta & 1023;
+    const int sgy = ((meta >> 12) & 3) - 1, sgz = D == 3 ? ((meta >> 14) & 3) - 1 : 0;
+    const int ax = sgy != 0 ? 1 : (sgz != 0 ? 2 : 0);  // the moved axis
+    const int sg = ((meta >> (10 + 2 * ax)) & 3) - 1;
+    const int b = ax == 0 ? 1 : 0, c = ax == 2 ? 1 : 2;
+    const int sa = ax == 0 ? 1 : (ax == 1 ? JX : JXY);
+    const int sb = b == 0 ? 1 : JX, sc = c == 1 ? JX : JXY;
+    const int af = sg > 0 ? 2 : -1, as = sg > 0 ? 3 : -1;
+    const R half = R(0.5);
+    R a0a[3], w1a[3], fla[2], wo, fo;
+    const R s0a = qu.s[ax * S::QCAP + e];
+    qweights<R>(s0a, a0a);
+    axis_window<R>(s0a, qu.s[(3 + ax) * S::QCAP + e], sg, wq, a0a, w1a, fla, wo, fo);
+    R a0b[3], w1b[3], flb[2];
+    axis_kept<R>(qu.s[b * S::QCAP + e], qu.s[(3 + b) * S::QCAP + e], wq, a0b, w1b, flb);
+    const int cx = lc % TX, cy = (lc / TX) % TY, cz = D == 3 ? lc / (TX * TY) : 0;
+    double* j1 = Jl + (cx + 1) + JX * ((cy + 1) + S::JY * (D == 3 ? cz + 1 : 0));
+    double* pa = j1 + ax * PVOL + af * sa;
+    double* pb = j1 + b * PVOL + as * sa;
+    if constexpr (D == 2) {
+#pragma unroll
+      for (int m = 0; m < 3; m++) atomicAdd(pa + m * sb, (double)(fo * (a0b[m] + w1b[m])));
+#pragma unroll
+      for (int f = 0; f < 2; f++) atomicAdd(pb + f * sb, (double)(flb[f] * wo));
+    } else {
+      R a0c[3], w1c[3], flc[2];
+      axis_kept<R>(qu.s[c * S::QCAP + e], qu.s[(3 + c) * S::QCAP + e], wq, a0c, w1c, flc);
+      double* pc = j1 + c * PVOL + as * sa;
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        const R p0 = a0b[m] + half * w1b[m], p1 = w1b[m] + half * a0b[m];
+#pragma unroll
+        for (int l = 0; l < 3; l++)
+          atomicAdd(pa + m * sb + l * sc, (double)(fo * (p0 * a0c[l] + p1 * w1c[l])));
+      }
+#pragma unroll
+      for (int l = 0; l < 3; l++) {
+        const R tc = wo * (half * a0c[l] + w1c[l]), tb = wo * (half * a0b[l] + w1b[l]);
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+          atomicAdd(pb + f * sb + l * sc, (double)(flb[f] * tc));
+          atomicAdd(pc + l * sb + f * sc, (double)(flc[f] * tb));
+        }
+      }
+    }
   }
 }
 
@@ -429,7 +515,8 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
   R* Fl = (R*)lds;
   double* Jl = (double*)(lds + S::F_BYTES);
   char* Q = lds + S::F_BYTES + S::J_BYTES;
-  int* offs = (int*)(Q + S::Q_BYTES + S::M_BYTES);  // NCL + 1
+  char* E = Q + S::Q_BYTES;
+  int* offs = (int*)(E + S::Q_BYTES + S::M_BYTES);  // NCL + 1
   int* fillc = offs + NCL + 1;  // NCL
   int* misc = fillc + NCL;      // [0] chunk slot base, [1] stayers, [2] max per cell
   int* wsum = misc + 4;          // per-wave totals of the cell-count scan
@@ -461,18 +548,27 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
   // the first chunk's cells, is issued before the first LDS store, so the
   // HBM latency is paid once per tile
   {
-    constexpr int FPT = (NF * GVOL + NT - 1) / NT;
-    R fv[FPT];
+    // rows of GX consecutive x nodes: row = (c*GZ + lk)*GY + lj holds LDS
+    // entries row*GX .. row*GX+GX-1; a pass covers RPP rows, one node per
+    // thread, so the index arithmetic is per row and mostly per thread
+    constexpr int GZ = S::GZ, ROWS = NF * GZ * GY, RPP = NT / GX;
+    constexpr int PASSES = (ROWS + RPP - 1) / RPP;
+    const int li = me % GX, r0 = me / GX;
+    const int i = T0x - 1 + li;
+    const bool xin = r0 < RPP && i <= ta.n1p;
+    // node (T0x-1+li, T0y-1, T0z-1) of component 0
+    const R* eb = ebf + (i - 1) +
+                  (int64_t)ta.n1p * ((T0y - 2) + (int64_t)ta.n2p * (D == 3 ? T0z - 2 : 0));
+    R fv[PASSES];
 #pragma unroll
-    for (int u = 0; u < FPT; u++) {
-      const int q = threadIdx.x + u * NT;
-      const int c = q / GVOL, r = q - c * GVOL;
-      const int li = r % GX, lj = (r / GX) % GY, lk = r / (GX * GY);
-      const int i = T0x - 1 + li, j = T0y - 1 + lj, k = D == 3 ? T0z - 1 + lk : 1;
+    for (int u = 0; u < PASSES; u++) {
+      const int row = r0 + u * RPP;
+      const int c = row / (GZ * GY), rem = row - c * (GZ * GY);
+      const int lk = rem / GY, lj = rem - lk * GY;
+      const int j = T0y - 1 + lj, k = D == 3 ? T0z - 1 + lk : 1;
       fv[u] = R(0);
-      if (q < NF * GVOL && i <= ta.n1p && j <= ta.n2p && k <= ta.n3p)
-        fv[u] = ebf[(int64_t)c * ta.ncell + (i - 1) +
-                    (int64_t)ta.n1p * ((j - 1) + (int64_t)ta.n2p * (k - 1))];
+      if (xin && row < ROWS && j <= ta.n2p && k <= ta.n3p)
+        fv[u] = eb[(int64_t)c * ta.ncell + ta.n1p * (lj + ta.n2p * lk)];
     }
     load_cells(0);
     // the patch holds alp*charge*F: the push's charge multiply and alp
@@ -480,16 +576,18 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
     // once per node instead of once per particle
     const R fs = sc.alp * sc.qch;
 #pragma unroll
-    for (int u = 0; u < FPT; u++) {
-      const int q = threadIdx.x + u * NT;
-      if (q < NF * GVOL) Fl[q] = fs * fv[u];
+    for (int u = 0; u < PASSES; u++) {
+      const int row = r0 + u * RPP;
+      if (r0 < RPP && row < ROWS) Fl[row * GX + li] = fs * fv[u];
     }
   }
   for (int q = threadIdx.x; q < D * PVOL; q += NT) Jl[q] = 0.0;
 
   const WaveQueue<R, QCAP> qu(Q + (me >> 6) * S::QW_BYTES);
   int qcount = 0;  // wave-uniform: entries 0 .. qcount-1 are waiting
-  const MoverStage<R, S::MCAP> mst(Q + S::Q_BYTES + (me >> 6) * S::MW_BYTES);
+  const WaveQueue<R, QCAP> equ(E + (me >> 6) * S::QW_BYTES);
+  int ecount = 0;  // wave-uniform: single-axis movers waiting for their layer
+  const MoverStage<R, S::MCAP> mst(E + S::Q_BYTES + (me >> 6) * S::MW_BYTES);
   int mcount = 0;  // wave-uniform: staged movers
   // LDS current patch node N-1 of local cell lc (the patch starts at T0-2)
   const auto patch_node = [&](int lc) -> double* {
@@ -584,6 +682,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
       Rec<R> q = nxt;
       if (qi + 1 < q1) load_raw<D, R>(A, cb + perm[qi + 1], nxt);
       bool keep = false, cross = false;
+      int nmv = 0;  // axes on which the nearest node moved
       R so[3] = {R(0), R(0), R(0)}, wq = R(0);
       R a0[3][3] = {{R(1), R(0), R(0)}, {R(1), R(0), R(0)}, {R(1), R(0), R(0)}};
       int lc = 0, N0[3] = {0, 0, 0};
@@ -615,6 +714,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
           sg[d] = q.N[d] - N0[d];
           nmoved += sg[d] != 0 ? 1 : 0;
         }
+        nmv = nmoved;
         cross = nmoved > 1;  // node moved on two or three axes: the queue
         if (keep && !cross) {
           // a thread's slice runs through the cells in order: sum in
@@ -629,9 +729,35 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
 #pragma unroll
           for (int d = 0; d < D; d++) axis_window<R>(so[d], q.s[d], sg[d], wq, a0[d], w1[d], fl[d], wo[d], fo[d]);
           acc.add(a0, w1, fl);
-          if (nmoved) acc.template edge<JX, JY, PVOL>(patch_node(lc), sg, a0, w1, fl, wo, fo);
         }
       }
+      // particles whose node moved on one axis: the layer of their current
+      // outside the node block goes through the wave's edge queue
+      const bool emove = keep && nmv == 1;
+      const unsigned long long em = __ballot(emove);
+      const int enew = __popcll(em);
+      if (ecount + enew > QCAP) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        drain_edges<D, TX, TY, TZ, NT, R>(equ, ecount, Jl);
+        ecount = 0;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+      if (emove) {
+        const int e = ecount + __popcll(em & lt_mask);
+        int meta = lc;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          equ.s[d * QCAP + e] = so[d];
+          equ.s[(3 + d) * QCAP + e] = q.s[d];
+          meta |= (q.N[d] - N0[d] + 1) << (10 + 2 * d);
+        }
+        equ.wq[e] = (float)wq;
+        equ.meta[e] = meta;
+      }
+      ecount += enew;
       // particles whose node moved on two or three axes go to the wave's queue
       const unsigned long long cm = __ballot(keep && cross);
       const int nnew = __popcll(cm);
@@ -686,6 +812,9 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
         mst.id[e] = q.id;
       }
       mcount += mnew;
+      // stayers of a round take consecutive slots of the chunk's reserved
+      // range (one wave-aggregated LDS atomic), so a wave's record stores
+      // are contiguous
       if (stay) {
         const int slot = chunk_base + reserve_one(&misc[1]);
         int64_t dst = base + slot;
@@ -713,11 +842,12 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
     flush_movers<R, S::MCAP>(mst, mcount, B, seg_start, seg_cap, cnt_next, ovf_next, grow,
                              sc.ovf_base);
   }
-  if (qcount > 0) {
+  if (qcount > 0 || ecount > 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    drain_queue<D, TX, TY, TZ, NT, R>(qu, qcount, Jl);
+    if (qcount > 0) drain_queue<D, TX, TY, TZ, NT, R>(qu, qcount, Jl);
+    if (ecount > 0) drain_edges<D, TX, TY, TZ, NT, R>(equ, ecount, Jl);
   }
   if (acc_lc >= 0) acc.template flush<JX, JY, PVOL>(patch_node(acc_lc));
   __syncthreads();
