@@ -57,6 +57,7 @@ SIGNATURES = {
     "aladyn_get_current": [_P, _D],
     "aladyn_lpf2_evolve": [_P, C.c_int32, C.c_int32],
     "aladyn_set_graphs": [_P, C.c_int32],
+    "aladyn_set_dt": [_P, C.c_double],
     "aladyn_pfields_prepare": [_P],
     "aladyn_curr_clean": [_P],
     "aladyn_particle_stage": [_P, C.c_int32, C.c_int32],
@@ -69,6 +70,9 @@ SIGNATURES = {
     "aladyn_set_injection": [_P, C.c_int32, _D, _I64, _D, C.c_int32, _D, C.c_int32, _D, _D, _I64,
                              C.c_double, C.c_double],
     "aladyn_injection_state": [_P, C.c_int32, C.POINTER(C.c_int64)],
+    "aladyn_put_envelope": [_P, C.c_double, _D],
+    "aladyn_get_envelope": [_P, _D],
+    "aladyn_env_lpf2_evolve": [_P, C.c_int32],
     "aladyn_energy_momenta": [_P, C.c_int32, _D, _D],
     "aladyn_field_energy": [_P, _D],
     "aladyn_grid_charge": [_P, C.c_int32, _D],

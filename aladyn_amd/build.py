@@ -16,7 +16,8 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libaladyn_gpu.so"
-SOURCES = ["capi.hip", "fields.hip", "tile_kernel.hip", "tile_store.hip", "reductions.hip"]
+SOURCES = ["capi.hip", "fields.hip", "tile_kernel.hip", "tile_store.hip", "reductions.hip",
+           "envelope.hip"]
 ARCH = os.environ.get("ALADYN_OFFLOAD_ARCH", "gfx950")
 
 

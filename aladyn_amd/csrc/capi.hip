@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/aladyn_gpu.h"
+#include "env.hpp"
 #include "lp_kernels.hpp"
 #include "tile_store.hpp"
 
@@ -99,6 +100,16 @@ struct aladyn_ctx {
   double win_xlapse = 0.0;
   int win_wi1 = 0;
   InjTable inj[ALADYN_MAX_SPECIES];
+  // envelope model (aladyn_put_envelope): env(n1p, n2p, n3p, 4) = A_R^n,
+  // A_I^n, A_R^{n-1}, A_I^{n-1}; je(n1p, n2p, n3p, curr_ndim + 1) the
+  // envelope step's reuse of jc; den the FP64 density source
+  bool have_env = false;
+  double env_oml = 0.0;
+  double* env64 = nullptr;
+  float* env32 = nullptr;
+  double* je64 = nullptr;
+  float* je32 = nullptr;
+  double* den = nullptr;
   // timing: event pairs around the particle kernel and the field solve,
   // recorded on the context stream (ring of kRing steps)
   TimerRing tpart, tfield;
@@ -275,6 +286,11 @@ extern "C" int aladyn_destroy(aladyn_ctx* c) {
   (void)hipFree(c->rho0);
   (void)hipFree(c->rho1);
   (void)hipFree(c->jraw);
+  (void)hipFree(c->env64);
+  (void)hipFree(c->env32);
+  (void)hipFree(c->je64);
+  (void)hipFree(c->je32);
+  (void)hipFree(c->den);
   clear_graphs(c);
   c->tpart.destroy();
   c->tfield.destroy();
@@ -755,6 +771,23 @@ static int lpf2_evolve_graph(aladyn_ctx* c) {
   return ALADYN_OK;
 }
 
+// Lp_cycle's trim of the last step, dt_loc = tmax - tnow
+// (ALaDyn.F90:101): every dt-dependent quantity (push constants, Yee and
+// Mur coefficients, the window's lapse, the envelope solver's constants) is
+// formed from g.dt at each call, so only the captured steps, whose kernel
+// arguments hold it, are dropped.
+extern "C" int aladyn_set_dt(aladyn_ctx* c, double dt) {
+  if (!c) return ALADYN_EINVAL;
+  const DevGrid& g = c->g;
+  if (!(dt > 0.0)) return fail(c, ALADYN_EINVAL, "set_dt: dt must be positive");
+  if (dt >= g.dx || dt >= g.dy || (g.ndim == 3 && dt >= g.dz))
+    return fail(c, ALADYN_EINVAL, "set_dt: dt must be smaller than every cell size");
+  c->cfg.dt = dt;
+  c->g.dt = dt;
+  clear_graphs(c);
+  return ALADYN_OK;
+}
+
 extern "C" int aladyn_set_graphs(aladyn_ctx* c, int32_t enable) {
   if (!c) return ALADYN_EINVAL;
   c->use_graphs = enable != 0;
@@ -782,6 +815,71 @@ extern "C" int aladyn_lpf2_evolve(aladyn_ctx* c, int32_t iter, int32_t initial_t
   HIPCK(c, hipEventRecord(c->tfield.end(), c->stream));
   c->tfield.commit();
   return ALADYN_OK;
+}
+
+// --------------------------------------------------------- envelope model
+extern "C" int aladyn_put_envelope(aladyn_ctx* c, double oml, const double* env) {
+  if (!c || !env) return fail(c, ALADYN_EINVAL, "put_envelope: NULL argument");
+  if (!(oml > 0.0) || !std::isfinite(oml))
+    return fail(c, ALADYN_EINVAL, "put_envelope: the laser frequency oml must be positive");
+  const DevGrid& g = c->g;
+  const int64_t n = g.ncell * 4;
+  for (int64_t i = 0; i < n; i++)
+    if (!std::isfinite(env[i])) return fail(c, ALADYN_EINVAL, "put_envelope: non-finite value");
+  const size_t nje = (size_t)g.ncell * (g.curr_ndim + 1);
+  if (f64(c)) {
+    if (!c->env64) HIPCK(c, hipMalloc(&c->env64, (size_t)n * sizeof(double)));
+    if (!c->je64) HIPCK(c, hipMalloc(&c->je64, nje * sizeof(double)));
+    if (int r = put_field_array(c, c->env64, env, n)) return r;
+  } else {
+    if (!c->env32) HIPCK(c, hipMalloc(&c->env32, (size_t)n * sizeof(float)));
+    if (!c->je32) HIPCK(c, hipMalloc(&c->je32, nje * sizeof(float)));
+    if (int r = put_field_array(c, c->env32, env, n)) return r;
+  }
+  if (!c->den) HIPCK(c, hipMalloc(&c->den, (size_t)g.ncell * sizeof(double)));
+  c->env_oml = oml;
+  c->have_env = true;
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_get_envelope(aladyn_ctx* c, double* env) {
+  if (!c || !env) return fail(c, ALADYN_EINVAL, "get_envelope: NULL argument");
+  if (!c->have_env) return fail(c, ALADYN_ESTATE, "get_envelope before put_envelope");
+  const int64_t n = c->g.ncell * 4;
+  return f64(c) ? get_field_array(c, env, c->env64, n) : get_field_array(c, env, c->env32, n);
+}
+
+template <typename R>
+static int env_step(aladyn_ctx* c, R* ebf, R* jc, R* env, R* je) {
+  const DevGrid& g = c->g;
+  const EnvConst ec = env_const(g, c->cfg.der_ord, c->env_oml, c->cfg.cfl);
+  std::string msg;
+  // jc = 0; jc(1) = |A^n|^2/2, jc(2:D+1) its gradient (env_amp_prepare)
+  HIPCK(c, env_amp_prepare<R>(g, ec, env, je, c->stream));
+  fill<double>(c->den, g.ncell, 0.0, c->stream);
+  // set_env_acc, lpf_env_momenta, set_env_density for species 1
+  env_particles_push<R>(&c->ts[0], c->tg, g, ebf, je, c->den, c->cfg.lorentz_fact[0], c->stream);
+  HIPCK(c, hipGetLastError());
+  // env_den_collect, advance_lpf_envelope, env_fields_average
+  HIPCK(c, env_grid_advance<R>(g, ec, env, je, c->den, c->stream));
+  // set_env_grad_interp, lpf_env_positions, cell_part_dist, curr_accumulate
+  if (int r = env_particles_move<R>(&c->ts[0], c->tg, g, je, jc, c->patch, c->stream, &msg))
+    return fail(c, r, "%s", msg.c_str());
+  // curr_mpi_collect, advance_lpf_fields (env_evolve_in_time.f90:345-358)
+  curr_collect<R>(g, jc, c->stream);
+  advance_fields<R>(g, ebf, jc, c->stream);
+  HIPCK(c, hipGetLastError());
+  c->j_is_zero = false;
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_env_lpf2_evolve(aladyn_ctx* c, int32_t iter) {
+  if (!c) return ALADYN_EINVAL;
+  (void)iter;
+  if (!c->have_fields) return fail(c, ALADYN_ESTATE, "env_lpf2_evolve before put_fields");
+  if (!c->have_env) return fail(c, ALADYN_ESTATE, "env_lpf2_evolve before put_envelope");
+  return f64(c) ? env_step<double>(c, c->ebf64, c->jc64, c->env64, c->je64)
+                : env_step<float>(c, c->ebf32, c->jc32, c->env32, c->je32);
 }
 
 // ------------------------------------------------------------ diagnostics

@@ -191,6 +191,17 @@ __global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g,
   }
 }
 
+template <typename R>
+void tile_store_fold(const TileGeom& tg, const DevGrid& g, const double* patch, R* jc,
+                     int accumulate, hipStream_t s) {
+  const dim3 fb((g.n1p + 63) / 64, (g.n2p + 3) / 4, g.n3p);
+  k_fold<R><<<fb, 256, 0, s>>>(tg, g, patch, jc, accumulate);
+}
+template void tile_store_fold<float>(const TileGeom&, const DevGrid&, const double*, float*, int,
+                                     hipStream_t);
+template void tile_store_fold<double>(const TileGeom&, const DevGrid&, const double*, double*,
+                                      int, hipStream_t);
+
 // ------------------------------------------------- relayout / transfers
 static constexpr int kB = 256;
 

@@ -135,6 +135,11 @@ int tile_store_step_finish(TileStore* ts, const TileGeom& tg, const DevGrid& g, 
                            std::string* msg);
 int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
                         std::string* msg);
+// jc = (accumulate != 0 ? jc : 0) + the sum of the per-tile FP64 current
+// patches, rounded to the grid's type once per node
+template <typename R>
+void tile_store_fold(const TileGeom& tg, const DevGrid& g, const double* patch, R* jc,
+                     int accumulate, hipStream_t s);
 void tile_store_grid_charge(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* rho,
                             hipStream_t s);
 void tile_store_energy(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* out,

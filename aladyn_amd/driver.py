@@ -26,11 +26,12 @@ def lp_cycle(gpu, *, tmax: float, tstart: float = 0.0, w_speed: float = 0.0,
     """Lp_cycle (ALaDyn.F90:89-107) on an LPStep whose configuration carries
     dt_loc: while tnow < tmax, lp_run(tnow, iter) -- lpf2_evolve(iter), then
     lp_window_xshift(w_sh, iter) when w_speed > 0, wi_time <= tnow < wf_time
-    and mod(iter, w_sh) == 0 -- and timing's tnow += dt_loc, iter += 1.
-    initial_time holds for the first step when tstart < dt_loc.  The
-    reference trims the last step to tmax - tnow, a change of dt_loc by
-    rounding only; the device keeps dt_loc.  on_step(iter, tnow) runs after
-    each step (the place of Data_out).  Returns (iter, tnow)."""
+    and mod(iter, w_sh) == 0 -- and timing's tnow += dt_loc, iter += 1
+    (run_data_info.f90:68-71); then, when tnow + dt_loc >= tmax, dt_loc =
+    tmax - tnow for the steps left (set_dt on the device).  initial_time
+    holds for the first step when tstart < dt_loc (ALaDyn.F90:66-67).
+    on_step(iter, tnow) runs after each step (the place of Data_out).
+    Returns (iter, tnow)."""
     dt = gpu.cfg.dt
     tnow, it = tstart, 0
     initial = tnow < dt
@@ -40,9 +41,13 @@ def lp_cycle(gpu, *, tmax: float, tstart: float = 0.0, w_speed: float = 0.0,
             gpu.lp_window_xshift(w_sh, it, w_speed)
         tnow += dt
         it += 1
-        initial = False
         if on_step is not None:
             on_step(it, tnow)
+        if tnow + dt >= tmax and tmax - tnow != dt:
+            dt = tmax - tnow
+            if dt > 0.0:
+                gpu.set_dt(dt)
+        initial = False
     return it, tnow
 
 

@@ -12,6 +12,7 @@ Every call goes to the HIP library; errors raise AladynError.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 
 import numpy as np
 
@@ -147,6 +148,32 @@ class LPStep:
 
     def advance_lpf_fields(self):
         self._ck(self.lib.aladyn_advance_lpf_fields(self._h))
+
+    def set_dt(self, dt: float):
+        """Lp_cycle's trim of the last step, dt_loc = tmax - tnow
+        (ALaDyn.F90:101): the following steps use dt."""
+        self._ck(self.lib.aladyn_set_dt(self._h, float(dt)))
+        self.cfg = dataclasses.replace(self.cfg, dt=float(dt))
+
+    # ------------------------------------------------------ envelope model
+    def put_envelope(self, env: np.ndarray, oml: float):
+        """The envelope env(n1p, n2p, n3p, 4) = A_R^n, A_I^n, A_R^{n-1},
+        A_I^{n-1} (array_alloc.f90:108-111) and the laser frequency oml."""
+        c = self.cfg
+        a = np.asfortranarray(env, dtype=np.float64)
+        if a.shape != (c.n1p, c.n2p, c.n3p, 4):
+            raise ValueError(f"env shape {a.shape}, expected {(c.n1p, c.n2p, c.n3p, 4)}")
+        self._ck(self.lib.aladyn_put_envelope(self._h, float(oml), _dptr(a)))
+
+    def get_envelope(self) -> np.ndarray:
+        c = self.cfg
+        out = np.zeros((c.n1p, c.n2p, c.n3p, 4), dtype=np.float64, order="F")
+        self._ck(self.lib.aladyn_get_envelope(self._h, _dptr(out)))
+        return out
+
+    def env_lpf2_evolve(self, it: int = 0):
+        """One env_lpf2_evolve (env_evolve_in_time.f90:196-361) for species 0."""
+        self._ck(self.lib.aladyn_env_lpf2_evolve(self._h, int(it)))
 
     def synchronize(self):
         self._ck(self.lib.aladyn_synchronize(self._h))

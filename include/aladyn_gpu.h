@@ -105,6 +105,10 @@ int aladyn_get_current(aladyn_ctx *ctx, double *jc);
  * curr_accumulate}, curr_mpi_collect, advance_lpf_fields.  initial_time != 0
  * applies the first-step momentum back-shift (ALaDyn.F90:67). */
 int aladyn_lpf2_evolve(aladyn_ctx *ctx, int32_t iter, int32_t initial_time);
+/* Changes the time step of the following steps: Lp_cycle's trim of the last
+ * step, dt_loc = tmax - tnow when tnow + dt_loc >= tmax (ALaDyn.F90:101).
+ * 0 < dt < every cell size (ALADYN_EINVAL otherwise). */
+int aladyn_set_dt(aladyn_ctx *ctx, double dt);
 /* No counterpart in the reference (its steps are Fortran calls).  For
  * callers whose steps are short enough that launching them costs as much
  * as running them (many small runs stepped in turn): with enable != 0,
@@ -184,6 +188,25 @@ int aladyn_set_box_origin(aladyn_ctx *ctx, double xmin);
 /* The current particle box: out = {xmin, xmax, ymin, ymax, zmin, zmax}
  * (loc_*grid%gmin/gmax; x moves with the window). */
 int aladyn_box(aladyn_ctx *ctx, double out[6]);
+
+/* The envelope (ponderomotive) model, env_lpf2_evolve
+ * (env_evolve_in_time.f90:196-361), for species 0 on one rank: one laser
+ * colour, no ionization, no fluid.  put_envelope copies in
+ * env(n1p, n2p, n3p, 4) = A_R^n, A_I^n, A_R^{n-1}, A_I^{n-1} (the envelope
+ * array of v_alloc, array_alloc.f90:108-111) and the laser frequency oml
+ * (set_init_param.f90:291, 2 pi/lam0); get_envelope copies it out. */
+int aladyn_put_envelope(aladyn_ctx *ctx, double oml, const double *env);
+int aladyn_get_envelope(aladyn_ctx *ctx, double *env);
+/* One env_lpf2_evolve(iter): |A^n|^2/2 and its gradient (env_amp_prepare);
+ * per particle of species 0, E, B, |A|^2/2 and its gradient at x^n
+ * (set_env_acc), the envelope pusher's momenta (lpf_env_momenta) and the
+ * density source (set_env_density); the envelope advance A^n -> A^{n+1}
+ * (env_den_collect, advance_lpf_envelope with env_maxw_solve); |A|^2/2 at
+ * t^{n+1/2} (env_fields_average); per particle, its gradient at x^n
+ * (set_env_grad_interp), the positions (lpf_env_positions), removal at the
+ * faces (cell_part_dist), the Esirkepov current; curr_mpi_collect and
+ * advance_lpf_fields.  Needs put_fields and put_envelope (ALADYN_ESTATE). */
+int aladyn_env_lpf2_evolve(aladyn_ctx *ctx, int32_t iter);
 
 /* energy_momenta for species ic (diag_part_and_fields.f90:172-219), FP64
  * device reduction: ek[0..6] = ek(1:7), *ekmax = max(gamma-1). */

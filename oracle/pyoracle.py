@@ -223,6 +223,32 @@ class Oracle:
                                    _p(ebfp), _p(_f(ebf)), _p(_f(jc)), 1 if initial_time else 0)
         return np.asfortranarray(part[:n])
 
+    def lp_cycle(self, part, ebf, jc, *, tmax, tstart=0.0, w_speed=0.0, wi_time=0.0,
+                 wf_time=float("inf"), w_sh=1, win=None):
+        """Lp_cycle (ALaDyn.F90:89-107) with lp_run (pic_evolve_in_time.f90:
+        113-131) and timing (run_data_info.f90:68-71): initial_time = tnow <
+        dt_loc (ALaDyn.F90:66-67); while tnow < tmax: lpf2_evolve(iter), the
+        window call when w_speed > 0, wi_time <= tnow < wf_time and mod(iter,
+        w_sh) == 0; tnow += dt_loc, iter += 1; if tnow + dt_loc >= tmax,
+        dt_loc = tmax - tnow (this oracle's grid dt is changed in place).
+        Returns (particles, iter, tnow, the dt of every step)."""
+        tnow, it = tstart, 0
+        initial = tnow < self.g.dt
+        win = OrcWindow() if win is None else win
+        dts = []
+        while tnow < tmax:
+            part = self.lpf2_evolve(part, ebf, jc, initial_time=initial)
+            dts.append(self.g.dt)
+            if w_speed > 0.0 and wi_time <= tnow < wf_time and it % w_sh == 0:
+                _, part = self.window_xshift(win, w_sh, it, w_speed, ebf, part)
+            if tnow < tmax:
+                tnow += self.g.dt
+                it += 1
+            if tnow + self.g.dt >= tmax:
+                self.g.dt = tmax - tnow
+            initial = False
+        return part, it, tnow, dts
+
     def window_xshift(self, win, witr, init_iter, w_speed, ebf, part):
         """lp_window_xshift: returns (nshx, surviving particles as a new array);
         moves this oracle's box (xmn, xmx) with the window."""

@@ -3,6 +3,9 @@ run_data_info.f90:68-71, start_all.F90:219-226) on a recording stand-in for
 the device, with expectations worked out from those lines."""
 
 
+import numpy as np
+import pytest
+
 from aladyn_amd.driver import dt_loc, lp_cycle, with_dt_loc
 from cases import small_2d
 
@@ -28,6 +31,10 @@ class _Rec:
         self.calls.append(("window", witr, it))
         return 0
 
+    def set_dt(self, dt):
+        self.calls.append(("dt", dt))
+        self.cfg.dt = dt
+
 
 def test_cycle_steps_and_window_calls():
     dt = 0.25
@@ -46,3 +53,48 @@ def test_cycle_restart_time_has_no_initial_step():
     g = _Rec(0.25)
     lp_cycle(g, tmax=1.0, tstart=0.5)
     assert [c for c in g.calls] == [("step", 0, False), ("step", 1, False)]
+
+
+def test_cycle_trims_the_last_step_to_tmax():
+    """ALaDyn.F90:101: after the step that leaves tnow + dt_loc >= tmax,
+    dt_loc = tmax - tnow, so the run ends on tmax."""
+    g = _Rec(0.25)
+    it, tnow = lp_cycle(g, tmax=1.1)
+    assert it == 5 and tnow == 1.0 + (1.1 - 1.0)
+    dts = [c for c in g.calls if c[0] == "dt"]
+    assert dts == [("dt", 1.1 - 1.0)]
+    # the trim comes after the fourth step, before the fifth
+    assert g.calls.index(("dt", 1.1 - 1.0)) == g.calls.index(("step", 4, False)) - 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [(64, 1e-10), (32, 1e-4)])
+def test_gpu_cycle_matches_the_oracle_loop(precision, tol):
+    """Lp_cycle on the device (driver.lp_cycle) against the oracle's
+    restatement of the same loop (Oracle.lp_cycle), with window calls and a
+    tmax that leaves a last step of 0.4 dt: same step count, same trimmed
+    dt, and the particles and fields agree."""
+    from aladyn_amd import LPStep
+    from conftest import rel_maxerr
+    from oracle.pyoracle import Oracle
+
+    cfg, part, ebf = small_2d()
+    tmax = 9.4 * cfg.dt
+    print(f"lp_cycle precision={precision} tmax={tmax}", flush=True)
+    orc = Oracle(cfg)
+    e = ebf.copy(order="F")
+    p, it_o, t_o, dts = orc.lp_cycle(part.copy(order="F"), e, orc.zeros_current(), tmax=tmax,
+                                     w_speed=1.0, wi_time=0.0, w_sh=4)
+    with LPStep(cfg, precision, track_ids=True) as gpu:
+        gpu.put_species(0, part)
+        gpu.put_fields(ebf)
+        it_g, t_g = lp_cycle(gpu, tmax=tmax, w_speed=1.0, wi_time=0.0, w_sh=4)
+        assert gpu.cfg.dt == dts[-1]
+        pg, eg = gpu.get_species(0), gpu.get_fields()
+    assert (it_g, t_g) == (it_o, t_o) == (10, t_o)
+    assert abs(dts[-1] - 0.4 * cfg.dt) < 1e-12 * cfg.dt
+    assert pg.shape == p.shape
+    D = cfg.curr_ndim
+    assert rel_maxerr(pg[:, D:2 * D], p[:, D:2 * D]) <= tol
+    assert np.max(np.abs(pg[:, 0] - p[:, 0])) <= tol * (cfg.xmx - cfg.xmn)
+    assert rel_maxerr(eg, e) <= tol
