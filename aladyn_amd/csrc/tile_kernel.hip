@@ -855,54 +855,58 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
   for (int q = threadIdx.x; q < D * PVOL; q += NT) P[q] = Jl[q];
 }
 
-// threads per workgroup: one per node-cell of the tile (4 wavefronts for an
-// 8x8x4 tile); the register sums of the current need more than the 128
-// VGPRs a 1024-thread workgroup would allow
-template <int D, int TX, int TY, int TZ>
-static constexpr int tile_threads_for() {
-  return TX * TY * TZ;
-}
-
-template <int D, int TX, int TY, int TZ, typename R>
+// Threads per workgroup: one per node-cell of the tile by default (4
+// wavefronts for an 8x8x4 tile), or two per node-cell (NT = 2 NCL, the
+// aladyn_config.block knob, compiled for the 8x8x4 and 16x16 shapes): twice
+// the threads split the same sorted chunk into half as long slices, and the
+// workgroup holds twice the per-wave queues in LDS.  The register sums of
+// the current need more than the 128 VGPRs a 1024-thread workgroup would
+// allow, so NT stays at most 512.
+template <int D, int TX, int TY, int TZ, int NT, typename R>
 static int lds_bytes_for() {
-  return TileShape<D, TX, TY, TZ, tile_threads_for<D, TX, TY, TZ>(), R>::BYTES;
+  return TileShape<D, TX, TY, TZ, NT, R>::BYTES;
 }
 
-template <int D, int TX, int TY, int TZ, typename R>
+template <int D, int TX, int TY, int TZ, int NT, typename R>
 static hipError_t launch(const TileGeom& tg, const TileArgs& ta, const StepConst<R>& sc,
                          const R* ebf, const RecPtr& A, const RecPtr& B, const int* ss,
                          const int* sc_, const int* cc, int* cn, int* oc, int* on, double* patch,
                          int* grow, hipStream_t s) {
-  constexpr int NT = tile_threads_for<D, TX, TY, TZ>();
-  k_tile<D, TX, TY, TZ, NT, R><<<tg.ntiles, NT, lds_bytes_for<D, TX, TY, TZ, R>(), s>>>(
+  k_tile<D, TX, TY, TZ, NT, R><<<tg.ntiles, NT, lds_bytes_for<D, TX, TY, TZ, NT, R>(), s>>>(
       ta, sc, ebf, A, B, ss, sc_, cc, cn, oc, on, patch, grow);
   return hipGetLastError();
 }
 
-// shapes with a compiled kernel; tile_geom_init accepts only these (and only
-// those whose LDS fits, see tile_lds_bytes)
-bool tile_shape_supported(int D, int tx, int ty, int tz) {
-  if (D == 3) return (tx == 8 && ty == 8 && tz == 8) || (tx == 8 && ty == 8 && tz == 4) ||
-                     (tx == 16 && ty == 8 && tz == 4);
-  return (tx == 16 && ty == 16) || (tx == 32 && ty == 16);
+// shapes and workgroup sizes with a compiled kernel; tile_geom_init accepts
+// only these (and only those whose LDS fits, see tile_lds_bytes)
+bool tile_shape_supported(int D, int tx, int ty, int tz, int nt, int real_bytes) {
+  const int ncl = tx * ty * (D == 3 ? tz : 1);
+  if (D == 3) {
+    // the FP64 kernel's registers do not fit two wavefronts per SIMD
+    if (tx == 8 && ty == 8 && tz == 4) return nt == ncl || (nt == 2 * ncl && real_bytes == 4);
+    return nt == ncl && ((tx == 8 && ty == 8 && tz == 8) || (tx == 16 && ty == 8 && tz == 4));
+  }
+  if (tx == 16 && ty == 16) return nt == ncl || nt == 2 * ncl;
+  return nt == ncl && tx == 32 && ty == 16;
 }
 
 template <typename R>
-static int lds_bytes_r(int D, int tx, int ty, int tz) {
+static int lds_bytes_r(int D, int tx, int ty, int tz, int nt) {
   if (D == 3) {
-    if (tx == 8 && ty == 8 && tz == 8) return lds_bytes_for<3, 8, 8, 8, R>();
-    if (tx == 8 && ty == 8 && tz == 4) return lds_bytes_for<3, 8, 8, 4, R>();
-    return lds_bytes_for<3, 16, 8, 4, R>();
+    if (tx == 8 && ty == 8 && tz == 8) return lds_bytes_for<3, 8, 8, 8, 512, R>();
+    if (tx == 8 && ty == 8 && tz == 4)
+      return nt == 512 ? lds_bytes_for<3, 8, 8, 4, 512, R>() : lds_bytes_for<3, 8, 8, 4, 256, R>();
+    return lds_bytes_for<3, 16, 8, 4, 512, R>();
   }
-  if (tx == 16) return lds_bytes_for<2, 16, 16, 1, R>();
-  return lds_bytes_for<2, 32, 16, 1, R>();
+  if (tx == 16)
+    return nt == 512 ? lds_bytes_for<2, 16, 16, 1, 512, R>() : lds_bytes_for<2, 16, 16, 1, 256, R>();
+  return lds_bytes_for<2, 32, 16, 1, 512, R>();
 }
 
-int tile_lds_bytes(int D, int tx, int ty, int tz, int real_bytes) {
-  return real_bytes == 8 ? lds_bytes_r<double>(D, tx, ty, tz) : lds_bytes_r<float>(D, tx, ty, tz);
+int tile_lds_bytes(int D, int tx, int ty, int tz, int nt, int real_bytes) {
+  return real_bytes == 8 ? lds_bytes_r<double>(D, tx, ty, tz, nt)
+                         : lds_bytes_r<float>(D, tx, ty, tz, nt);
 }
-
-int tile_threads(int tx, int ty, int tz) { return tx * ty * tz; }
 
 template <typename R>
 hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepConst<R>& sc,
@@ -912,17 +916,19 @@ hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepCo
                               int* grow, hipStream_t s) {
   TileArgs ta{tg.nt[0], tg.nt[1], tg.bx, tg.by, g.n1p, g.n2p, g.n3p, g.ncell};
   const int* T = tg.T;
-#define ALADYN_TILE_CASE(DD, X, Y, Z)                                                           \
-  if (g.ndim == DD && T[0] == X && T[1] == Y && (DD == 2 || T[2] == Z))                         \
-    return launch<DD, X, Y, Z, R>(tg, ta, sc, ebf, A, B, seg_start, seg_cap, cnt_cur, cnt_next, \
-                                  ovf_cur, ovf_next, patch, grow, s);
-  ALADYN_TILE_CASE(3, 8, 8, 8)
-  ALADYN_TILE_CASE(3, 8, 8, 4)
-  if constexpr (sizeof(R) == 4) {  // the FP64 patches of this shape exceed the LDS
-    ALADYN_TILE_CASE(3, 16, 8, 4)
+#define ALADYN_TILE_CASE(DD, X, Y, Z, NT)                                                    \
+  if (g.ndim == DD && T[0] == X && T[1] == Y && (DD == 2 || T[2] == Z) && tg.block == NT)    \
+    return launch<DD, X, Y, Z, NT, R>(tg, ta, sc, ebf, A, B, seg_start, seg_cap, cnt_cur,    \
+                                      cnt_next, ovf_cur, ovf_next, patch, grow, s);
+  ALADYN_TILE_CASE(3, 8, 8, 8, 512)
+  ALADYN_TILE_CASE(3, 8, 8, 4, 256)
+  if constexpr (sizeof(R) == 4) {  // the FP64 queues and patches of these exceed the LDS
+    ALADYN_TILE_CASE(3, 8, 8, 4, 512)
+    ALADYN_TILE_CASE(3, 16, 8, 4, 512)
   }
-  ALADYN_TILE_CASE(2, 16, 16, 1)
-  ALADYN_TILE_CASE(2, 32, 16, 1)
+  ALADYN_TILE_CASE(2, 16, 16, 1, 256)
+  ALADYN_TILE_CASE(2, 16, 16, 1, 512)
+  ALADYN_TILE_CASE(2, 32, 16, 1, 512)
 #undef ALADYN_TILE_CASE
   return hipErrorInvalidConfiguration;
 }

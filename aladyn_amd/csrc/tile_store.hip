@@ -13,9 +13,8 @@
 namespace aladyn {
 
 // tile_kernel.hip
-bool tile_shape_supported(int D, int tx, int ty, int tz);
-int tile_lds_bytes(int D, int tx, int ty, int tz, int real_bytes);
-int tile_threads(int tx, int ty, int tz);
+bool tile_shape_supported(int D, int tx, int ty, int tz, int nt, int real_bytes);
+int tile_lds_bytes(int D, int tx, int ty, int tz, int nt, int real_bytes);
 template <typename R>
 hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepConst<R>& sc,
                               const R* ebf, const RecPtr& A, const RecPtr& B,
@@ -32,7 +31,9 @@ static int bits_for(int n) {  // bits to hold values 0..n
 
 const char* tile_geom_error(int code) {
   switch (code) {
-    case 1: return "tile shape has no compiled kernel (3-D: 8x8x8, 8x8x4, 16x8x4; 2-D: 16x16, 32x16)";
+    case 1: return "tile shape or threads per tile (block) has no compiled kernel (3-D: 8x8x8, "
+                   "8x8x4, 16x8x4; 2-D: 16x16, 32x16; block 0 or one thread per node-cell, or "
+                   "two per node-cell for 8x8x4 and 16x16)";
     case 2: return "grid too large for the 31-bit packed cell index";
     case 3: return "tile patches exceed the LDS of one workgroup";
     default: return "invalid tile geometry";
@@ -47,7 +48,12 @@ int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg) {
   const int def[3] = {D == 3 ? 8 : 16, D == 3 ? 8 : 16, D == 3 ? 4 : 1};
   for (int d = 0; d < 3; d++) tg->T[d] = cfg.tile[d] > 0 ? cfg.tile[d] : def[d];
   if (D == 2) tg->T[2] = 1;
-  if (!tile_shape_supported(D, tg->T[0], tg->T[1], tg->T[2])) return 1;
+  // threads per workgroup (aladyn_config.block): one per node-cell unless
+  // the caller asks for two
+  const int ncl = tg->T[0] * tg->T[1] * tg->T[2];
+  tg->block = cfg.block > 0 ? cfg.block : ncl;
+  const int rbytes = cfg.precision == ALADYN_PREC_F64 ? 8 : 4;
+  if (!tile_shape_supported(D, tg->T[0], tg->T[1], tg->T[2], tg->block, rbytes)) return 1;
   // particle nearest nodes span Fortran indices 3 .. n+3 (x_loc in (0, n])
   const int n[3] = {g.nx, g.ny, g.nz};
   for (int d = 0; d < 3; d++) tg->nt[d] = d < D ? (n[d] + 1 + tg->T[d] - 1) / tg->T[d] : 1;
@@ -62,10 +68,9 @@ int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg) {
   tg->by = bits_for(g.n2p);
   const int bz = D == 3 ? bits_for(g.n3p) : 0;
   if (tg->bx + tg->by + bz > 31) return 2;
-  tg->block = tile_threads(tg->T[0], tg->T[1], tg->T[2]);
   tg->sort_every = cfg.sort_every > 0 ? cfg.sort_every : 64;
   tg->rbytes = cfg.precision == ALADYN_PREC_F64 ? 8 : 4;
-  tg->lds_bytes = tile_lds_bytes(D, tg->T[0], tg->T[1], tg->T[2], tg->rbytes);
+  tg->lds_bytes = tile_lds_bytes(D, tg->T[0], tg->T[1], tg->T[2], tg->block, tg->rbytes);
   if (tg->lds_bytes > 160 * 1024) return 3;
   return 0;
 }

@@ -88,14 +88,15 @@ def log(*a):
 WINDOW_START, WINDOW_EVERY, WINDOW_SPEED = 100, 20, 1.0
 
 
-def run_path(cfg, part, ebf, precision, steps, warmup, table=None, graphs=False, sort_every=0):
+def run_path(cfg, part, ebf, precision, steps, warmup, table=None, graphs=False, sort_every=0,
+             tile=None, block=0):
     """Times `steps` lpf2_evolve calls after `warmup` untimed ones.  With a
     target table the box moves as lp_run moves it and the entering plasma is
     injected on the device from the table.  graphs: steps replayed as HIP
     graphs (the per-kernel HIP-event timing is then not recorded)."""
     from aladyn_amd import LPStep
 
-    gpu = LPStep(cfg, precision, sort_every=sort_every)
+    gpu = LPStep(cfg, precision, sort_every=sort_every, tile=tile, block=block)
     gpu.put_species(0, part)
     gpu.put_fields(ebf)
     gpu.set_graphs(graphs)
@@ -232,6 +233,9 @@ def main():
                          "default)")
     ap.add_argument("--graphs", action="store_true",
                     help="also time the FP32 path with steps replayed as HIP graphs")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="use the library's default tile shape, threads per tile and cadence "
+                         "instead of the autotuner's pick recorded in aladyn_amd/tuned.json")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -261,18 +265,29 @@ def main():
         log(f"cpu baseline: {cpu['value']:.4g} particle-steps/s on {cpu['cores']} cores "
             f"({cpu['sample']})")
 
+    # the autotuner's pick for this workload (python -m aladyn_amd.tune), FP32
+    from aladyn_amd.tune import load_pick
+
+    pick = None if args.no_tune else load_pick(args.workload)
+    tile32, block32 = None, 0
+    se32 = args.sort_every
+    if pick:
+        tile32, block32 = tuple(pick["tile"]), int(pick["block"])
+        se32 = args.sort_every or int(pick["sort_every"])
+        log(f"tuned configuration: tile {tile32}, block {block32}, sort_every {se32}")
+
     # the plasma continues beyond the box: the columns the window brings in
     table = None if args.no_window else plasma_table(cfg, (2, 2, 2), x_start_cell=cfg.nx,
                                                      x_end_cell=cfg.nx + 400)
     r32 = run_path(cfg, part, ebf, PREC_F32, args.steps, args.warmup, table,
-                   sort_every=args.sort_every)
+                   sort_every=se32, tile=tile32, block=block32)
     log(f"fp32: {r32['ms_per_step']:.3f} ms/step, {r32['pps']:.4g} particle-steps/s, "
         f"particle kernel {r32['part_ms']:.3f} ms, fields {r32['field_ms']:.3f} ms, "
         f"{r32['moves']} window moves")
     rg = None
     if args.graphs:
         rg = run_path(cfg, part, ebf, PREC_F32, args.steps, args.warmup, table, graphs=True,
-                      sort_every=args.sort_every)
+                      sort_every=se32, tile=tile32, block=block32)
         log(f"fp32 graphs: {rg['ms_per_step']:.3f} ms/step, {rg['pps']:.4g} particle-steps/s")
     r64 = None
     if not args.no_fp64:
@@ -316,6 +331,11 @@ def main():
             "cells": cells,
             "path": "fp32 tile-sorted (int32 cell + fp32 offset)",
             "steps_timed": f"{args.warmup}..{args.warmup + args.steps - 1}",
+            "tiles": {"tile": list(tile32) if tile32 else ([16, 16] if two_d else [8, 8, 4]),
+                      "threads_per_tile": block32 or None,
+                      "sort_every": se32 or 64,
+                      "source": ("autotuner pick, aladyn_amd/tuned.json" if pick
+                                 else "library defaults")},
             "moving_window": None if table is None else
             f"from step {WINDOW_START}, every {WINDOW_EVERY} steps at c, plasma injected from "
             f"the target table ({r32['moves']} moves timed)",
@@ -327,7 +347,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None if two_d else measured_traffic(cfg.nx, cfg.ny, cfg.nz),
-            "kernel": "k_tile<2, 16x16 tiles, float>" if two_d else "k_tile<3, 8x8x4 tiles, float>",
+            "kernel": (f"k_tile<{2 if two_d else 3}, "
+                       f"{'x'.join(map(str, tile32 or ((16, 16) if two_d else (8, 8, 4))))} tiles, "
+                       f"{block32 or 'one thread per node-cell'}, float>"),
             "kernel_ms": r32["part_ms"],
             "algorithmic_bytes_per_launch": kern_bytes,
         },

@@ -65,8 +65,9 @@ typedef struct aladyn_config {
   double lorentz_fact[ALADYN_MAX_SPECIES]; /* 1/m per species */
   /* tile path tuning (0 = default) */
   int32_t tile[3];       /* tile extent in nodes per axis */
-  int32_t block;         /* unused: the particle kernel runs one thread per
-                            node-cell of the tile */
+  int32_t block;         /* threads per tile workgroup: 0 = one per node-cell;
+                            twice the node-cells is compiled for 8x8x4 and
+                            16x16 tiles (FP32 only in 3-D) */
   int32_t sort_every;    /* steps between full re-layouts of the tile store */
   /* reserved[0] != 0 keeps each particle's original index, so that
    * get_species returns particles in put/inject order (costs 4 B/particle of
