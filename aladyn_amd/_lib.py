@@ -81,6 +81,12 @@ SIGNATURES = {
     "aladyn_set_diagnostics": [_P, C.c_int32],
     "aladyn_synchronize": [_P],
     "aladyn_kernel_timing": [_P, C.c_int32, _D, _D, C.POINTER(_I64)],
+    "aladyn_env_kernel_timing": [_P, C.c_int32, _D, _D, C.POINTER(_I64)],
+    "aladyn_batch_create": [C.POINTER(_P), C.c_int32, C.c_int32, C.POINTER(_P)],
+    "aladyn_batch_lpf2_evolve": [_P, C.c_int32, C.c_int32],
+    "aladyn_batch_synchronize": [_P],
+    "aladyn_batch_destroy": [_P],
+    "aladyn_batch_last_error": [_P],
 }
 
 _lib = None
@@ -99,7 +105,7 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
     for name, argtypes in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = argtypes
-        fn.restype = C.c_char_p if name == "aladyn_last_error" else C.c_int
+        fn.restype = C.c_char_p if name.endswith("last_error") else C.c_int
     if path is None:
         _lib = lib
     return lib

@@ -113,6 +113,8 @@ struct aladyn_ctx {
   // timing: event pairs around the particle kernel and the field solve,
   // recorded on the context stream (ring of kRing steps)
   TimerRing tpart, tfield;
+  // envelope steps: the two particle kernels (push, move)
+  TimerRing tenv_push, tenv_move;
   // captured lpf2_evolve steps (aladyn_set_graphs): one executable graph per
   // state of the stores and of the box, see step_key
   bool use_graphs = false;
@@ -247,7 +249,7 @@ extern "C" int aladyn_create(const aladyn_config* cfg, aladyn_ctx** out) {
   HIPCK(c, hipGetDevice(&c->dev));
   if (int r = noalloc_scratch_init(c)) return r;
   HIPCK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  if (c->tpart.create() || c->tfield.create())
+  if (c->tpart.create() || c->tfield.create() || c->tenv_push.create() || c->tenv_move.create())
     return fail(c, ALADYN_EHIP, "event creation failed");
   const DevGrid& g = c->g;
   const size_t nf = (size_t)g.ncell * g.nfield, nj = (size_t)g.ncell * g.curr_ndim;
@@ -294,6 +296,8 @@ extern "C" int aladyn_destroy(aladyn_ctx* c) {
   clear_graphs(c);
   c->tpart.destroy();
   c->tfield.destroy();
+  c->tenv_push.destroy();
+  c->tenv_move.destroy();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return ALADYN_OK;
@@ -474,6 +478,13 @@ extern "C" int aladyn_lp_window_xshift(aladyn_ctx* c, int32_t witr, int32_t init
         fields_left_xshift<double>(g, c->ebf64, (int)nshx, c->stream);
       else
         fields_left_xshift<float>(g, c->ebf32, (int)nshx, c->stream);
+      HIPCK(c, hipGetLastError());
+    }
+    if (c->have_env) {  // the envelope's four components (window.f90:327-331)
+      if (f64(c))
+        fields_left_xshift<double>(g, c->env64, (int)nshx, c->stream, 4);
+      else
+        fields_left_xshift<float>(g, c->env32, (int)nshx, c->stream, 4);
       HIPCK(c, hipGetLastError());
     }
     // particles behind the new left face leave (cell_part_dist(.true.))
@@ -858,13 +869,20 @@ static int env_step(aladyn_ctx* c, R* ebf, R* jc, R* env, R* je) {
   HIPCK(c, env_amp_prepare<R>(g, ec, env, je, c->stream));
   fill<double>(c->den, g.ncell, 0.0, c->stream);
   // set_env_acc, lpf_env_momenta, set_env_density for species 1
-  env_particles_push<R>(&c->ts[0], c->tg, g, ebf, je, c->den, c->cfg.lorentz_fact[0], c->stream);
+  const bool timed = c->ts[0].loaded;
+  env_particles_push<R>(&c->ts[0], c->tg, g, ebf, je, c->den, c->cfg.lorentz_fact[0], c->stream,
+                        timed ? c->tenv_push.begin() : nullptr,
+                        timed ? c->tenv_push.end() : nullptr);
   HIPCK(c, hipGetLastError());
+  if (timed) c->tenv_push.commit();
   // env_den_collect, advance_lpf_envelope, env_fields_average
   HIPCK(c, env_grid_advance<R>(g, ec, env, je, c->den, c->stream));
   // set_env_grad_interp, lpf_env_positions, cell_part_dist, curr_accumulate
-  if (int r = env_particles_move<R>(&c->ts[0], c->tg, g, je, jc, c->patch, c->stream, &msg))
+  if (int r = env_particles_move<R>(&c->ts[0], c->tg, g, je, jc, c->patch, c->stream, &msg,
+                                    timed ? c->tenv_move.begin() : nullptr,
+                                    timed ? c->tenv_move.end() : nullptr))
     return fail(c, r, "%s", msg.c_str());
+  if (timed) c->tenv_move.commit();
   // curr_mpi_collect, advance_lpf_fields (env_evolve_in_time.f90:345-358)
   curr_collect<R>(g, jc, c->stream);
   advance_fields<R>(g, ebf, jc, c->stream);
@@ -880,6 +898,227 @@ extern "C" int aladyn_env_lpf2_evolve(aladyn_ctx* c, int32_t iter) {
   if (!c->have_env) return fail(c, ALADYN_ESTATE, "env_lpf2_evolve before put_envelope");
   return f64(c) ? env_step<double>(c, c->ebf64, c->jc64, c->env64, c->je64)
                 : env_step<float>(c, c->ebf32, c->jc32, c->env32, c->je32);
+}
+
+// ------------------------------------------------------------ batched runs
+// Many independent contexts stepped together: each kernel of the step runs
+// once for all of them (the particle kernel over a work list of (run, tile)
+// pairs, the grid kernels with one row of workgroups per run), the step's
+// launches replayed as one captured HIP graph.  The kernels read each run's
+// buffers from a descriptor array refreshed before every step, so the graph
+// stays valid across buffer swaps and re-layouts.
+struct aladyn_batch {
+  std::vector<aladyn_ctx*> ctx;
+  hipStream_t stream = nullptr;
+  bool f64 = false, graphs = true;
+  int ndim = 2, nruns = 0, nwork = 0, max_ntiles = 0;
+  int64_t max_ncell = 0;
+  BatchDims dims{};
+  TileGeom tg0{};
+  void* d_tile = nullptr;    // TileRun<R>[nruns]
+  void* d_field = nullptr;   // FieldRun<R>[nruns]
+  void* h_tile[2] = {nullptr, nullptr};   // pinned, one per step parity
+  void* h_field[2] = {nullptr, nullptr};
+  int2* d_work = nullptr;
+  int* d_ovf = nullptr;
+  int* h_ovf[2] = {nullptr, nullptr};     // pinned overflow counts per step parity
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool ev_valid[2] = {false, false};
+  int64_t steps = 0;
+  hipGraphExec_t graph = nullptr;
+  std::string err;
+};
+
+static int bfail(aladyn_batch* b, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf;
+  return code;
+}
+
+#define BHIPCK(b, call)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return bfail(b, ALADYN_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),  \
+                   __FILE__, __LINE__);                                                    \
+  } while (0)
+
+extern "C" const char* aladyn_batch_last_error(const aladyn_batch* b) {
+  return b ? b->err.c_str() : "null batch";
+}
+
+extern "C" int aladyn_batch_destroy(aladyn_batch* b) {
+  if (!b) return ALADYN_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  (void)hipFree(b->d_tile);
+  (void)hipFree(b->d_field);
+  (void)hipFree(b->d_work);
+  (void)hipFree(b->d_ovf);
+  for (int k = 0; k < 2; k++) {
+    if (b->h_tile[k]) (void)hipHostFree(b->h_tile[k]);
+    if (b->h_field[k]) (void)hipHostFree(b->h_field[k]);
+    if (b->h_ovf[k]) (void)hipHostFree(b->h_ovf[k]);
+    if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
+  }
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_batch_create(aladyn_ctx* const* ctxs, int32_t n, int32_t use_graphs,
+                                   aladyn_batch** out) {
+  if (!out) return ALADYN_EINVAL;
+  *out = nullptr;
+  aladyn_batch* b = new aladyn_batch();
+  *out = b;
+  if (!ctxs || n < 1) return bfail(b, ALADYN_EINVAL, "batch_create: no contexts");
+  b->graphs = use_graphs != 0;
+  for (int r = 0; r < n; r++) {
+    aladyn_ctx* c = ctxs[r];
+    if (!c) return bfail(b, ALADYN_EINVAL, "batch_create: context %d is NULL", r);
+    const aladyn_ctx* c0 = ctxs[0];
+    if (c->cfg.nsp != 1)
+      return bfail(b, ALADYN_EINVAL, "batch_create: context %d runs %d species (1 supported)", r,
+                   c->cfg.nsp);
+    if (!c->ts[0].loaded || !c->have_fields)
+      return bfail(b, ALADYN_ESTATE, "batch_create: context %d has no species or fields", r);
+    if (c->cfg.precision != c0->cfg.precision || c->g.ndim != c0->g.ndim ||
+        c->tg.T[0] != c0->tg.T[0] || c->tg.T[1] != c0->tg.T[1] || c->tg.T[2] != c0->tg.T[2] ||
+        c->tg.block != c0->tg.block)
+      return bfail(b, ALADYN_EINVAL,
+                   "batch_create: context %d differs from context 0 in precision, dimension, "
+                   "tile shape or threads per tile", r);
+    if (c->dev != c0->dev) return bfail(b, ALADYN_EINVAL, "batch_create: contexts on two devices");
+    BHIPCK(b, hipStreamSynchronize(c->stream));
+    b->ctx.push_back(c);
+  }
+  const aladyn_ctx* c0 = ctxs[0];
+  b->f64 = c0->cfg.precision == ALADYN_PREC_F64;
+  b->ndim = c0->g.ndim;
+  b->nruns = n;
+  b->tg0 = c0->tg;
+  std::vector<DevGrid> gs;
+  std::vector<int2> work;
+  for (aladyn_ctx* c : b->ctx) {
+    gs.push_back(c->g);
+    b->max_ntiles = std::max(b->max_ntiles, c->tg.ntiles);
+    b->max_ncell = std::max(b->max_ncell, c->g.ncell);
+  }
+  // the work list: the runs' tiles, run-major
+  for (int r = 0; r < n; r++)
+    for (int t = 0; t < b->ctx[r]->tg.ntiles; t++) work.push_back(make_int2(r, t));
+  b->nwork = (int)work.size();
+  b->dims = batch_dims(gs.data(), n);
+  const size_t tsz = b->f64 ? sizeof(TileRun<double>) : sizeof(TileRun<float>);
+  const size_t fsz = b->f64 ? sizeof(FieldRun<double>) : sizeof(FieldRun<float>);
+  BHIPCK(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  BHIPCK(b, hipMalloc(&b->d_tile, tsz * n));
+  BHIPCK(b, hipMalloc(&b->d_field, fsz * n));
+  BHIPCK(b, hipMalloc(&b->d_work, sizeof(int2) * work.size()));
+  BHIPCK(b, hipMalloc(&b->d_ovf, sizeof(int) * n));
+  for (int k = 0; k < 2; k++) {
+    BHIPCK(b, hipHostMalloc(&b->h_tile[k], tsz * n, hipHostMallocDefault));
+    BHIPCK(b, hipHostMalloc(&b->h_field[k], fsz * n, hipHostMallocDefault));
+    BHIPCK(b, hipHostMalloc(&b->h_ovf[k], sizeof(int) * n, hipHostMallocDefault));
+    BHIPCK(b, hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming));
+  }
+  BHIPCK(b, hipMemcpyAsync(b->d_work, work.data(), sizeof(int2) * work.size(),
+                           hipMemcpyHostToDevice, b->stream));
+  BHIPCK(b, hipStreamSynchronize(b->stream));
+  return ALADYN_OK;
+}
+
+// The device work of one batched step (what the graph holds).
+template <typename R>
+static hipError_t batch_enqueue(aladyn_batch* b) {
+  const auto* tr = static_cast<const TileRun<R>*>(b->d_tile);
+  const auto* fr = static_cast<const FieldRun<R>*>(b->d_field);
+  field_xyzbd_batch<R>(fr, b->nruns, b->ndim, b->dims, b->stream);
+  if (hipError_t e = tile_store_batch_launch<R>(b->tg0, b->ndim, tr, b->nruns, b->d_work,
+                                                 b->nwork, b->max_ntiles, b->max_ncell, b->d_ovf,
+                                                 b->stream))
+    return e;
+  curr_collect_batch<R>(fr, b->nruns, b->ndim, b->dims, b->stream);
+  advance_fields_batch<R>(fr, b->nruns, b->dims, b->stream);
+  return hipGetLastError();
+}
+
+template <typename R>
+static int batch_step(aladyn_batch* b, int initial) {
+  const int par = (int)(b->steps & 1), prev = 1 - par;
+  // the pinned descriptors of this parity were last read by the copy of two
+  // steps ago, which the event wait of the previous step has seen complete
+  auto* ht = static_cast<TileRun<R>*>(b->h_tile[par]);
+  auto* hf = static_cast<FieldRun<R>*>(b->h_field[par]);
+  for (int r = 0; r < b->nruns; r++) {
+    aladyn_ctx* c = b->ctx[r];
+    R* ebf = f64(c) ? (R*)c->ebf64 : (R*)c->ebf32;
+    R* jc = f64(c) ? (R*)c->jc64 : (R*)c->jc32;
+    const double alp = 0.5 * c->g.dt * c->cfg.lorentz_fact[0];
+    tile_store_batch_desc<R>(&c->ts[0], c->tg, c->g, ebf, jc, c->patch, (R)alp, initial,
+                             &ht[r]);
+    hf[r] = FieldRun<R>{c->g, ebf, jc};
+  }
+  BHIPCK(b, hipMemcpyAsync(b->d_tile, ht, sizeof(TileRun<R>) * b->nruns, hipMemcpyHostToDevice,
+                           b->stream));
+  BHIPCK(b, hipMemcpyAsync(b->d_field, hf, sizeof(FieldRun<R>) * b->nruns,
+                           hipMemcpyHostToDevice, b->stream));
+  if (b->graphs) {
+    if (!b->graph) {
+      BHIPCK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      const hipError_t e = batch_enqueue<R>(b);
+      hipGraph_t graph = nullptr;
+      const hipError_t ee = hipStreamEndCapture(b->stream, &graph);
+      if (e != hipSuccess || ee != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        BHIPCK(b, e != hipSuccess ? e : ee);
+      }
+      const hipError_t ei = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      BHIPCK(b, ei);
+    }
+    BHIPCK(b, hipGraphLaunch(b->graph, b->stream));
+  } else {
+    BHIPCK(b, batch_enqueue<R>(b));
+  }
+  BHIPCK(b, hipMemcpyAsync(b->h_ovf[par], b->d_ovf, sizeof(int) * b->nruns,
+                           hipMemcpyDeviceToHost, b->stream));
+  BHIPCK(b, hipEventRecord(b->ev[par], b->stream));
+  b->ev_valid[par] = true;
+  // the previous step's overflow counts decide this step's re-layouts
+  const int* oprev = nullptr;
+  if (b->ev_valid[prev]) {
+    BHIPCK(b, hipEventSynchronize(b->ev[prev]));
+    oprev = b->h_ovf[prev];
+  }
+  b->steps++;
+  std::string msg;
+  for (int r = 0; r < b->nruns; r++) {
+    aladyn_ctx* c = b->ctx[r];
+    if (int e = tile_store_batch_finish(&c->ts[0], c->tg, c->g, oprev ? oprev[r] : -1, b->stream,
+                                        &msg))
+      return bfail(b, e, "run %d: %s", r, msg.c_str());
+    c->j_is_zero = false;
+  }
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_batch_lpf2_evolve(aladyn_batch* b, int32_t iter, int32_t initial_time) {
+  if (!b || b->nruns < 1) return ALADYN_EINVAL;
+  (void)iter;
+  const int init = initial_time ? 1 : 0;
+  return b->f64 ? batch_step<double>(b, init) : batch_step<float>(b, init);
+}
+
+extern "C" int aladyn_batch_synchronize(aladyn_batch* b) {
+  if (!b) return ALADYN_EINVAL;
+  BHIPCK(b, hipStreamSynchronize(b->stream));
+  return ALADYN_OK;
 }
 
 // ------------------------------------------------------------ diagnostics
@@ -998,6 +1237,20 @@ extern "C" int aladyn_continuity_residual(aladyn_ctx* c, double out[2]) {
 extern "C" int aladyn_synchronize(aladyn_ctx* c) {
   if (!c) return ALADYN_EINVAL;
   HIPCK(c, hipStreamSynchronize(c->stream));
+  return ALADYN_OK;
+}
+
+extern "C" int aladyn_env_kernel_timing(aladyn_ctx* c, int32_t reset, double* push_ms,
+                                        double* move_ms, int64_t* launches) {
+  if (!c) return ALADYN_EINVAL;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  if (push_ms) *push_ms = c->tenv_push.average();
+  if (move_ms) *move_ms = c->tenv_move.average();
+  if (launches) *launches = c->tenv_move.n;
+  if (reset) {
+    c->tenv_push.n = 0;
+    c->tenv_move.n = 0;
+  }
   return ALADYN_OK;
 }
 

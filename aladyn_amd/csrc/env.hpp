@@ -31,9 +31,11 @@ hipError_t env_amp_prepare(const DevGrid& g, const EnvConst& ec, const R* env, R
 
 // The particles' first half (set_env_acc, lpf_env_momenta, set_env_density)
 // on species store ts: momenta to p^{n+1/2} in place, den += wgh/gamp shapes.
+// The events, when given, are recorded around the particle kernel.
 template <typename R>
 void env_particles_push(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf,
-                        const R* je, double* den, double lorentz_fact, hipStream_t s);
+                        const R* je, double* den, double lorentz_fact, hipStream_t s,
+                        hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 
 // den_zyxbd, je(1) = je(3) = den, advance_lpf_envelope (A^n -> A^{n+1}) and
 // env_fields_average (je(1:D+1) = F at t^{n+1/2} and its gradient).
@@ -46,6 +48,7 @@ hipError_t env_grid_advance(const DevGrid& g, const EnvConst& ec, R* env, R* je,
 // the folded raw current.  Ends with the store's host-side step bookkeeping.
 template <typename R>
 int env_particles_move(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* je, R* jc,
-                       double* patch, hipStream_t s, std::string* msg);
+                       double* patch, hipStream_t s, std::string* msg,
+                       hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 
 }  // namespace aladyn

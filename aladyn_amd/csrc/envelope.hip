@@ -563,8 +563,10 @@ hipError_t env_amp_prepare(const DevGrid& g, const EnvConst& ec, const R* env, R
 
 template <typename R>
 void env_particles_push(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf,
-                        const R* je, double* den, double lorentz_fact, hipStream_t s) {
+                        const R* je, double* den, double lorentz_fact, hipStream_t s,
+                        hipEvent_t ev_begin, hipEvent_t ev_end) {
   if (!ts->loaded) return;
+  if (ev_begin) (void)hipEventRecord(ev_begin, s);
   const int cur = ts->cur;
   const RecPtr& A = ts->buf[cur];
   const R qch = (R)ts->charge, dth = (R)(0.5 * g.dt), alp = (R)(0.5 * g.dt * lorentz_fact);
@@ -577,11 +579,13 @@ void env_particles_push(TileStore* ts, const TileGeom& tg, const DevGrid& g, con
     k_env_push<3, R><<<nb, kEB, 0, s>>>(tg, g, A, ts->seg_start, ts->seg_cap, ts->count[cur],
                                         ts->ovf_base, ts->ovf_count + cur, ebf, je, qch, dth,
                                         alp, den);
+  if (ev_end) (void)hipEventRecord(ev_end, s);
 }
 
 template <typename R>
 int env_particles_move(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* je, R* jc,
-                       double* patch, hipStream_t s, std::string* msg) {
+                       double* patch, hipStream_t s, std::string* msg, hipEvent_t ev_begin,
+                       hipEvent_t ev_end) {
   const int cur = ts->cur, nxt = 1 - cur;
   const size_t npatch = (size_t)tg.ntiles * tg.pvol * g.curr_ndim;
   fill<double>(patch, (int64_t)npatch, 0.0, s);
@@ -599,6 +603,7 @@ int env_particles_move(TileStore* ts, const TileGeom& tg, const DevGrid& g, cons
     fill<int>(ts->count[nxt], tg.ntiles, 0, s);
     fill<int>(ts->ovf_count + nxt, 1, 0, s);
     const unsigned nb = tg.ntiles + 64;
+    if (ev_begin) (void)hipEventRecord(ev_begin, s);
     if (g.ndim == 2)
       k_env_move<2, R><<<nb, kEB, 0, s>>>(tg, g, sc, ts->buf[cur], ts->buf[nxt], ts->seg_start,
                                           ts->seg_cap, ts->count[cur], ts->ovf_count + cur, je,
@@ -607,6 +612,7 @@ int env_particles_move(TileStore* ts, const TileGeom& tg, const DevGrid& g, cons
       k_env_move<3, R><<<nb, kEB, 0, s>>>(tg, g, sc, ts->buf[cur], ts->buf[nxt], ts->seg_start,
                                           ts->seg_cap, ts->count[cur], ts->ovf_count + cur, je,
                                           ts->count[nxt], ts->ovf_count + nxt, patch, ts->grow);
+    if (ev_end) (void)hipEventRecord(ev_end, s);
     if (hipGetLastError() != hipSuccess) {
       *msg = "envelope particle kernel launch failed";
       return ALADYN_EHIP;
@@ -630,9 +636,10 @@ int env_particles_move(TileStore* ts, const TileGeom& tg, const DevGrid& g, cons
   template hipError_t env_amp_prepare<R>(const DevGrid&, const EnvConst&, const R*, R*,         \
                                          hipStream_t);                                           \
   template void env_particles_push<R>(TileStore*, const TileGeom&, const DevGrid&, const R*,    \
-                                      const R*, double*, double, hipStream_t);                   \
+                                      const R*, double*, double, hipStream_t, hipEvent_t,        \
+                                      hipEvent_t);                                               \
   template int env_particles_move<R>(TileStore*, const TileGeom&, const DevGrid&, const R*, R*, \
-                                     double*, hipStream_t, std::string*);
+                                     double*, hipStream_t, std::string*, hipEvent_t, hipEvent_t);
 ALADYN_ENV_INST(float)
 ALADYN_ENV_INST(double)
 #undef ALADYN_ENV_INST

@@ -5,6 +5,8 @@
 // the FP64 path, float for the FP32 path); all arrays are in the
 // reference's Fortran layout ebf(n1p,n2p,n3p,nc), x fastest, so consecutive
 // lanes touch consecutive addresses.
+#include <algorithm>
+
 #include "lp_kernels.hpp"
 
 namespace aladyn {
@@ -20,7 +22,7 @@ static inline unsigned nblocks(int64_t n) {
 // mpi_field_interface.f90:53-198 on one rank.  Three launches (x, y, z faces)
 // because the y faces extend over the x ghosts and z over both.
 template <typename T>
-__global__ void k_field_xbd(DevGrid g, T* ebf, int nc) {
+__device__ inline void b_field_xbd(DevGrid g, T* ebf, int nc) {
   const int nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1;
   const int64_t n = (int64_t)nj * nk * nc;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
@@ -39,9 +41,16 @@ __global__ void k_field_xbd(DevGrid g, T* ebf, int nc) {
     }
   }
 }
+template <typename T>
+__global__ void k_field_xbd(DevGrid g, T* ebf, int nc) { b_field_xbd<T>(g, ebf, nc); }
+template <typename T>
+__global__ void kb_field_xbd(const FieldRun<T>* runs) {
+  const FieldRun<T>& r = runs[blockIdx.y];
+  b_field_xbd<T>(r.g, r.ebf, r.g.nfield);
+}
 
 template <typename T>
-__global__ void k_field_ybd(DevGrid g, T* ebf, int nc) {
+__device__ inline void b_field_ybd(DevGrid g, T* ebf, int nc) {
   const int i1 = g.ix1 - 1, ni = g.ix2 - g.ix1 + 3, nk = g.kz2 - g.kz1 + 1;
   const int64_t n = (int64_t)ni * nk * nc;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
@@ -60,9 +69,16 @@ __global__ void k_field_ybd(DevGrid g, T* ebf, int nc) {
     }
   }
 }
+template <typename T>
+__global__ void k_field_ybd(DevGrid g, T* ebf, int nc) { b_field_ybd<T>(g, ebf, nc); }
+template <typename T>
+__global__ void kb_field_ybd(const FieldRun<T>* runs) {
+  const FieldRun<T>& r = runs[blockIdx.y];
+  b_field_ybd<T>(r.g, r.ebf, r.g.nfield);
+}
 
 template <typename T>
-__global__ void k_field_zbd(DevGrid g, T* ebf, int nc) {
+__device__ inline void b_field_zbd(DevGrid g, T* ebf, int nc) {
   const int i1 = g.ix1 - 1, ni = g.ix2 - g.ix1 + 3;
   const int j1 = g.jy1 - 1, nj = g.jy2 - g.jy1 + 3;
   const int64_t n = (int64_t)ni * nj * nc;
@@ -83,6 +99,13 @@ __global__ void k_field_zbd(DevGrid g, T* ebf, int nc) {
     }
   }
 }
+template <typename T>
+__global__ void k_field_zbd(DevGrid g, T* ebf, int nc) { b_field_zbd<T>(g, ebf, nc); }
+template <typename T>
+__global__ void kb_field_zbd(const FieldRun<T>* runs) {
+  const FieldRun<T>& r = runs[blockIdx.y];
+  b_field_zbd<T>(r.g, r.ebf, r.g.nfield);
+}
 
 template <typename T>
 void field_xyzbd(const DevGrid& g, T* ebf, int nc, hipStream_t s) {
@@ -99,7 +122,7 @@ void field_xyzbd(const DevGrid& g, T* ebf, int nc, hipStream_t s) {
 // ------------------------------------------------------------------- jc_xyzbd
 // mpi_curr_interface.f90:551-643 on one rank (ib* = 0 folds).
 template <typename T>
-__global__ void k_jc_xfold(DevGrid g, T* jc) {
+__device__ inline void b_jc_xfold(DevGrid g, T* jc) {
   const int nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1, nc = g.curr_ndim;
   const int64_t n = (int64_t)nj * nk * nc;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
@@ -116,9 +139,16 @@ __global__ void k_jc_xfold(DevGrid g, T* jc) {
     c[fofs(g, g.ix2 + 2, j, k)] = T(0);
   }
 }
+template <typename T>
+__global__ void k_jc_xfold(DevGrid g, T* jc) { b_jc_xfold<T>(g, jc); }
+template <typename T>
+__global__ void kb_jc_xfold(const FieldRun<T>* runs) {
+  const FieldRun<T>& r = runs[blockIdx.y];
+  b_jc_xfold<T>(r.g, r.jc);
+}
 
 template <typename T>
-__global__ void k_jc_yfold(DevGrid g, T* jc) {
+__device__ inline void b_jc_yfold(DevGrid g, T* jc) {
   const int ni = g.ix2 - g.ix1 + 1, nk = g.kz2 - g.kz1 + 1, nc = g.curr_ndim;
   const int64_t n = (int64_t)ni * nk * nc;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
@@ -138,9 +168,16 @@ __global__ void k_jc_yfold(DevGrid g, T* jc) {
     c[fofs(g, i, n2 + 2, k)] = T(0);
   }
 }
+template <typename T>
+__global__ void k_jc_yfold(DevGrid g, T* jc) { b_jc_yfold<T>(g, jc); }
+template <typename T>
+__global__ void kb_jc_yfold(const FieldRun<T>* runs) {
+  const FieldRun<T>& r = runs[blockIdx.y];
+  b_jc_yfold<T>(r.g, r.jc);
+}
 
 template <typename T>
-__global__ void k_jc_zfold(DevGrid g, T* jc) {
+__device__ inline void b_jc_zfold(DevGrid g, T* jc) {
   const int ni = g.ix2 - g.ix1 + 1, nj = g.jy2 - g.jy1 + 1, nc = g.curr_ndim;
   const int64_t n = (int64_t)ni * nj * nc;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
@@ -158,10 +195,17 @@ __global__ void k_jc_zfold(DevGrid g, T* jc) {
     c[fofs(g, i, j, n3 + 2)] = T(0);
   }
 }
+template <typename T>
+__global__ void k_jc_zfold(DevGrid g, T* jc) { b_jc_zfold<T>(g, jc); }
+template <typename T>
+__global__ void kb_jc_zfold(const FieldRun<T>* runs) {
+  const FieldRun<T>& r = runs[blockIdx.y];
+  b_jc_zfold<T>(r.g, r.jc);
+}
 
 // curr(:,:,:,i) = djc(i)*curr(:,:,:,i), i = 1..ndim (curr_and_fields_util.f90:115-119)
 template <typename T>
-__global__ void k_jc_scale(DevGrid g, T* jc) {
+__device__ inline void b_jc_scale(DevGrid g, T* jc) {
   const int64_t n = g.ncell;
   for (int c = 0; c < g.ndim; c++) {
     const T f = T(g.djc[c]);
@@ -170,6 +214,13 @@ __global__ void k_jc_scale(DevGrid g, T* jc) {
          t += (int64_t)gridDim.x * blockDim.x)
       a[t] = f * a[t];
   }
+}
+template <typename T>
+__global__ void k_jc_scale(DevGrid g, T* jc) { b_jc_scale<T>(g, jc); }
+template <typename T>
+__global__ void kb_jc_scale(const FieldRun<T>* runs) {
+  const FieldRun<T>& r = runs[blockIdx.y];
+  b_jc_scale<T>(r.g, r.jc);
 }
 
 template <typename T>
@@ -194,7 +245,7 @@ void curr_collect(const DevGrid& g, T* jc, hipStream_t s) {
 // (B ghosts on the left faces), imbd = 0, unstretched metric = 1.  Each
 // thread owns one face cell; faces do not read what another face writes.
 template <typename T>
-__global__ void k_ef_bds(DevGrid g, T* ebf, double dtl) {
+__device__ inline void b_ef_bds(DevGrid g, T* ebf, double dtl) {
   const int nf = g.nfield;
   const int ni = g.ix2 - g.ix1 + 1, nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1;
   const int64_t nxf = (g.ibx < 2) ? (int64_t)nj * nk : 0;
@@ -244,9 +295,16 @@ __global__ void k_ef_bds(DevGrid g, T* ebf, double dtl) {
     }
   }
 }
+template <typename T>
+__global__ void k_ef_bds(DevGrid g, T* ebf, double dtl) { b_ef_bds<T>(g, ebf, dtl); }
+template <typename T>
+__global__ void kb_ef_bds(const FieldRun<T>* runs, int mode) {
+  const FieldRun<T>& r = runs[blockIdx.y];
+  b_ef_bds<T>(r.g, r.ebf, mode ? r.g.dt : 0.0);
+}
 
 template <typename T>
-__global__ void k_bf_bds(DevGrid g, T* ebf, double dtl) {
+__device__ inline void b_bf_bds(DevGrid g, T* ebf, double dtl) {
   const int nf = g.nfield;
   const int ni = g.ix2 - g.ix1 + 1, nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1;
   const int64_t nxf = (g.ibx < 2) ? (int64_t)nj * nk : 0;
@@ -292,11 +350,18 @@ __global__ void k_bf_bds(DevGrid g, T* ebf, double dtl) {
     }
   }
 }
+template <typename T>
+__global__ void k_bf_bds(DevGrid g, T* ebf, double dtl) { b_bf_bds<T>(g, ebf, dtl); }
+template <typename T>
+__global__ void kb_bf_bds(const FieldRun<T>* runs) {
+  const FieldRun<T>& r = runs[blockIdx.y];
+  b_bf_bds<T>(r.g, r.ebf, r.g.dt);
+}
 
 // --------------------------------------------------------------- rote / rotb
 // B -= dtf*curl E on the interior (grid_fields.f90:1405-1478).
 template <typename T>
-__global__ void k_rote(DevGrid g, T* ebf, double dtf) {
+__device__ inline void b_rote(DevGrid g, T* ebf, double dtf) {
   const int nf = g.nfield;
   const int ni = g.ix2 - g.ix1 + 1, nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1;
   const int64_t n = (int64_t)ni * nj * nk;
@@ -327,11 +392,18 @@ __global__ void k_rote(DevGrid g, T* ebf, double dtf) {
     }
   }
 }
+template <typename T>
+__global__ void k_rote(DevGrid g, T* ebf, double dtf) { b_rote<T>(g, ebf, dtf); }
+template <typename T>
+__global__ void kb_rote(const FieldRun<T>* runs) {
+  const FieldRun<T>& r = runs[blockIdx.y];
+  b_rote<T>(r.g, r.ebf, 0.5 * r.g.dt);
+}
 
 // E += dtf*curl B on the interior (grid_fields.f90:1480-1552) followed by the
 // source term E(1:curr_ndim) -= ompe*jc (curr_and_fields_util.f90:233-236).
 template <typename T>
-__global__ void k_rotb_source(DevGrid g, T* ebf, const T* jc, double dtf) {
+__device__ inline void b_rotb_source(DevGrid g, T* ebf, const T* jc, double dtf) {
   const int nf = g.nfield;
   const int ni = g.ix2 - g.ix1 + 1, nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1;
   const int64_t n = (int64_t)ni * nj * nk;
@@ -370,6 +442,15 @@ __global__ void k_rotb_source(DevGrid g, T* ebf, const T* jc, double dtf) {
     ey[o] = vy - ompe * jy[o];
   }
 }
+template <typename T>
+__global__ void k_rotb_source(DevGrid g, T* ebf, const T* jc, double dtf) {
+  b_rotb_source<T>(g, ebf, jc, dtf);
+}
+template <typename T>
+__global__ void kb_rotb_source(const FieldRun<T>* runs) {
+  const FieldRun<T>& r = runs[blockIdx.y];
+  b_rotb_source<T>(r.g, r.ebf, r.jc, r.g.dt);
+}
 
 template <typename T>
 void advance_fields(const DevGrid& g, T* ebf, const T* jc, hipStream_t s) {
@@ -383,6 +464,62 @@ void advance_fields(const DevGrid& g, T* ebf, const T* jc, hipStream_t s) {
   k_rotb_source<T><<<nblocks(nint), kFieldBlock, 0, s>>>(g, ebf, jc, dt_lp);
   k_ef_bds<T><<<nblocks(nface), kFieldBlock, 0, s>>>(g, ebf, dt_lp);
   k_rote<T><<<nblocks(nint), kFieldBlock, 0, s>>>(g, ebf, dth);
+}
+
+// ------------------------------------------------------- batched launches
+BatchDims batch_dims(const DevGrid* gs, int n) {
+  BatchDims d{};
+  for (int r = 0; r < n; r++) {
+    const DevGrid& g = gs[r];
+    const int64_t ni = g.ix2 - g.ix1 + 1, nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1;
+    const int64_t nf = g.nfield, nc = g.curr_ndim;
+    d.fx = std::max(d.fx, nj * nk * nf);
+    d.fy = std::max(d.fy, (ni + 2) * nk * nf);
+    d.fz = std::max(d.fz, (ni + 2) * (nj + 2) * nf);
+    d.jx = std::max(d.jx, nj * nk * nc);
+    d.jy = std::max(d.jy, ni * nk * nc);
+    d.jz = std::max(d.jz, ni * nj * nc);
+    d.ncell = std::max(d.ncell, g.ncell);
+    d.nface = std::max(d.nface, nj * nk + ni * nk + (g.ndim > 2 ? ni * nj : 0));
+    d.nint = std::max(d.nint, ni * nj * nk);
+  }
+  return d;
+}
+
+// x extent of a batched launch: enough workgroups for the largest run, at
+// most 16384 in all (each run's workgroups stride over its own work), so the
+// many small runs of a scan do not each schedule the largest run's grid
+static inline dim3 bgrid(int64_t n, int nruns) {
+  const unsigned cap = (unsigned)std::max(16, 16384 / std::max(nruns, 1));
+  return dim3(std::min(nblocks(n), cap), nruns);
+}
+
+template <typename T>
+void field_xyzbd_batch(const FieldRun<T>* runs, int nruns, int ndim, const BatchDims& d,
+                       hipStream_t s) {
+  kb_field_xbd<T><<<bgrid(d.fx, nruns), kFieldBlock, 0, s>>>(runs);
+  kb_field_ybd<T><<<bgrid(d.fy, nruns), kFieldBlock, 0, s>>>(runs);
+  if (ndim > 2) kb_field_zbd<T><<<bgrid(d.fz, nruns), kFieldBlock, 0, s>>>(runs);
+}
+
+// open boundaries on every face (the only ones a context accepts)
+template <typename T>
+void curr_collect_batch(const FieldRun<T>* runs, int nruns, int ndim, const BatchDims& d,
+                        hipStream_t s) {
+  kb_jc_xfold<T><<<bgrid(d.jx, nruns), kFieldBlock, 0, s>>>(runs);
+  kb_jc_yfold<T><<<bgrid(d.jy, nruns), kFieldBlock, 0, s>>>(runs);
+  if (ndim > 2) kb_jc_zfold<T><<<bgrid(d.jz, nruns), kFieldBlock, 0, s>>>(runs);
+  kb_jc_scale<T><<<bgrid(d.ncell, nruns), kFieldBlock, 0, s>>>(runs);
+}
+
+template <typename T>
+void advance_fields_batch(const FieldRun<T>* runs, int nruns, const BatchDims& d, hipStream_t s) {
+  kb_ef_bds<T><<<bgrid(d.nface, nruns), kFieldBlock, 0, s>>>(runs, 0);
+  kb_rote<T><<<bgrid(d.nint, nruns), kFieldBlock, 0, s>>>(runs);
+  kb_bf_bds<T><<<bgrid(d.nface, nruns), kFieldBlock, 0, s>>>(runs);
+  kb_rotb_source<T><<<bgrid(d.nint, nruns), kFieldBlock, 0, s>>>(runs);
+  kb_ef_bds<T><<<bgrid(d.nface, nruns), kFieldBlock, 0, s>>>(runs, 1);
+  kb_rote<T><<<bgrid(d.nint, nruns), kFieldBlock, 0, s>>>(runs);
 }
 
 template <typename T>
@@ -421,18 +558,24 @@ __global__ __launch_bounds__(kFieldBlock) void k_left_xshift(DevGrid g, T* ebf, 
 }
 
 template <typename T>
-void fields_left_xshift(const DevGrid& g, T* ebf, int shx, hipStream_t s) {
+void fields_left_xshift(const DevGrid& g, T* ebf, int shx, hipStream_t s, int ncomp) {
   if (shx <= 0) return;
-  const int rows = (g.jy2 - g.jy1 + 1) * (g.kz2 - g.kz1 + 1) * g.nfield;
+  if (ncomp <= 0) ncomp = g.nfield;
+  const int rows = (g.jy2 - g.jy1 + 1) * (g.kz2 - g.kz1 + 1) * ncomp;
   k_left_xshift<T><<<rows, kFieldBlock, 0, s>>>(g, ebf, shx);
 }
 
 #define ALADYN_INST_FIELDS(T)                                              \
-  template void fields_left_xshift<T>(const DevGrid&, T*, int, hipStream_t); \
+  template void fields_left_xshift<T>(const DevGrid&, T*, int, hipStream_t, int); \
   template void field_xyzbd<T>(const DevGrid&, T*, int, hipStream_t);      \
   template void curr_collect<T>(const DevGrid&, T*, hipStream_t);          \
   template void advance_fields<T>(const DevGrid&, T*, const T*, hipStream_t); \
-  template void fill<T>(T*, int64_t, T, hipStream_t);
+  template void fill<T>(T*, int64_t, T, hipStream_t);                                        \
+  template void field_xyzbd_batch<T>(const FieldRun<T>*, int, int, const BatchDims&,          \
+                                     hipStream_t);                                             \
+  template void curr_collect_batch<T>(const FieldRun<T>*, int, int, const BatchDims&,         \
+                                      hipStream_t);                                            \
+  template void advance_fields_batch<T>(const FieldRun<T>*, int, const BatchDims&, hipStream_t);
 
 ALADYN_INST_FIELDS(double)
 ALADYN_INST_FIELDS(float)

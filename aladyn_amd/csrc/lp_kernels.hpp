@@ -11,7 +11,33 @@ template <typename T> void field_xyzbd(const DevGrid& g, T* ebf, int nc, hipStre
 template <typename T> void curr_collect(const DevGrid& g, T* jc, hipStream_t s);
 template <typename T> void advance_fields(const DevGrid& g, T* ebf, const T* jc, hipStream_t s);
 template <typename T> void fill(T* a, int64_t n, T v, hipStream_t s);
-template <typename T> void fields_left_xshift(const DevGrid& g, T* ebf, int shx, hipStream_t s);
+// ncomp components of the grid's layout (0: the grid's nfield)
+template <typename T>
+void fields_left_xshift(const DevGrid& g, T* ebf, int shx, hipStream_t s, int ncomp = 0);
+
+// Batched launches: one launch covers the same kernel of many independent
+// runs, blockIdx.y selecting the run.  FieldRun is one run's grids; the
+// BatchDims are the largest per-run work counts, which size the x extent.
+template <typename T>
+struct FieldRun {
+  DevGrid g;
+  T* ebf;
+  T* jc;
+};
+struct BatchDims {
+  int64_t fx, fy, fz;   // field ghost faces (times nfield)
+  int64_t jx, jy, jz;   // current ghost folds (times curr_ndim)
+  int64_t ncell, nface, nint;
+};
+BatchDims batch_dims(const DevGrid* g, int n);
+template <typename T>
+void field_xyzbd_batch(const FieldRun<T>* runs, int nruns, int ndim, const BatchDims& d,
+                       hipStream_t s);
+template <typename T>
+void curr_collect_batch(const FieldRun<T>* runs, int nruns, int ndim, const BatchDims& d,
+                        hipStream_t s);
+template <typename T>
+void advance_fields_batch(const FieldRun<T>* runs, int nruns, const BatchDims& d, hipStream_t s);
 
 // ---------------------------------------------------------- reductions.hip
 template <typename T>

@@ -41,15 +41,6 @@ struct Rec {
   int id;
 };
 
-template <typename R>
-struct StepConst {
-  R alp, qch, dt;
-  R inv[3];
-  int nn[3];
-  int initial;
-  int64_t ovf_base;
-};
-
 // Record layouts (the RecPtr comment in tile_store.hpp):
 //   float : 2 x float4  = [cell, s0, s1, s2 | p0, p1, p2, w]            32 B
 //   double: 4 x double2 = [cell w | s0 | s1 | s2 | p0 | p1 | p2 | 0]     64 B

@@ -30,12 +30,6 @@
 
 namespace aladyn {
 
-struct TileArgs {
-  int nt0, nt1, bx, by;
-  int n1p, n2p, n3p;
-  int64_t ncell;
-};
-
 // Union-window Esirkepov deposit (grid_part_connect.f90:979-1060, 1168-1334)
 // for a particle whose nearest node changed on two or three axes.  Per axis
 // the old and new 3-node windows merge into 4 slots starting at
@@ -501,12 +495,14 @@ __device__ inline void flush_movers(const MoverStage<R, MCAP>& ms, int n, const 
   }
 }
 
+// The workgroup of tile t.
 template <int D, int TX, int TY, int TZ, int NT, typename R>
-__global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
-    TileArgs ta, StepConst<R> sc, const R* __restrict__ ebf, RecPtr A, RecPtr B,
-    const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
-    const int* __restrict__ cnt_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_cur,
-    int* __restrict__ ovf_next, double* __restrict__ patch, int* __restrict__ grow) {
+__device__ inline void tile_body(const TileArgs& ta, const StepConst<R>& sc,
+                                 const R* __restrict__ ebf, const RecPtr& A, const RecPtr& B,
+                                 const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
+                                 const int* __restrict__ cnt_cur, int* __restrict__ cnt_next,
+                                 int* __restrict__ ovf_cur, int* __restrict__ ovf_next,
+                                 double* __restrict__ patch, int* __restrict__ grow, const int t) {
   using S = TileShape<D, TX, TY, TZ, NT, R>;
   constexpr int NCL = S::NCL, NF = S::NF, GX = S::GX, GY = S::GY, GVOL = S::GVOL;
   constexpr int JX = S::JX, JY = S::JY, PVOL = S::PVOL, QCAP = S::QCAP;
@@ -522,7 +518,6 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
   int* wsum = misc + 4;          // per-wave totals of the cell-count scan
   unsigned short* perm = (unsigned short*)(wsum + S::NW);
 
-  const int t = blockIdx.x;
   const int tx = t % ta.nt0, ty = (t / ta.nt0) % ta.nt1, tz = t / (ta.nt0 * ta.nt1);
   const int T0x = 3 + tx * TX, T0y = 3 + ty * TY, T0z = D == 3 ? 3 + tz * TZ : 1;
   const int me = threadIdx.x;
@@ -855,6 +850,28 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
   for (int q = threadIdx.x; q < D * PVOL; q += NT) P[q] = Jl[q];
 }
 
+template <int D, int TX, int TY, int TZ, int NT, typename R>
+__global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
+    TileArgs ta, StepConst<R> sc, const R* __restrict__ ebf, RecPtr A, RecPtr B,
+    const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
+    const int* __restrict__ cnt_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_cur,
+    int* __restrict__ ovf_next, double* __restrict__ patch, int* __restrict__ grow) {
+  tile_body<D, TX, TY, TZ, NT, R>(ta, sc, ebf, A, B, seg_start, seg_cap, cnt_cur, cnt_next,
+                                  ovf_cur, ovf_next, patch, grow, (int)blockIdx.x);
+}
+
+// Many independent runs in one launch: workgroup b takes tile work[b].y of
+// run work[b].x, whose stores and grids the run's descriptor holds.
+template <int D, int TX, int TY, int TZ, int NT, typename R>
+__global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile_batch(
+    const TileRun<R>* __restrict__ runs, const int2* __restrict__ work) {
+  const int2 w = work[blockIdx.x];
+  const TileRun<R>& r = runs[w.x];
+  tile_body<D, TX, TY, TZ, NT, R>(r.ta, r.sc, r.ebf, r.A, r.B, r.seg_start, r.seg_cap,
+                                  r.cnt_cur, r.cnt_next, r.ovf_cur, r.ovf_next, r.patch, r.grow,
+                                  w.y);
+}
+
 // Threads per workgroup: one per node-cell of the tile by default (4
 // wavefronts for an 8x8x4 tile), or two per node-cell (NT = 2 NCL, the
 // aladyn_config.block knob, compiled for the 8x8x4 and 16x16 shapes): twice
@@ -932,6 +949,39 @@ hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepCo
 #undef ALADYN_TILE_CASE
   return hipErrorInvalidConfiguration;
 }
+
+template <int D, int TX, int TY, int TZ, int NT, typename R>
+static hipError_t launch_b(const TileRun<R>* runs, const int2* work, int nwork, hipStream_t s) {
+  k_tile_batch<D, TX, TY, TZ, NT, R><<<nwork, NT, lds_bytes_for<D, TX, TY, TZ, NT, R>(), s>>>(
+      runs, work);
+  return hipGetLastError();
+}
+
+// The batched launch: every run has the geometry kind of tg (dimension, tile
+// shape, threads per tile) and the work list holds nwork (run, tile) pairs.
+template <typename R>
+hipError_t launch_tile_batch(const TileGeom& tg, int ndim, const TileRun<R>* runs,
+                             const int2* work, int nwork, hipStream_t s) {
+  const int* T = tg.T;
+#define ALADYN_TILE_CASE(DD, X, Y, Z, NT)                                                  \
+  if (ndim == DD && T[0] == X && T[1] == Y && (DD == 2 || T[2] == Z) && tg.block == NT)    \
+    return launch_b<DD, X, Y, Z, NT, R>(runs, work, nwork, s);
+  ALADYN_TILE_CASE(3, 8, 8, 8, 512)
+  ALADYN_TILE_CASE(3, 8, 8, 4, 256)
+  if constexpr (sizeof(R) == 4) {
+    ALADYN_TILE_CASE(3, 8, 8, 4, 512)
+    ALADYN_TILE_CASE(3, 16, 8, 4, 512)
+  }
+  ALADYN_TILE_CASE(2, 16, 16, 1, 256)
+  ALADYN_TILE_CASE(2, 16, 16, 1, 512)
+  ALADYN_TILE_CASE(2, 32, 16, 1, 512)
+#undef ALADYN_TILE_CASE
+  return hipErrorInvalidConfiguration;
+}
+template hipError_t launch_tile_batch<float>(const TileGeom&, int, const TileRun<float>*,
+                                             const int2*, int, hipStream_t);
+template hipError_t launch_tile_batch<double>(const TileGeom&, int, const TileRun<double>*,
+                                              const int2*, int, hipStream_t);
 
 template hipError_t launch_tile_kernel<float>(const TileGeom&, const DevGrid&,
                                               const StepConst<float>&, const float*,

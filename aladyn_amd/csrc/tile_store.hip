@@ -16,6 +16,9 @@ namespace aladyn {
 bool tile_shape_supported(int D, int tx, int ty, int tz, int nt, int real_bytes);
 int tile_lds_bytes(int D, int tx, int ty, int tz, int nt, int real_bytes);
 template <typename R>
+hipError_t launch_tile_batch(const TileGeom& tg, int ndim, const TileRun<R>* runs,
+                             const int2* work, int nwork, hipStream_t s);
+template <typename R>
 hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepConst<R>& sc,
                               const R* ebf, const RecPtr& A, const RecPtr& B,
                               const int* seg_start, const int* seg_cap, const int* cnt_cur,
@@ -97,11 +100,12 @@ struct PatchAcc {
 // rounded to the grid's type once like everyone else's), re-bin like the
 // tile kernel.  Runs every step; usually there is nothing to do.
 template <int D, typename R>
-__global__ __launch_bounds__(256) void k_ovf_particles(
-    TileGeom tg, DevGrid g, StepConst<R> sc, const R* __restrict__ ebf, RecPtr A, RecPtr B,
-    const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
-    const int* __restrict__ ovf_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_next,
-    double* __restrict__ patch, int* __restrict__ grow) {
+__device__ inline void ovf_body(const TileGeom& tg, const DevGrid& g, const StepConst<R>& sc,
+                                const R* __restrict__ ebf, const RecPtr& A, const RecPtr& B,
+                                const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
+                                const int* __restrict__ ovf_cur, int* __restrict__ cnt_next,
+                                int* __restrict__ ovf_next, double* __restrict__ patch,
+                                int* __restrict__ grow) {
   const int cnt = *ovf_cur;
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < cnt; n += gridDim.x * blockDim.x) {
     Rec<R> q;
@@ -137,6 +141,24 @@ __global__ __launch_bounds__(256) void k_ovf_particles(
   }
 }
 
+template <int D, typename R>
+__global__ __launch_bounds__(256) void k_ovf_particles(
+    TileGeom tg, DevGrid g, StepConst<R> sc, const R* __restrict__ ebf, RecPtr A, RecPtr B,
+    const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
+    const int* __restrict__ ovf_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_next,
+    double* __restrict__ patch, int* __restrict__ grow) {
+  ovf_body<D, R>(tg, g, sc, ebf, A, B, seg_start, seg_cap, ovf_cur, cnt_next, ovf_next, patch,
+                 grow);
+}
+
+// the same for many runs in one launch, blockIdx.y selecting the run
+template <int D, typename R>
+__global__ __launch_bounds__(256) void kb_ovf_particles(const TileRun<R>* __restrict__ runs) {
+  const TileRun<R>& r = runs[blockIdx.y];
+  ovf_body<D, R>(r.tg, r.g, r.sc, r.ebf, r.A, r.B, r.seg_start, r.seg_cap, r.ovf_cur,
+                 r.cnt_next, r.ovf_next, r.patch, r.grow);
+}
+
 // ------------------------------------------------------------ fold kernel
 // jc(node) [+]= the sum of the tile patches covering node: at most two tiles
 // per axis (T >= 4), the one whose interior holds the node and the neighbour
@@ -145,13 +167,10 @@ __global__ __launch_bounds__(256) void k_ovf_particles(
 // all loaded before they are summed in FP64, in the order of increasing tile
 // index, and the sum is rounded to the grid's type once.
 template <typename R>
-__global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g,
-                                              const double* __restrict__ patch,
-                                              R* __restrict__ jc, int accumulate) {
+__device__ inline void fold_node(const TileGeom& tg, const DevGrid& g,
+                                 const double* __restrict__ patch, R* __restrict__ jc,
+                                 int accumulate, const int idx[3]) {
   const int D = g.ndim;
-  const int idx[3] = {(int)(blockIdx.x * 64 + (threadIdx.x & 63)) + 1,
-                      (int)(blockIdx.y * 4 + (threadIdx.x >> 6)) + 1, (int)blockIdx.z + 1};
-  if (idx[0] > g.n1p || idx[1] > g.n2p) return;
   // per axis: candidate tiles t0 = t1 - 1 and t1, local patch indices, validity
   int tl[3][2], loc[3][2];
   bool ok[3][2];
@@ -194,6 +213,46 @@ __global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g,
     for (int q = 0; q < 8; q++) sum += v[q];
     jc[(int64_t)c * g.ncell + o] = (R)sum;
   }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g,
+                                              const double* __restrict__ patch,
+                                              R* __restrict__ jc, int accumulate) {
+  const int idx[3] = {(int)(blockIdx.x * 64 + (threadIdx.x & 63)) + 1,
+                      (int)(blockIdx.y * 4 + (threadIdx.x >> 6)) + 1, (int)blockIdx.z + 1};
+  if (idx[0] > g.n1p || idx[1] > g.n2p) return;
+  fold_node<R>(tg, g, patch, jc, accumulate, idx);
+}
+
+// jc = the folded patches of many runs in one launch (blockIdx.y = run), one
+// node per thread in storage order
+template <typename R>
+__global__ __launch_bounds__(256) void kb_fold(const TileRun<R>* __restrict__ runs) {
+  const TileRun<R>& r = runs[blockIdx.y];
+  const DevGrid& g = r.g;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < g.ncell;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t jk = q / g.n1p;
+    const int idx[3] = {(int)(q - jk * g.n1p) + 1, (int)(jk % g.n2p) + 1, (int)(jk / g.n2p) + 1};
+    fold_node<R>(r.tg, g, r.patch, r.jc, 0, idx);
+  }
+}
+
+// the counters the step's kernels fill (next buffer's tile counts and
+// overflow count) start at zero
+template <typename R>
+__global__ __launch_bounds__(256) void kb_clear(const TileRun<R>* __restrict__ runs) {
+  const TileRun<R>& r = runs[blockIdx.y];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < r.tg.ntiles; t += gridDim.x * blockDim.x)
+    r.cnt_next[t] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *r.ovf_next = 0;
+}
+
+template <typename R>
+__global__ void kb_ovf_out(const TileRun<R>* __restrict__ runs, int nruns, int* out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nruns) out[r] = *runs[r].ovf_next;
 }
 
 template <typename R>
@@ -1090,6 +1149,79 @@ int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R
 ALADYN_STEP_INST(float)
 ALADYN_STEP_INST(double)
 #undef ALADYN_STEP_INST
+
+// ------------------------------------------------------ batched runs
+template <typename R>
+void tile_store_batch_desc(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf,
+                           R* jc, double* patch, R alp, int initial, TileRun<R>* d) {
+  const int cur = ts->cur, nxt = 1 - cur;
+  d->tg = tg;
+  d->g = g;
+  d->ta = TileArgs{tg.nt[0], tg.nt[1], tg.bx, tg.by, g.n1p, g.n2p, g.n3p, g.ncell};
+  StepConst<R>& sc = d->sc;
+  sc.alp = alp;
+  sc.qch = (R)ts->charge;
+  sc.dt = (R)g.dt;
+  sc.inv[0] = (R)g.dx_inv;
+  sc.inv[1] = (R)g.dy_inv;
+  sc.inv[2] = (R)g.dz_inv;
+  sc.nn[0] = g.nx;
+  sc.nn[1] = g.ny;
+  sc.nn[2] = g.nz;
+  sc.initial = initial;
+  sc.ovf_base = ts->ovf_base;
+  d->ebf = ebf;
+  d->jc = jc;
+  d->A = ts->buf[cur];
+  d->B = ts->buf[nxt];
+  d->seg_start = ts->seg_start;
+  d->seg_cap = ts->seg_cap;
+  d->cnt_cur = ts->count[cur];
+  d->cnt_next = ts->count[nxt];
+  d->ovf_cur = ts->ovf_count + cur;
+  d->ovf_next = ts->ovf_count + nxt;
+  d->patch = patch;
+  d->grow = ts->grow;
+}
+
+template <typename R>
+hipError_t tile_store_batch_launch(const TileGeom& tg0, int ndim, const TileRun<R>* runs,
+                                   int nruns, const int2* work, int nwork, int max_ntiles,
+                                   int64_t max_ncell, int* ovf_out, hipStream_t s) {
+  kb_clear<R><<<dim3(grid_for(max_ntiles, 64), nruns), kB, 0, s>>>(runs);
+  if (hipError_t e = launch_tile_batch<R>(tg0, ndim, runs, work, nwork, s)) return e;
+  if (ndim == 2)
+    kb_ovf_particles<2, R><<<dim3(64, nruns), kB, 0, s>>>(runs);
+  else
+    kb_ovf_particles<3, R><<<dim3(64, nruns), kB, 0, s>>>(runs);
+  // at most 16384 workgroups in all (each run's stride over its own nodes)
+  const int64_t fcap = std::max(16, 16384 / std::max(nruns, 1));
+  kb_fold<R><<<dim3(grid_for(max_ncell, fcap), nruns), kB, 0, s>>>(runs);
+  kb_ovf_out<R><<<(nruns + 63) / 64, 64, 0, s>>>(runs, nruns, ovf_out);
+  return hipGetLastError();
+}
+
+int tile_store_batch_finish(TileStore* ts, const TileGeom& tg, const DevGrid& g, int ovf_prev,
+                            hipStream_t s, std::string* msg) {
+  if (!ts->loaded) return 0;
+  ts->cur = 1 - ts->cur;
+  ts->steps_since_sort++;
+  // the single-run path's pinned overflow slots do not hold this step
+  ts->ovf_valid[0] = ts->ovf_valid[1] = false;
+  const bool relayout = ts->steps_since_sort >= tg.sort_every ||
+                        (ovf_prev >= 0 && ovf_prev > (int)(ts->ovf_cap / 8192 + 2048));
+  return relayout ? tile_store_relayout(ts, tg, g, s, msg) : 0;
+}
+
+#define ALADYN_BATCH_INST(R)                                                                     \
+  template void tile_store_batch_desc<R>(TileStore*, const TileGeom&, const DevGrid&, const R*, \
+                                         R*, double*, R, int, TileRun<R>*);                      \
+  template hipError_t tile_store_batch_launch<R>(const TileGeom&, int, const TileRun<R>*, int,  \
+                                                 const int2*, int, int, int64_t, int*,           \
+                                                 hipStream_t);
+ALADYN_BATCH_INST(float)
+ALADYN_BATCH_INST(double)
+#undef ALADYN_BATCH_INST
 
 // ------------------------------------------------------------ diagnostics
 template <int D, typename R>

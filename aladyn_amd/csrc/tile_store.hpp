@@ -84,6 +84,45 @@ struct TileStore {
   int ovf_slot = 0;
 };
 
+// Constants of one species' step (alp = dt/2 * lorentz_fact, the charge,
+// the cell inverses and counts, the first-step flag, the overflow region).
+template <typename R>
+struct StepConst {
+  R alp, qch, dt;
+  R inv[3];
+  int nn[3];
+  int initial;
+  int64_t ovf_base;
+};
+
+struct TileArgs {
+  int nt0, nt1, bx, by;
+  int n1p, n2p, n3p;
+  int64_t ncell;
+};
+
+// Everything one run's particle kernels of a step read: the batched launches
+// (many independent runs in one launch) take an array of these.  A and B are
+// the buffers read and written this step; cnt/ovf the matching counters.
+template <typename R>
+struct TileRun {
+  TileGeom tg;
+  DevGrid g;
+  TileArgs ta;
+  StepConst<R> sc;
+  const R* ebf;
+  R* jc;
+  RecPtr A, B;
+  const int* seg_start;
+  const int* seg_cap;
+  const int* cnt_cur;
+  int* cnt_next;
+  int* ovf_cur;
+  int* ovf_next;
+  double* patch;
+  int* grow;
+};
+
 int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg);
 const char* tile_geom_error(int code);
 
@@ -140,6 +179,21 @@ int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hip
 template <typename R>
 void tile_store_fold(const TileGeom& tg, const DevGrid& g, const double* patch, R* jc,
                      int accumulate, hipStream_t s);
+// Batched runs: the step's descriptor of one store (the buffers it reads and
+// writes this step), the batched particle-side launches of a step (counter
+// clear, tile kernel over the work list of (run, tile) pairs, overflow
+// kernel, fold of the patches onto jc, the overflow counts into ovf_out),
+// and the host side of one store after such a step (buffer swap, re-layout
+// on cadence or when ovf_prev, the previous step's overflow count, is large).
+template <typename R>
+void tile_store_batch_desc(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf,
+                           R* jc, double* patch, R alp, int initial, TileRun<R>* d);
+template <typename R>
+hipError_t tile_store_batch_launch(const TileGeom& tg0, int ndim, const TileRun<R>* runs,
+                                   int nruns, const int2* work, int nwork, int max_ntiles,
+                                   int64_t max_ncell, int* ovf_out, hipStream_t s);
+int tile_store_batch_finish(TileStore* ts, const TileGeom& tg, const DevGrid& g, int ovf_prev,
+                            hipStream_t s, std::string* msg);
 void tile_store_grid_charge(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* rho,
                             hipStream_t s);
 void tile_store_energy(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* out,

@@ -69,3 +69,18 @@ def run_interleaved(steppers, steps: int, *, start_iter: int = 0, initial_time: 
         first = initial_time and k == 0
         for gpu in steppers:
             gpu.lpf2_evolve(it, initial_time=first)
+
+
+def run_batched(steppers, steps: int, *, start_iter: int = 0, initial_time: bool = True,
+                graphs: bool = True):
+    """Advances independent runs (one LPStep each, one species, same precision
+    and tile configuration) by `steps` lpf2_evolve steps with every kernel of
+    a step launched once for all runs (aladyn_batch_lpf2_evolve: the particle
+    kernel over the work list of every run's tiles), the step's launches
+    replayed as one HIP graph.  The result of each run is that of running it
+    alone."""
+    from .lp_step import LPBatch
+
+    with LPBatch(steppers, graphs=graphs) as b:
+        for k in range(steps):
+            b.lpf2_evolve(start_iter + k, initial_time=initial_time and k == 0)

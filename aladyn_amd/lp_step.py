@@ -171,6 +171,14 @@ class LPStep:
         self._ck(self.lib.aladyn_get_envelope(self._h, _dptr(out)))
         return out
 
+    def env_kernel_timing(self, reset: bool = False):
+        """(push_ms, move_ms, launches): mean durations of the envelope step's
+        two particle kernels (HIP events on the context stream)."""
+        a, b, n = C.c_double(), C.c_double(), C.c_int64()
+        self._ck(self.lib.aladyn_env_kernel_timing(self._h, 1 if reset else 0, C.byref(a),
+                                                   C.byref(b), C.byref(n)))
+        return a.value, b.value, n.value
+
     def env_lpf2_evolve(self, it: int = 0):
         """One env_lpf2_evolve (env_evolve_in_time.f90:196-361) for species 0."""
         self._ck(self.lib.aladyn_env_lpf2_evolve(self._h, int(it)))
@@ -256,3 +264,52 @@ class LPStep:
         self._ck(self.lib.aladyn_kernel_timing(self._h, 1 if reset else 0, _dptr(pm), _dptr(fm),
                                                C.byref(n)))
         return float(pm[0]), float(fm[0])
+
+
+class LPBatch:
+    """Independent LPStep contexts advanced together, each kernel of the step
+    launched once for all of them (aladyn_batch_*, include/aladyn_gpu.h)."""
+
+    def __init__(self, steppers, graphs: bool = True):
+        self.lib = load()
+        self.steppers = list(steppers)
+        arr = (C.c_void_p * len(self.steppers))(*[g._h.value for g in self.steppers])
+        h = C.c_void_p()
+        rc = self.lib.aladyn_batch_create(arr, len(self.steppers), 1 if graphs else 0,
+                                          C.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = self.last_error()
+            self.close()
+            raise AladynError(rc, msg)
+
+    def last_error(self) -> str:
+        if not self._h:
+            return "no batch"
+        s = self.lib.aladyn_batch_last_error(self._h)
+        return s.decode() if s else ""
+
+    def _ck(self, rc: int):
+        if rc != 0:
+            raise AladynError(rc, self.last_error())
+
+    def lpf2_evolve(self, it: int = 0, initial_time: bool = False):
+        self._ck(self.lib.aladyn_batch_lpf2_evolve(self._h, int(it), 1 if initial_time else 0))
+
+    def synchronize(self):
+        self._ck(self.lib.aladyn_batch_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.aladyn_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.synchronize()
+        self.close()

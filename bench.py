@@ -31,8 +31,11 @@ written) for the calibration pass.
 --workload lwfa2d measures the 2-D LWFA configuration instead (512x256,
 458,752 particles, no window, no CPU leg).
 
+--workload env3d measures the envelope model's step (env_lpf2_evolve) on the
+same 3-D plasma with a Gaussian envelope pulse (no CPU leg).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu] [--no-window]
-                       [--workload lwfa3d|lwfa2d] [--graphs]
+                       [--workload lwfa3d|lwfa2d|env3d] [--graphs] [--no-tune]
        python bench.py --calibrate-copy
 """
 from __future__ import annotations
@@ -200,8 +203,13 @@ def cpu_baseline(cfg, part, ebf, budget_s=15.0):
     cores = max(1, min(16, len(os.sched_getaffinity(0))))
     share = min(pool.shape[0] // cores, n)
     _CPU["shares"] = [pool[w * share:(w + 1) * share] for w in range(cores)]
-    with mp.get_context("fork").Pool(cores) as workers:
+    workers = mp.get_context("fork").Pool(cores)
+    try:
         res = workers.map(_cpu_worker, range(cores))
+        workers.close()
+    finally:
+        workers.terminate()
+        workers.join()  # no worker outlives the leg
     many = sum(m * (steps - 1) / (b - a) for m, a, b in res)
     wall = max(b for _, _, b in res)
     return dict(value=many, unit="particle-steps/s", cores=cores, kind="reference",
@@ -211,6 +219,97 @@ def cpu_baseline(cfg, part, ebf, budget_s=15.0):
                        f"step, rates summed ({wall:.1f} s); one core alone: {n} particles, "
                        f"{one:.3g} particle-steps/s. Gather+push+Esirkepov, FP64, reference "
                        f"Fortran routines compiled -O2 -fdefault-real-8")
+
+
+def envelope_pulse(cfg, a0=3.0, w0=10.0, length=2.0, xc=None):
+    """A Gaussian envelope A_R = a0 exp(-((x-xc)/length)^2 - (r/w0)^2) on every
+    node, A_I = 0 and A^{n-1} = A^n: env(n1p, n2p, n3p, 4)."""
+    import numpy as np
+
+    xc = 0.25 * (cfg.xmn + cfg.xmx) if xc is None else xc
+    x = cfg.xmn + (np.arange(cfg.n1p) - 2.0) * cfg.dx
+    y = cfg.ymn + (np.arange(cfg.n2p) - 2.0) * cfg.dy
+    z = cfg.zmn + (np.arange(cfg.n3p) - 2.0) * cfg.dz if cfg.ndim == 3 else np.zeros(1)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    a = a0 * np.exp(-((X - xc) / length) ** 2 - (Y * Y + Z * Z) / (w0 * w0))
+    env = np.zeros((cfg.n1p, cfg.n2p, cfg.n3p, 4), order="F")
+    env[..., 0] = a
+    env[..., 2] = a
+    return env
+
+
+def run_env(cfg, part, env, precision, steps, warmup):
+    """Times `steps` env_lpf2_evolve calls after `warmup` untimed ones (the
+    envelope model's step, species 0), wake fields starting at zero."""
+    import numpy as np
+
+    from aladyn_amd import LPStep
+
+    oml = 2.0 * np.pi / cfg.lam0
+    gpu = LPStep(cfg, precision)
+    gpu.put_species(0, part)
+    gpu.put_fields(np.zeros(cfg.field_shape, order="F"))
+    gpu.put_envelope(env, oml)
+    for it in range(warmup):
+        gpu.env_lpf2_evolve(it)
+    gpu.synchronize()
+    n0 = gpu.species_count(0)
+    gpu.env_kernel_timing(reset=True)
+    t0 = time.perf_counter()
+    for it in range(warmup, warmup + steps):
+        gpu.env_lpf2_evolve(it)
+    gpu.synchronize()
+    wall = time.perf_counter() - t0
+    push_ms, move_ms, _ = gpu.env_kernel_timing()
+    n1 = gpu.species_count(0)
+    gpu.close()
+    return {"ms_per_step": 1e3 * wall / steps, "pps": 0.5 * (n0 + n1) * steps / wall,
+            "push_ms": push_ms, "move_ms": move_ms, "n0": n0, "n1": n1}
+
+
+def bench_env(args):
+    """The envelope model's step on the 3-D LWFA plasma (--workload env3d)."""
+    from aladyn_amd import PREC_F32, PREC_F64
+    from aladyn_amd.inputs import lwfa_3d_case
+
+    cfg, part, _ = lwfa_3d_case(args.nx, args.ny, args.nz, seed=0)
+    env = envelope_pulse(cfg)
+    r32 = run_env(cfg, part, env, PREC_F32, args.steps, args.warmup)
+    log(f"env fp32: {r32['ms_per_step']:.3f} ms/step, push {r32['push_ms']:.3f} ms, "
+        f"move {r32['move_ms']:.3f} ms")
+    r64 = None if args.no_fp64 else run_env(cfg, part, env, PREC_F64, args.steps, args.warmup)
+    n_p, cells = 0.5 * (r32["n0"] + r32["n1"]), cfg.nx * cfg.ny * cfg.nz
+    # algorithmic bytes, FP32: push reads the record (32 B) and writes p
+    # (12 B), gathers E, B and |A|^2/2 with its gradient once per cell (40 B)
+    # and deposits the FP64 density (8 B/cell); move reads the record and
+    # writes it to its tile (32 + 32 B), gathers |A|^2/2 and its gradient (16
+    # B/cell) and writes J (12 B/cell)
+    push_b, move_b = n_p * 44 + cells * 48, n_p * 64 + cells * 28
+    k_ms = max(r32["push_ms"], r32["move_ms"])
+    k_b = push_b if r32["push_ms"] >= r32["move_ms"] else move_b
+    achieved = k_b / (k_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC + " (envelope model step)",
+        "value": r32["pps"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r32["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded plasma + Gaussian envelope pulse, wake fields from zero)",
+        "config": {"workload": f"3-D envelope (ponderomotive) LWFA {cfg.nx}x{cfg.ny}x{cfg.nz}, "
+                               f"dx=0.05 dy=dz=0.2, n/nc=0.005, 8 ppc, envelope a0=3 w0=10, "
+                               f"env_lpf2_evolve",
+                   "particles": int(r32["n0"]), "cells": cells},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_env_push<3, float>" if k_b == push_b else "k_env_move<3, float>",
+                     "kernel_ms": k_ms, "algorithmic_bytes_per_launch": k_b},
+        "kernels_ms": {"k_env_push": r32["push_ms"], "k_env_move": r32["move_ms"]},
+        "cpu_baseline": None,
+    }
+    if r64:
+        out["fp64"] = {"value": r64["pps"], "ms_per_step": r64["ms_per_step"],
+                       "push_ms": r64["push_ms"], "move_ms": r64["move_ms"]}
+        out["fp32_over_fp64"] = r32["pps"] / r64["pps"]
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -225,9 +324,10 @@ def main():
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--no-window", action="store_true")
-    ap.add_argument("--workload", choices=("lwfa3d", "lwfa2d"), default="lwfa3d",
+    ap.add_argument("--workload", choices=("lwfa3d", "lwfa2d", "env3d"), default="lwfa3d",
                     help="lwfa3d: the 3-D LWFA configuration (default); lwfa2d: the 2-D "
-                         "LWFA configuration, 512x256 (no window, CPU leg skipped)")
+                         "LWFA configuration, 512x256 (no window, CPU leg skipped); env3d: "
+                         "the envelope model's step on the 3-D LWFA plasma")
     ap.add_argument("--sort-every", type=int, default=0,
                     help="steps between re-layouts of the particle store (0: the library's "
                          "default)")
@@ -245,6 +345,10 @@ def main():
               "and has no multi-GPU exchange (the step would shard by spatial domain, as "
               "the reference's MPI decomposition does)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "env3d":
+        bench_env(args)
+        return
 
     from aladyn_amd import PREC_F32, PREC_F64
     from aladyn_amd.inputs import lwfa_2d_case, lwfa_3d_case, plasma_table

@@ -121,6 +121,25 @@ int aladyn_set_dt(aladyn_ctx *ctx, double dt);
  * time graph steps. */
 int aladyn_set_graphs(aladyn_ctx *ctx, int32_t enable);
 
+/* Many independent runs stepped together (density or a0 scans).  No
+ * counterpart in the reference, which runs one simulation per process.  The
+ * contexts (each put with one species and its fields, all of one precision,
+ * dimension, tile shape and threads per tile, on one device) keep their own
+ * grids and stores; aladyn_batch_lpf2_evolve advances every one of them by
+ * one lpf2_evolve with each kernel of the step launched once for all runs:
+ * the particle kernel over the work list of every run's tiles, the grid
+ * kernels with one row of workgroups per run.  With use_graphs != 0 the
+ * step's launches are captured once and replayed as a HIP graph.  Steps are
+ * queued on the batch's own stream: call aladyn_batch_synchronize before
+ * using a context alone again.  Destroying the batch leaves the contexts. */
+typedef struct aladyn_batch aladyn_batch;
+int aladyn_batch_create(aladyn_ctx *const *ctxs, int32_t n, int32_t use_graphs,
+                        aladyn_batch **out);
+int aladyn_batch_lpf2_evolve(aladyn_batch *batch, int32_t iter, int32_t initial_time);
+int aladyn_batch_synchronize(aladyn_batch *batch);
+int aladyn_batch_destroy(aladyn_batch *batch);
+const char *aladyn_batch_last_error(const aladyn_batch *batch);
+
 /* The stages of the step, for drivers that interleave their own work:
  * pfields_prepare(ebf, nfield, 2, 2) (curr_and_fields_util.f90:156-174) */
 int aladyn_pfields_prepare(aladyn_ctx *ctx);
@@ -247,6 +266,11 @@ int aladyn_synchronize(aladyn_ctx *ctx);
  * events on the context stream. */
 int aladyn_kernel_timing(aladyn_ctx *ctx, int32_t reset, double *particle_ms,
                          double *field_ms, int64_t *launches);
+/* The same for the envelope step's two particle kernels: the gather + push +
+ * density deposit (push_ms) and the gradient gather + position update +
+ * Esirkepov deposit + re-binning (move_ms). */
+int aladyn_env_kernel_timing(aladyn_ctx *ctx, int32_t reset, double *push_ms, double *move_ms,
+                             int64_t *launches);
 
 #ifdef __cplusplus
 }

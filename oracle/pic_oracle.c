@@ -1076,10 +1076,25 @@ void orc_field_energy(const orc_grid *g, const double *ebf, double out[6]) {
   }
 }
 
+void orc_fields_left_xshift(const orc_grid *g, double *fld, int ncomp, int nshx) {
+  /* fields_left_xshift(fld, i1, i2, 1, ncomp, nshx), one rank
+   * (mpi_field_interface.f90:299-359): (i1+nshx : i2+nshx) -> (i1 : i2) on the
+   * interior rows, then the last nshx columns zero */
+  if (nshx <= 0) return;
+  const int i1 = g->ix1, i2 = g->ix2 - nshx;
+  for (int ic = 1; ic <= ncomp; ic++)
+    for (int iz = g->kz1; iz <= g->kz2; iz++)
+      for (int iy = g->jy1; iy <= g->jy2; iy++) {
+        for (int ix = i1; ix <= i2; ix++) F4(fld, ix, iy, iz, ic) = F4(fld, ix + nshx, iy, iz, ic);
+        for (int ix = i2 + 1; ix <= i2 + nshx; ix++) F4(fld, ix, iy, iz, ic) = 0.0;
+      }
+}
+
 int orc_window_xshift(orc_grid *g, orc_window *w, int witr, int init_iter,
                       double w_speed, double *ebf, int64_t *np, double *part,
                       int64_t ld) {
-  /* lp_window_xshift (window.f90:276-347), one rank, no fluid/envelope */
+  /* lp_window_xshift (window.f90:276-347), one rank, no fluid; the envelope
+   * arrays are shifted by the caller with orc_fields_left_xshift (:327-331) */
   if (init_iter == 0) {
     w->xlapse = 0.0;
     w->wi1 = 0;
@@ -1091,18 +1106,8 @@ int orc_window_xshift(orc_grid *g, orc_window *w, int witr, int init_iter,
   w->wi1 = wi2;
   g->xmn = g->xmn + g->dx * nshx;
   g->xmx = g->xmx + g->dx * nshx;
-  const int i1 = g->ix1, i2 = g->ix2 - nshx;
-  if (i2 <= 0) return -1;
-  if (nshx > 0) {
-    /* fields_left_xshift(ebf, i1, i2, 1, nfield, nshx) */
-    for (int ic = 1; ic <= g->nfield; ic++)
-      for (int iz = g->kz1; iz <= g->kz2; iz++)
-        for (int iy = g->jy1; iy <= g->jy2; iy++) {
-          for (int ix = i1; ix <= i2; ix++)
-            F4(ebf, ix, iy, iz, ic) = F4(ebf, ix + nshx, iy, iz, ic);
-          for (int ix = i2 + 1; ix <= i2 + nshx; ix++) F4(ebf, ix, iy, iz, ic) = 0.0;
-        }
-  }
+  if (g->ix2 - nshx <= 0) return -1; /* wi2 = n1p - nshx <= 0 (window.f90:314-319) */
+  orc_fields_left_xshift(g, ebf, g->nfield, nshx);
   /* cell_part_dist(.true.): x only, (xmn, xmx], stable */
   const int ndv = 2 * g->curr_ndim + 1;
   int64_t p = 0;

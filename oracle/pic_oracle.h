@@ -144,6 +144,8 @@ typedef struct orc_window {
  * are removed keeping the order (cell_part_dist(.true.) ->
  * reset_all_part_dist on x, mpi_part_interface.f90:467-523).  Returns nshx,
  * or -1 if nshx >= ix2 (the reference's ier = 2); *np is updated. */
+/* fields_left_xshift (mpi_field_interface.f90:299-359) of ncomp components */
+void orc_fields_left_xshift(const orc_grid *g, double *fld, int ncomp, int nshx);
 int orc_window_xshift(orc_grid *g, orc_window *w, int witr, int init_iter,
                       double w_speed, double *ebf, int64_t *np, double *part,
                       int64_t ld);

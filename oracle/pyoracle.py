@@ -100,6 +100,7 @@ class Oracle:
             "orc_field_energy": [_G, _D, _D],
             "orc_window_xshift": [_G, C.POINTER(OrcWindow), C.c_int, C.c_int, C.c_double, _D,
                                   C.POINTER(C.c_int64), _D, _I64],
+            "orc_fields_left_xshift": [_G, _D, C.c_int, C.c_int],
             "orc_inject_select": [_D, _I64, C.POINTER(C.c_int64), C.c_double, C.c_double,
                                   C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
             "orc_add_particles": [_G, _D, _I64, _I64, _D, C.c_int, _D, C.c_int, _D, _D, C.c_int,
@@ -256,6 +257,12 @@ class Oracle:
         nshx = self.L.orc_window_xshift(C.byref(self.g), C.byref(win), witr, init_iter, w_speed,
                                         _p(_f(ebf)), C.byref(n), _p(_f(part)), self._ld(part))
         return int(nshx), np.asfortranarray(part[:n.value])
+
+    def fields_left_xshift(self, fld, nshx):
+        """fields_left_xshift of every component of fld (n1p, n2p, n3p, nc),
+        in place: what lp_window_xshift does to the envelope arrays."""
+        assert fld.flags.f_contiguous and fld.dtype == np.float64
+        self.L.orc_fields_left_xshift(C.byref(self.g), _p(fld), fld.shape[3], nshx)
 
     def inject_select(self, xpt, nptx, xmx, targ_in, targ_end):
         """particles_inject's column selection: returns (ncols, c0, c1, new nptx)."""

@@ -23,7 +23,7 @@ from aladyn_amd.config import LPConfig
 from aladyn_amd.inputs import uniform_plasma
 from cases import random_fields
 from conftest import rel_maxerr
-from oracle.pyoracle import Oracle
+from oracle.pyoracle import Oracle, OrcWindow
 
 pytestmark = pytest.mark.gpu
 
@@ -103,6 +103,36 @@ def test_env_step_high_order_matches_oracle(der_ord):
     print(f"envelope 2-D der_ord={der_ord}", flush=True)
     cfg, part, ebf, env = _case(2, der_ord)
     _check(cfg, _run(cfg, part, ebf, env, PREC_F64, 3), 1e-10)
+
+
+def test_env_steps_with_moving_window_match_oracle():
+    """lp_window_xshift after each envelope step: the envelope's four
+    components shift with the fields (window.f90:327-331)."""
+    cfg, part, ebf, env = _case(2)
+    orc = Oracle(cfg)
+    orc.set_om0(OML)
+    win = OrcWindow()
+    p, e, a = part.copy(order="F"), ebf.copy(order="F"), env.copy(order="F")
+    jc = np.zeros((cfg.n1p, cfg.n2p, cfg.n3p, cfg.curr_ndim + 1), order="F")
+    shifts = []
+    with LPStep(cfg, PREC_F64, track_ids=True) as gpu:
+        gpu.put_species(0, part)
+        gpu.put_fields(ebf)
+        gpu.put_envelope(env, OML)
+        orc.window_xshift(win, 1, 0, 2.0, e, p)
+        gpu.lp_window_xshift(1, 0, 2.0)
+        for it in range(1, 6):
+            p = orc.env_lpf2_evolve(p, e, a, jc)
+            gpu.env_lpf2_evolve(it)
+            nsh, p = orc.window_xshift(win, 1, it, 2.0, e, p)
+            orc.fields_left_xshift(a, nsh)
+            assert gpu.lp_window_xshift(1, it, 2.0) == nsh
+            shifts.append(nsh)
+        r = dict(p=p, e=e, a=a, j=jc[..., :cfg.curr_ndim], pg=gpu.get_species(0),
+                 eg=gpu.get_fields(), ag=gpu.get_envelope(), jg=gpu.get_current())
+    print("shifts", shifts, flush=True)
+    assert sum(shifts) >= 3
+    _check(cfg, r, 1e-10)
 
 
 def test_env_step_needs_the_envelope():

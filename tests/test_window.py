@@ -39,6 +39,13 @@ def test_oracle_window_shift_moves_fields_box_and_particles():
     assert np.all(ebf[i2 + 1 - nsh:i2 + 1, j] == 0.0)
     assert np.array_equal(ebf[:i1], before[:i1]) and np.array_equal(ebf[i2 + 1:], before[i2 + 1:])
     assert np.array_equal(ebf[:, :cfg.jy1 - 1], before[:, :cfg.jy1 - 1])
+    # the envelope's four components shift the same way (window.f90:327-331)
+    env = np.asfortranarray(rng.normal(size=(cfg.n1p, cfg.n2p, cfg.n3p, 4)))
+    env0 = env.copy()
+    orc.fields_left_xshift(env, nsh)
+    assert np.array_equal(env[i1:i2 + 1 - nsh, j], env0[i1 + nsh:i2 + 1, j])
+    assert np.all(env[i2 + 1 - nsh:i2 + 1, j] == 0.0)
+    assert np.array_equal(env[:i1], env0[:i1]) and np.array_equal(env[i2 + 1:], env0[i2 + 1:])
     assert orc.g.xmn == pytest.approx(x0 + nsh * cfg.dx, abs=1e-15)
     xmn = orc.g.xmn
     inside = part[:, 0] > xmn
