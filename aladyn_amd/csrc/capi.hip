@@ -642,8 +642,9 @@ static int grid_charge_dev(aladyn_ctx* c, int ic, double* rho) {
   return 0;
 }
 
-extern "C" int aladyn_particle_stage(aladyn_ctx* c, int32_t ic, int32_t initial_time) {
-  if (int r = check_species(c, ic)) return r;
+// One species' stage; with normalise its current goes onto jc times djc
+// (the fused step, whose collect then skips the normalisation).
+static int particle_stage(aladyn_ctx* c, int ic, int initial_time, bool normalise) {
   if (!c->have_fields) return fail(c, ALADYN_ESTATE, "particle_stage before put_fields");
   const DevGrid& g = c->g;
   const double alp = 0.5 * g.dt * c->cfg.lorentz_fact[ic];
@@ -661,10 +662,10 @@ extern "C" int aladyn_particle_stage(aladyn_ctx* c, int32_t ic, int32_t initial_
   const int r =
       f64(c) ? tile_store_step<double>(&c->ts[ic], c->tg, g, c->ebf64, c->jc64, c->patch, alp,
                                        init, c->j_is_zero, jraw, c->stream, c->tpart.begin(),
-                                       c->tpart.end(), &msg)
+                                       c->tpart.end(), &msg, normalise)
              : tile_store_step<float>(&c->ts[ic], c->tg, g, c->ebf32, c->jc32, c->patch,
                                       (float)alp, init, c->j_is_zero, jraw, c->stream,
-                                      c->tpart.begin(), c->tpart.end(), &msg);
+                                      c->tpart.begin(), c->tpart.end(), &msg, normalise);
   if (r) return fail(c, r, "%s", msg.c_str());
   if (c->ts[ic].loaded) c->tpart.commit();
   c->j_is_zero = false;
@@ -676,14 +677,23 @@ extern "C" int aladyn_particle_stage(aladyn_ctx* c, int32_t ic, int32_t initial_
   return ALADYN_OK;
 }
 
-extern "C" int aladyn_curr_mpi_collect(aladyn_ctx* c) {
-  if (!c) return ALADYN_EINVAL;
+extern "C" int aladyn_particle_stage(aladyn_ctx* c, int32_t ic, int32_t initial_time) {
+  if (int r = check_species(c, ic)) return r;
+  return particle_stage(c, ic, initial_time, false);
+}
+
+static int curr_mpi_collect(aladyn_ctx* c, bool normalised) {
   if (f64(c))
-    curr_collect<double>(c->g, c->jc64, c->stream);
+    curr_collect<double>(c->g, c->jc64, c->stream, normalised);
   else
-    curr_collect<float>(c->g, c->jc32, c->stream);
+    curr_collect<float>(c->g, c->jc32, c->stream, normalised);
   HIPCK(c, hipGetLastError());
   return ALADYN_OK;
+}
+
+extern "C" int aladyn_curr_mpi_collect(aladyn_ctx* c) {
+  if (!c) return ALADYN_EINVAL;
+  return curr_mpi_collect(c, false);
 }
 
 extern "C" int aladyn_advance_lpf_fields(aladyn_ctx* c) {
@@ -732,17 +742,17 @@ static int enqueue_step(aladyn_ctx* c) {
     const int r =
         f64(c) ? tile_store_step_enqueue<double>(&c->ts[ic], c->tg, g, c->ebf64, c->jc64,
                                                  c->patch, alp, 0, ic == 0, nullptr, c->stream,
-                                                 nullptr, nullptr, &msg)
+                                                 nullptr, nullptr, &msg, true)
                : tile_store_step_enqueue<float>(&c->ts[ic], c->tg, g, c->ebf32, c->jc32,
                                                 c->patch, (float)alp, 0, ic == 0, nullptr,
-                                                c->stream, nullptr, nullptr, &msg);
+                                                c->stream, nullptr, nullptr, &msg, true);
     if (r) return fail(c, r, "%s", msg.c_str());
   }
   if (f64(c)) {
-    curr_collect<double>(g, c->jc64, c->stream);
+    curr_collect<double>(g, c->jc64, c->stream, true);
     advance_fields<double>(g, c->ebf64, c->jc64, c->stream);
   } else {
-    curr_collect<float>(g, c->jc32, c->stream);
+    curr_collect<float>(g, c->jc32, c->stream, true);
     advance_fields<float>(g, c->ebf32, c->jc32, c->stream);
   }
   HIPCK(c, hipGetLastError());
@@ -818,9 +828,10 @@ extern "C" int aladyn_lpf2_evolve(aladyn_ctx* c, int32_t iter, int32_t initial_t
   } else if (int r = aladyn_curr_clean(c)) {
     return r;
   }
+  // the folds apply djc, so the collect does not
   for (int ic = 0; ic < c->cfg.nsp; ic++)
-    if (int r = aladyn_particle_stage(c, ic, initial_time)) return r;
-  if (int r = aladyn_curr_mpi_collect(c)) return r;
+    if (int r = particle_stage(c, ic, initial_time, true)) return r;
+  if (int r = curr_mpi_collect(c, true)) return r;
   HIPCK(c, hipEventRecord(c->tfield.begin(), c->stream));
   if (int r = aladyn_advance_lpf_fields(c)) return r;
   HIPCK(c, hipEventRecord(c->tfield.end(), c->stream));
@@ -867,11 +878,11 @@ static int env_step(aladyn_ctx* c, R* ebf, R* jc, R* env, R* je) {
   std::string msg;
   // jc = 0; jc(1) = |A^n|^2/2, jc(2:D+1) its gradient (env_amp_prepare)
   HIPCK(c, env_amp_prepare<R>(g, ec, env, je, c->stream));
-  fill<double>(c->den, g.ncell, 0.0, c->stream);
-  // set_env_acc, lpf_env_momenta, set_env_density for species 1
+  // set_env_acc, lpf_env_momenta, set_env_density for species 1 (den is
+  // written whole, zero without particles)
   const bool timed = c->ts[0].loaded;
-  env_particles_push<R>(&c->ts[0], c->tg, g, ebf, je, c->den, c->cfg.lorentz_fact[0], c->stream,
-                        timed ? c->tenv_push.begin() : nullptr,
+  env_particles_push<R>(&c->ts[0], c->tg, g, ebf, je, c->den, c->patch,
+                        c->cfg.lorentz_fact[0], c->stream, timed ? c->tenv_push.begin() : nullptr,
                         timed ? c->tenv_push.end() : nullptr);
   HIPCK(c, hipGetLastError());
   if (timed) c->tenv_push.commit();
@@ -884,7 +895,7 @@ static int env_step(aladyn_ctx* c, R* ebf, R* jc, R* env, R* je) {
     return fail(c, r, "%s", msg.c_str());
   if (timed) c->tenv_move.commit();
   // curr_mpi_collect, advance_lpf_fields (env_evolve_in_time.f90:345-358)
-  curr_collect<R>(g, jc, c->stream);
+  curr_collect<R>(g, jc, c->stream, true);
   advance_fields<R>(g, ebf, jc, c->stream);
   HIPCK(c, hipGetLastError());
   c->j_is_zero = false;

@@ -30,12 +30,14 @@ hipError_t env_amp_prepare(const DevGrid& g, const EnvConst& ec, const R* env, R
                            hipStream_t s);
 
 // The particles' first half (set_env_acc, lpf_env_momenta, set_env_density)
-// on species store ts: momenta to p^{n+1/2} in place, den += wgh/gamp shapes.
-// The events, when given, are recorded around the particle kernel.
+// on species store ts: momenta to p^{n+1/2} in place, den = the wgh/gamp
+// shapes (summed per tile in the patch array, then folded).  The events,
+// when given, are recorded around the particle kernels and the fold.
 template <typename R>
 void env_particles_push(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf,
-                        const R* je, double* den, double lorentz_fact, hipStream_t s,
-                        hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+                        const R* je, double* den, double* patch, double lorentz_fact,
+                        hipStream_t s, hipEvent_t ev_begin = nullptr,
+                        hipEvent_t ev_end = nullptr);
 
 // den_zyxbd, je(1) = je(3) = den, advance_lpf_envelope (A^n -> A^{n+1}) and
 // env_fields_average (je(1:D+1) = F at t^{n+1/2} and its gradient).
@@ -45,7 +47,9 @@ hipError_t env_grid_advance(const DevGrid& g, const EnvConst& ec, R* env, R* je,
 
 // The particles' second half (set_env_grad_interp, lpf_env_positions,
 // cell_part_dist, Esirkepov deposit) and the re-binning of the store; jc =
-// the folded raw current.  Ends with the store's host-side step bookkeeping.
+// djc x the folded current (the normalisation of curr_mpi_collect, which the
+// collect that follows then skips).  Ends with the store's host-side step
+// bookkeeping.
 template <typename R>
 int env_particles_move(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* je, R* jc,
                        double* patch, hipStream_t s, std::string* msg,

@@ -68,25 +68,13 @@ __device__ inline const R* comp_c(const R* a, const DevGrid& g, int c) {
   return a + (int64_t)c * g.ncell;
 }
 
-// iterates over the live records of A: tile t's segment for blocks t <
-// ntiles, the overflow list for the rest
-template <typename R, typename F>
-__device__ inline void for_live(const TileGeom& tg, const RecPtr& A, const int* seg_start,
-                                const int* seg_cap, const int* cnt, int64_t ovf_base,
-                                const int* ovf_cnt, F&& f) {
-  const int t = blockIdx.x;
-  if (t < tg.ntiles) {
-    const int n = min(cnt[t], seg_cap[t]);
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-      const int64_t src = (int64_t)seg_start[t] + q;
-      if (rec_cell<R>(A, src) != kHole) f(src);
-    }
-  } else {
-    const int n = *ovf_cnt;
-    for (int q = (t - tg.ntiles) * blockDim.x + threadIdx.x; q < n;
-         q += (gridDim.x - tg.ntiles) * blockDim.x)
-      f(ovf_base + q);
-  }
+// iterates over the records of the overflow list (the tiles' segments are
+// the tile workgroups' below)
+template <typename F>
+__device__ inline void for_overflow(int64_t ovf_base, const int* ovf_cnt, F&& f) {
+  const int n = *ovf_cnt;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+    f(ovf_base + q);
 }
 
 // ---------------------------------------------------- particles, phase 1
@@ -95,14 +83,14 @@ __device__ inline void for_live(const TileGeom& tg, const RecPtr& A, const int* 
 // per particle: E, B, F = je(1) and grad F = je(2:D+1) at x^n with qqh
 // shapes; gamp from p^{n-1/2}, F and the half-step energy change; the
 // rotation with gamp fixed gives p^{n+1/2} (written back in place); den +=
-// (wgh/gamp) x the quadratic shapes at x^n.
+// (wgh/gamp) x the quadratic shapes at x^n.  This kernel takes the overflow
+// list, reading the grids in HBM and adding to den with global atomics.
 template <int D, typename R>
 __global__ __launch_bounds__(kEB) void k_env_push(TileGeom tg, DevGrid g, RecPtr A,
-                                                  const int* seg_start, const int* seg_cap,
-                                                  const int* cnt, int64_t ovf_base,
-                                                  const int* ovf_cnt, const R* ebf, const R* je,
-                                                  R qch, R dth, R alp, double* den) {
-  for_live<R>(tg, A, seg_start, seg_cap, cnt, ovf_base, ovf_cnt, [&](int64_t n) {
+                                                  int64_t ovf_base, const int* ovf_cnt,
+                                                  const R* ebf, const R* je, R qch, R dth, R alp,
+                                                  double* den) {
+  for_overflow(ovf_base, ovf_cnt, [&](int64_t n) {
     Rec<R> q;
     load_rec<D, R>(A, n, q, tg.bx, tg.by);
     R ax[3], axh[3], ay[3], ayh[3], az[3] = {R(1), R(0), R(0)}, azh[3] = {R(1), R(0), R(0)};
@@ -208,15 +196,15 @@ struct EnvPatchAcc {
 // (grid_part_connect.f90:979-1060, 1168-1334) + re-binning into B: grad F
 // and F = je(1:D+1) at x^n with qqh shapes; gamp at t^{n+1/2} corrected by
 // grad F; x^{n+1} = x^n + dt p/gamp; particles leaving the box are dropped
-// before they deposit.
+// before they deposit.  This kernel takes the overflow list (global atomics
+// into the patches the tile workgroups wrote).
 template <int D, typename R>
 __global__ __launch_bounds__(kEB) void k_env_move(TileGeom tg, DevGrid g, StepConst<R> sc,
                                                   RecPtr A, RecPtr B, const int* seg_start,
-                                                  const int* seg_cap, const int* cnt,
-                                                  const int* ovf_cur, const R* je,
-                                                  int* cnt_next, int* ovf_next, double* patch,
-                                                  int* grow) {
-  for_live<R>(tg, A, seg_start, seg_cap, cnt, sc.ovf_base, ovf_cur, [&](int64_t n) {
+                                                  const int* seg_cap, const int* ovf_cur,
+                                                  const R* je, int* cnt_next, int* ovf_next,
+                                                  double* patch, int* grow) {
+  for_overflow(sc.ovf_base, ovf_cur, [&](int64_t n) {
     Rec<R> q;
     load_rec<D, R>(A, n, q, tg.bx, tg.by);
     R ax[3], axh[3], ay[3], ayh[3], az[3] = {R(1), R(0), R(0)}, azh[3] = {R(1), R(0), R(0)};
@@ -265,6 +253,266 @@ __global__ __launch_bounds__(kEB) void k_env_move(TileGeom tg, DevGrid g, StepCo
     }
     store_rec<D, R>(B, dst, q, tg.bx, tg.by);
   });
+}
+
+// ------------------------------------------------- tile workgroups
+// The same two phases with one workgroup per tile of the store: the tile's
+// fields (ebf and je, or je alone) are staged in LDS over nodes T0-2 ..
+// T0+T per axis (the reach of the qqh shapes of particles whose nearest
+// node lies in the tile's nodes T0 .. T0+T-1), and the density or current
+// is summed in an FP64 LDS patch over nodes T0-2 .. T0+T+1 (the current
+// patch's extent tg.J, so that the fold of the LP step sums it).  Nodes
+// of the staged box beyond the grid (the last tile per axis may reach past
+// it) are zero and no shape reaches them.  The overflow list is left to
+// the global-memory kernels above, launched after these.
+struct EnvTile {
+  int T0[3];  // first node of the tile per axis (1-based; 1 for z in 2-D)
+  int G[3];   // staged box extent: T + 3 (1 for z in 2-D)
+  int gvol;
+};
+
+template <int D>
+__device__ inline EnvTile env_tile(const TileGeom& tg, int t) {
+  EnvTile e;
+  const int tt[3] = {t % tg.nt[0], (t / tg.nt[0]) % tg.nt[1], t / (tg.nt[0] * tg.nt[1])};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    e.T0[d] = d < D ? 3 + tt[d] * tg.T[d] : 1;
+    e.G[d] = d < D ? tg.T[d] + 3 : 1;
+  }
+  e.gvol = e.G[0] * e.G[1] * e.G[2];
+  return e;
+}
+
+// Stages component c of src (ncell stride) over the tile's box into dst.
+template <int D, typename R>
+__device__ inline void env_stage(const DevGrid& g, const EnvTile& e, const R* src, R* dst) {
+  for (int l = threadIdx.x; l < e.gvol; l += blockDim.x) {
+    const int li = l % e.G[0], lj = (l / e.G[0]) % e.G[1], lk = l / (e.G[0] * e.G[1]);
+    const int i = e.T0[0] - 2 + li, j = e.T0[1] - 2 + lj, k = D == 3 ? e.T0[2] - 2 + lk : 1;
+    dst[l] = (i <= g.n1p && j <= g.n2p && k <= g.n3p) ? src[fofs(g, i, j, k)] : R(0);
+  }
+}
+
+// contract3 over an LDS box (x stride 1, y stride GX, z stride GXY), the
+// same loop order
+template <int D, typename R>
+__device__ inline R contract3l(const R* f, int GX, int GXY, const R wx[3], const R wy[3],
+                               const R wz[3], int i0, int j0, int k0) {
+  R r = R(0);
+  const int nk = D == 3 ? 3 : 1;
+  for (int k1 = 0; k1 < nk; k1++)
+#pragma unroll
+    for (int j1 = 0; j1 < 3; j1++) {
+      const R dvol = D == 3 ? wy[j1] * wz[k1] : wy[j1];
+      const R* row = f + i0 + GX * (j0 + j1) + GXY * (k0 + k1);
+#pragma unroll
+      for (int i1 = 0; i1 < 3; i1++) r = r + (dvol * wx[i1]) * row[i1];
+    }
+  return r;
+}
+
+// Adds to an FP64 LDS patch over nodes o .. o+J-1 (component stride pvol).
+template <int D>
+struct EnvLdsAcc {
+  double* P;
+  int o[3];
+  int J0, J1, pvol;
+  template <typename R>
+  __device__ void operator()(int c, int i, int j, int k, R v) const {
+    const int l = (i - o[0]) + J0 * ((j - o[1]) + J1 * (D == 3 ? k - o[2] : 0));
+    atomicAdd(P + c * pvol + l, (double)v);
+  }
+};
+
+// Copies nc components of the tile's LDS patch to its slot of the global
+// patch array (tile stride D components).
+template <int D>
+__device__ inline void env_patch_out(const TileGeom& tg, int t, const double* P, int nc,
+                                     double* patch) {
+  double* dst = patch + (int64_t)t * D * tg.pvol;
+  for (int l = threadIdx.x; l < nc * tg.pvol; l += blockDim.x) dst[l] = P[l];
+}
+
+// Phase 1 per tile: set_env_acc + lpf_env_momenta + set_env_density, the
+// density summed into component 0 of the tile's patch.
+template <int D, typename R>
+__global__ __launch_bounds__(kEB) void k_env_push_tile(TileGeom tg, DevGrid g, RecPtr A,
+                                                       const int* seg_start, const int* seg_cap,
+                                                       const int* cnt, const R* ebf, const R* je,
+                                                       R qch, R dth, R alp, double* patch) {
+  extern __shared__ double lds_d[];
+  const int t = blockIdx.x;
+  const EnvTile e = env_tile<D>(tg, t);
+  const int nf = g.nfield, nc = nf + D + 1;
+  double* Pd = lds_d;                 // density patch, pvol
+  R* Fl = (R*)(lds_d + tg.pvol);      // nf ebf components, then je(1:D+1)
+  for (int l = threadIdx.x; l < tg.pvol; l += blockDim.x) Pd[l] = 0.0;
+  for (int c = 0; c < nf; c++) env_stage<D, R>(g, e, comp_c(ebf, g, c), Fl + c * e.gvol);
+  for (int c = 0; c <= D; c++) env_stage<D, R>(g, e, comp_c(je, g, c), Fl + (nf + c) * e.gvol);
+  __syncthreads();
+  (void)nc;
+  const int GX = e.G[0], GXY = e.G[0] * e.G[1];
+  const R* EB = Fl;
+  const R* JE = Fl + nf * e.gvol;
+  const int n = min(cnt[t], seg_cap[t]);
+  const int64_t base = seg_start[t];
+  for (int q0 = threadIdx.x; q0 < n; q0 += blockDim.x) {
+    const int64_t src = base + q0;
+    if (rec_cell<R>(A, src) == kHole) continue;
+    Rec<R> q;
+    load_rec<D, R>(A, src, q, tg.bx, tg.by);
+    R ax[3], axh[3], ay[3], ayh[3], az[3] = {R(1), R(0), R(0)}, azh[3] = {R(1), R(0), R(0)};
+    int hx, hy, hz = 0;
+    qqh<R>(q.s[0], ax, axh, hx);
+    qqh<R>(q.s[1], ay, ayh, hy);
+    if (D == 3) qqh<R>(q.s[2], az, azh, hz);
+    // local box indices of the integer (i) and half-integer (ih) shape bases
+    const int i = q.N[0] - e.T0[0] + 1, j = q.N[1] - e.T0[1] + 1, k = D == 3 ? q.N[2] - e.T0[2] + 1 : 0;
+    const int ih = i + hx, jh = j + hy, kh = D == 3 ? k + hz : 0;
+    const R one = R(1), half = R(0.5);
+    const EnvLdsAcc<D> acc{Pd, {e.T0[0] - 2, e.T0[1] - 2, e.T0[2] - (D == 3 ? 2 : 0)},
+                           tg.J[0], tg.J[1], tg.pvol};
+    if constexpr (D == 2) {
+      R ap[6];
+      ap[5] = contract3l<2, R>(JE, GX, GXY, ax, ay, az, i, j, 0);
+      ap[0] = contract3l<2, R>(EB, GX, GXY, axh, ay, az, ih, j, 0);
+      ap[3] = contract3l<2, R>(JE + e.gvol, GX, GXY, axh, ay, az, ih, j, 0);
+      ap[1] = contract3l<2, R>(EB + e.gvol, GX, GXY, ax, ayh, az, i, jh, 0);
+      ap[4] = contract3l<2, R>(JE + 2 * e.gvol, GX, GXY, ax, ayh, az, i, jh, 0);
+      ap[2] = contract3l<2, R>(EB + 2 * e.gvol, GX, GXY, axh, ayh, az, ih, jh, 0);
+      const R u1 = q.p[0], u2 = q.p[1];
+      const R gam2 = one + u1 * u1 + u2 * u2 + ap[5];
+      for (int c = 0; c < 3; c++) ap[c] = qch * ap[c];
+      for (int c = 3; c < 5; c++) ap[c] = half * qch * qch * ap[c];
+      const R aa1 = dth * (ap[0] * u1 + ap[1] * u2);
+      const R b1 = dth * (ap[3] * u1 + ap[4] * u2);
+      const R gam = sqrt(gam2);
+      const R dgam = (aa1 * gam - b1) / gam2;
+      const R gam_inv = (gam - dgam) / gam2;
+      for (int c = 2; c < 5; c++) ap[c] = ap[c] * gam_inv;
+      const R e1 = alp * (ap[0] - ap[3]), e2 = alp * (ap[1] - ap[4]), bb1 = alp * ap[2];
+      const R vp0 = u1 + e1, vp1 = u2 + e2;
+      const R b2 = one + bb1 * bb1;
+      const R vh0 = (vp0 + vp1 * bb1) / b2, vh1 = (vp1 - vp0 * bb1) / b2;
+      q.p[0] = R(2) * vh0 - u1;
+      q.p[1] = R(2) * vh1 - u2;
+      const R wghp = (R)q.w * gam_inv;
+      for (int j1 = 0; j1 < 3; j1++) {
+        const R dvol = ay[j1] * wghp;
+        for (int i1 = 0; i1 < 3; i1++)
+          acc(0, q.N[0] - 1 + i1, q.N[1] - 1 + j1, 1, dvol * ax[i1]);
+      }
+    } else {
+      R ap[10];
+      ap[9] = contract3l<3, R>(JE, GX, GXY, ax, ay, az, i, j, k);
+      ap[0] = contract3l<3, R>(EB, GX, GXY, axh, ay, az, ih, j, k);
+      ap[6] = contract3l<3, R>(JE + e.gvol, GX, GXY, axh, ay, az, ih, j, k);
+      ap[1] = contract3l<3, R>(EB + e.gvol, GX, GXY, ax, ayh, az, i, jh, k);
+      ap[7] = contract3l<3, R>(JE + 2 * e.gvol, GX, GXY, ax, ayh, az, i, jh, k);
+      ap[5] = contract3l<3, R>(EB + 5 * e.gvol, GX, GXY, axh, ayh, az, ih, jh, k);
+      ap[3] = contract3l<3, R>(EB + 3 * e.gvol, GX, GXY, ax, ayh, azh, i, jh, kh);
+      ap[4] = contract3l<3, R>(EB + 4 * e.gvol, GX, GXY, axh, ay, azh, ih, j, kh);
+      ap[2] = contract3l<3, R>(EB + 2 * e.gvol, GX, GXY, ax, ay, azh, i, j, kh);
+      ap[8] = contract3l<3, R>(JE + 3 * e.gvol, GX, GXY, ax, ay, azh, i, j, kh);
+      const R u[3] = {q.p[0], q.p[1], q.p[2]};
+      const R gam2 = one + u[0] * u[0] + u[1] * u[1] + u[2] * u[2] + ap[9];
+      for (int c = 0; c < 6; c++) ap[c] = qch * ap[c];
+      for (int c = 6; c < 9; c++) ap[c] = half * qch * qch * ap[c];
+      const R aa1 = dth * (ap[0] * u[0] + ap[1] * u[1] + ap[2] * u[2]);
+      const R b1 = dth * (ap[6] * u[0] + ap[7] * u[1] + ap[8] * u[2]);
+      const R gam = sqrt(gam2);
+      const R dgam = (aa1 * gam - b1) / gam2;
+      const R gam_inv = (gam - dgam) / gam2;
+      for (int c = 3; c < 9; c++) ap[c] = ap[c] * gam_inv;
+      R vp[3], bb[3], vph[3];
+      for (int c = 0; c < 3; c++) vp[c] = alp * (ap[c] - ap[c + 6]) + u[c];
+      for (int c = 0; c < 3; c++) bb[c] = alp * ap[c + 3];
+      const R b2 = one + (bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2]);
+      const R bv = bb[0] * vp[0] + bb[1] * vp[1] + bb[2] * vp[2];
+      vph[0] = vp[0] + vp[1] * bb[2] - vp[2] * bb[1] + bb[0] * bv;
+      vph[1] = vp[1] + vp[2] * bb[0] - vp[0] * bb[2] + bb[1] * bv;
+      vph[2] = vp[2] + vp[0] * bb[1] - vp[1] * bb[0] + bb[2] * bv;
+      for (int c = 0; c < 3; c++) q.p[c] = R(2) * (vph[c] / b2) - u[c];
+      const R wghp = (R)q.w * gam_inv;
+      for (int k1 = 0; k1 < 3; k1++)
+        for (int j1 = 0; j1 < 3; j1++) {
+          const R dvol = ay[j1] * az[k1] * wghp;
+          for (int i1 = 0; i1 < 3; i1++)
+            acc(0, q.N[0] - 1 + i1, q.N[1] - 1 + j1, q.N[2] - 1 + k1, dvol * ax[i1]);
+        }
+    }
+    put_rec<D, R>(A, src, pack_cell<D>(q.N, tg.bx, tg.by), q.s, q.p, q.w);
+  }
+  __syncthreads();
+  env_patch_out<D>(tg, t, Pd, 1, patch);
+}
+
+// Phase 2 per tile: set_env_grad_interp + lpf_env_positions + removal +
+// Esirkepov deposit into the tile's LDS current patch + re-binning into B.
+template <int D, typename R>
+__global__ __launch_bounds__(kEB) void k_env_move_tile(TileGeom tg, DevGrid g, StepConst<R> sc,
+                                                       RecPtr A, RecPtr B, const int* seg_start,
+                                                       const int* seg_cap, const int* cnt,
+                                                       const R* je, int* cnt_next, int* ovf_next,
+                                                       double* patch, int* grow) {
+  extern __shared__ double lds_d[];
+  const int t = blockIdx.x;
+  const EnvTile e = env_tile<D>(tg, t);
+  double* Jl = lds_d;                      // D current components, pvol each
+  R* JE = (R*)(lds_d + D * tg.pvol);       // je(1:D+1)
+  for (int l = threadIdx.x; l < D * tg.pvol; l += blockDim.x) Jl[l] = 0.0;
+  for (int c = 0; c <= D; c++) env_stage<D, R>(g, e, comp_c(je, g, c), JE + c * e.gvol);
+  __syncthreads();
+  const int GX = e.G[0], GXY = e.G[0] * e.G[1];
+  const EnvLdsAcc<D> acc{Jl, {e.T0[0] - 2, e.T0[1] - 2, e.T0[2] - (D == 3 ? 2 : 0)},
+                         tg.J[0], tg.J[1], tg.pvol};
+  const int n = min(cnt[t], seg_cap[t]);
+  const int64_t base = seg_start[t];
+  for (int q0 = threadIdx.x; q0 < n; q0 += blockDim.x) {
+    const int64_t src = base + q0;
+    if (rec_cell<R>(A, src) == kHole) continue;
+    Rec<R> q;
+    load_rec<D, R>(A, src, q, tg.bx, tg.by);
+    R ax[3], axh[3], ay[3], ayh[3], az[3] = {R(1), R(0), R(0)}, azh[3] = {R(1), R(0), R(0)};
+    int hx, hy, hz = 0;
+    qqh<R>(q.s[0], ax, axh, hx);
+    qqh<R>(q.s[1], ay, ayh, hy);
+    if (D == 3) qqh<R>(q.s[2], az, azh, hz);
+    const int i = q.N[0] - e.T0[0] + 1, j = q.N[1] - e.T0[1] + 1, k = D == 3 ? q.N[2] - e.T0[2] + 1 : 0;
+    const int ih = i + hx, jh = j + hy, kh = D == 3 ? k + hz : 0;
+    R ap[4] = {R(0), R(0), R(0), R(0)};
+    ap[0] = contract3l<D, R>(JE + e.gvol, GX, GXY, axh, ay, az, ih, j, k);
+    ap[D] = contract3l<D, R>(JE, GX, GXY, ax, ay, az, i, j, k);
+    ap[1] = contract3l<D, R>(JE + 2 * e.gvol, GX, GXY, ax, ayh, az, i, jh, k);
+    if (D == 3) ap[2] = contract3l<D, R>(JE + 3 * e.gvol, GX, GXY, ax, ay, azh, i, j, kh);
+    const R one = R(1);
+    R pp2 = R(0), pv = R(0);
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      pp2 = pp2 + q.p[d] * q.p[d];
+      pv = pv + q.p[d] * ap[d];
+    }
+    const R gam2 = one + pp2 + ap[D];
+    const R gam = sqrt(gam2);
+    const R gam3 = gam2 * gam;
+    const R b2 = R(0.25) * pv;
+    const R gam_inv = (one / gam) * (one - sc.dt * b2 / gam3);
+    const int io[3] = {q.N[0], q.N[1], q.N[2]};
+    const R so[3] = {q.s[0], q.s[1], q.s[2]};
+    if (!move_and_test<D, R>(q, sc.dt * gam_inv, sc.inv, sc.nn)) continue;
+    esirkepov<D, R>(io, so, q.N, q.s, (R)((float)sc.qch * q.w), acc);
+    const int dtile = tile_of(tg, pack_cell<D>(q.N, tg.bx, tg.by), D);
+    const int slot = reserve_slot(dtile, cnt_next);
+    int64_t dst = (int64_t)seg_start[dtile] + slot;
+    if (slot >= seg_cap[dtile]) {
+      dst = sc.ovf_base + atomicAdd(ovf_next, 1);
+      grow[dtile] = 1;
+    }
+    store_rec<D, R>(B, dst, q, tg.bx, tg.by);
+  }
+  __syncthreads();
+  env_patch_out<D>(tg, t, Jl, D, patch);
 }
 
 // ------------------------------------------------------------ grid side
@@ -561,24 +809,48 @@ hipError_t env_amp_prepare(const DevGrid& g, const EnvConst& ec, const R* env, R
   return hipGetLastError();
 }
 
+// LDS bytes of the tile workgroups
+template <typename R>
+static size_t env_push_lds(const TileGeom& tg, const DevGrid& g) {
+  const int G = (tg.T[0] + 3) * (tg.T[1] + 3) * (g.ndim == 3 ? tg.T[2] + 3 : 1);
+  return (size_t)tg.pvol * sizeof(double) + (size_t)(g.nfield + g.ndim + 1) * G * sizeof(R);
+}
+template <typename R>
+static size_t env_move_lds(const TileGeom& tg, const DevGrid& g) {
+  const int G = (tg.T[0] + 3) * (tg.T[1] + 3) * (g.ndim == 3 ? tg.T[2] + 3 : 1);
+  return (size_t)g.ndim * tg.pvol * sizeof(double) + (size_t)(g.ndim + 1) * G * sizeof(R);
+}
+
 template <typename R>
 void env_particles_push(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf,
-                        const R* je, double* den, double lorentz_fact, hipStream_t s,
-                        hipEvent_t ev_begin, hipEvent_t ev_end) {
-  if (!ts->loaded) return;
+                        const R* je, double* den, double* patch, double lorentz_fact,
+                        hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end) {
+  if (!ts->loaded) {
+    fill<double>(den, g.ncell, 0.0, s);
+    return;
+  }
   if (ev_begin) (void)hipEventRecord(ev_begin, s);
   const int cur = ts->cur;
   const RecPtr& A = ts->buf[cur];
   const R qch = (R)ts->charge, dth = (R)(0.5 * g.dt), alp = (R)(0.5 * g.dt * lorentz_fact);
-  const unsigned nb = tg.ntiles + 64;
+  const size_t lds = env_push_lds<R>(tg, g);
+  // tiles: density into the patches, folded onto den; then the overflow
+  // list adds to den directly
   if (g.ndim == 2)
-    k_env_push<2, R><<<nb, kEB, 0, s>>>(tg, g, A, ts->seg_start, ts->seg_cap, ts->count[cur],
-                                        ts->ovf_base, ts->ovf_count + cur, ebf, je, qch, dth,
-                                        alp, den);
+    k_env_push_tile<2, R><<<tg.ntiles, kEB, lds, s>>>(tg, g, A, ts->seg_start, ts->seg_cap,
+                                                      ts->count[cur], ebf, je, qch, dth, alp,
+                                                      patch);
   else
-    k_env_push<3, R><<<nb, kEB, 0, s>>>(tg, g, A, ts->seg_start, ts->seg_cap, ts->count[cur],
-                                        ts->ovf_base, ts->ovf_count + cur, ebf, je, qch, dth,
-                                        alp, den);
+    k_env_push_tile<3, R><<<tg.ntiles, kEB, lds, s>>>(tg, g, A, ts->seg_start, ts->seg_cap,
+                                                      ts->count[cur], ebf, je, qch, dth, alp,
+                                                      patch);
+  tile_store_fold_density(tg, g, patch, den, s);
+  if (g.ndim == 2)
+    k_env_push<2, R><<<64, kEB, 0, s>>>(tg, g, A, ts->ovf_base, ts->ovf_count + cur, ebf, je,
+                                        qch, dth, alp, den);
+  else
+    k_env_push<3, R><<<64, kEB, 0, s>>>(tg, g, A, ts->ovf_base, ts->ovf_count + cur, ebf, je,
+                                        qch, dth, alp, den);
   if (ev_end) (void)hipEventRecord(ev_end, s);
 }
 
@@ -587,8 +859,6 @@ int env_particles_move(TileStore* ts, const TileGeom& tg, const DevGrid& g, cons
                        double* patch, hipStream_t s, std::string* msg, hipEvent_t ev_begin,
                        hipEvent_t ev_end) {
   const int cur = ts->cur, nxt = 1 - cur;
-  const size_t npatch = (size_t)tg.ntiles * tg.pvol * g.curr_ndim;
-  fill<double>(patch, (int64_t)npatch, 0.0, s);
   if (ts->loaded) {
     StepConst<R> sc{};
     sc.qch = (R)ts->charge;
@@ -602,23 +872,33 @@ int env_particles_move(TileStore* ts, const TileGeom& tg, const DevGrid& g, cons
     sc.ovf_base = ts->ovf_base;
     fill<int>(ts->count[nxt], tg.ntiles, 0, s);
     fill<int>(ts->ovf_count + nxt, 1, 0, s);
-    const unsigned nb = tg.ntiles + 64;
+    const size_t lds = env_move_lds<R>(tg, g);
     if (ev_begin) (void)hipEventRecord(ev_begin, s);
-    if (g.ndim == 2)
-      k_env_move<2, R><<<nb, kEB, 0, s>>>(tg, g, sc, ts->buf[cur], ts->buf[nxt], ts->seg_start,
-                                          ts->seg_cap, ts->count[cur], ts->ovf_count + cur, je,
-                                          ts->count[nxt], ts->ovf_count + nxt, patch, ts->grow);
-    else
-      k_env_move<3, R><<<nb, kEB, 0, s>>>(tg, g, sc, ts->buf[cur], ts->buf[nxt], ts->seg_start,
-                                          ts->seg_cap, ts->count[cur], ts->ovf_count + cur, je,
-                                          ts->count[nxt], ts->ovf_count + nxt, patch, ts->grow);
+    // every tile writes its whole patch; the overflow list adds to them
+    if (g.ndim == 2) {
+      k_env_move_tile<2, R><<<tg.ntiles, kEB, lds, s>>>(
+          tg, g, sc, ts->buf[cur], ts->buf[nxt], ts->seg_start, ts->seg_cap, ts->count[cur], je,
+          ts->count[nxt], ts->ovf_count + nxt, patch, ts->grow);
+      k_env_move<2, R><<<64, kEB, 0, s>>>(tg, g, sc, ts->buf[cur], ts->buf[nxt], ts->seg_start,
+                                          ts->seg_cap, ts->ovf_count + cur, je, ts->count[nxt],
+                                          ts->ovf_count + nxt, patch, ts->grow);
+    } else {
+      k_env_move_tile<3, R><<<tg.ntiles, kEB, lds, s>>>(
+          tg, g, sc, ts->buf[cur], ts->buf[nxt], ts->seg_start, ts->seg_cap, ts->count[cur], je,
+          ts->count[nxt], ts->ovf_count + nxt, patch, ts->grow);
+      k_env_move<3, R><<<64, kEB, 0, s>>>(tg, g, sc, ts->buf[cur], ts->buf[nxt], ts->seg_start,
+                                          ts->seg_cap, ts->ovf_count + cur, je, ts->count[nxt],
+                                          ts->ovf_count + nxt, patch, ts->grow);
+    }
     if (ev_end) (void)hipEventRecord(ev_end, s);
     if (hipGetLastError() != hipSuccess) {
       *msg = "envelope particle kernel launch failed";
       return ALADYN_EHIP;
     }
+  } else {
+    fill<double>(patch, (int64_t)tg.ntiles * tg.pvol * g.curr_ndim, 0.0, s);
   }
-  tile_store_fold<R>(tg, g, patch, jc, 0, s);
+  tile_store_fold<R>(tg, g, patch, jc, 0, s, true);
   if (ts->loaded) {
     if (hipMemcpyAsync(ts->h_ovf + ts->ovf_slot, ts->ovf_count + nxt, sizeof(int),
                        hipMemcpyDeviceToHost, s) != hipSuccess) {
@@ -636,8 +916,8 @@ int env_particles_move(TileStore* ts, const TileGeom& tg, const DevGrid& g, cons
   template hipError_t env_amp_prepare<R>(const DevGrid&, const EnvConst&, const R*, R*,         \
                                          hipStream_t);                                           \
   template void env_particles_push<R>(TileStore*, const TileGeom&, const DevGrid&, const R*,    \
-                                      const R*, double*, double, hipStream_t, hipEvent_t,        \
-                                      hipEvent_t);                                               \
+                                      const R*, double*, double*, double, hipStream_t,           \
+                                      hipEvent_t, hipEvent_t);                                   \
   template int env_particles_move<R>(TileStore*, const TileGeom&, const DevGrid&, const R*, R*, \
                                      double*, hipStream_t, std::string*, hipEvent_t, hipEvent_t);
 ALADYN_ENV_INST(float)

@@ -217,14 +217,9 @@ __device__ inline void b_jc_scale(DevGrid g, T* jc) {
 }
 template <typename T>
 __global__ void k_jc_scale(DevGrid g, T* jc) { b_jc_scale<T>(g, jc); }
-template <typename T>
-__global__ void kb_jc_scale(const FieldRun<T>* runs) {
-  const FieldRun<T>& r = runs[blockIdx.y];
-  b_jc_scale<T>(r.g, r.jc);
-}
 
 template <typename T>
-void curr_collect(const DevGrid& g, T* jc, hipStream_t s) {
+void curr_collect(const DevGrid& g, T* jc, hipStream_t s, bool normalised) {
   if (g.ibx == 0) {
     int64_t n = (int64_t)(g.jy2 - g.jy1 + 1) * (g.kz2 - g.kz1 + 1) * g.curr_ndim;
     k_jc_xfold<T><<<nblocks(n), kFieldBlock, 0, s>>>(g, jc);
@@ -237,171 +232,159 @@ void curr_collect(const DevGrid& g, T* jc, hipStream_t s) {
     int64_t n = (int64_t)(g.ix2 - g.ix1 + 1) * (g.jy2 - g.jy1 + 1) * g.curr_ndim;
     k_jc_zfold<T><<<nblocks(n), kFieldBlock, 0, s>>>(g, jc);
   }
-  k_jc_scale<T><<<nblocks(g.ncell), kFieldBlock, 0, s>>>(g, jc);
+  if (!normalised) k_jc_scale<T><<<nblocks(g.ncell), kFieldBlock, 0, s>>>(g, jc);
 }
 
 // ------------------------------------------------------------ ef_bds / bf_bds
 // grid_fields.f90:1204-1354 (E ghosts on the right faces) and 1069-1202
-// (B ghosts on the left faces), imbd = 0, unstretched metric = 1.  Each
-// thread owns one face cell; faces do not read what another face writes.
+// (B ghosts on the left faces), imbd = 0, unstretched metric = 1.  Every
+// ghost value is one of two first-order Mur forms of the interior node's B
+// and E (or E and B) and a = dtl/dh:
+//   mur_p(u, v) = (2u - (1-a)v)/(1+a),  mur_m(u, v) = -(2u + (1-a)v)/(1+a).
+template <typename T>
+__device__ inline T mur_p(T u, T v, T a) {
+  return (T(2) * u - (T(1) - a) * v) / (T(1) + a);
+}
+template <typename T>
+__device__ inline T mur_m(T u, T v, T a) {
+  return -(T(2) * u + (T(1) - a) * v) / (T(1) + a);
+}
+
+// ef_bds alone: one thread per face cell; faces do not read what another
+// face writes.  Used before the first Yee half-step when the x stencil
+// is wide (der_ord > 2), where a node reads the ghost two nodes to its right.
 template <typename T>
 __device__ inline void b_ef_bds(DevGrid g, T* ebf, double dtl) {
   const int nf = g.nfield;
   const int ni = g.ix2 - g.ix1 + 1, nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1;
-  const int64_t nxf = (g.ibx < 2) ? (int64_t)nj * nk : 0;
-  const int64_t nyf = (g.iby < 2) ? (int64_t)ni * nk : 0;
-  const int64_t nzf = (g.ndim > 2 && g.ibz < 2) ? (int64_t)ni * nj : 0;
+  const int64_t nxf = (int64_t)nj * nk;
+  const int64_t nyf = (int64_t)ni * nk;
+  const int64_t nzf = g.ndim > 2 ? (int64_t)ni * nj : 0;
   T* ex = comp(ebf, g, 1);
   T* ey = comp(ebf, g, 2);
   T* ez = comp(ebf, g, 3);
   T* bx = comp(ebf, g, 4);
   T* by = comp(ebf, g, 5);
   T* bz = comp(ebf, g, nf);
+  const int64_t sy = g.n1p, sz = (int64_t)g.n1p * g.n2p;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nxf + nyf + nzf;
        t += (int64_t)gridDim.x * blockDim.x) {
     if (t < nxf) {
-      int j = g.jy1 + (int)(t % nj), k = g.kz1 + (int)(t / nj);
-      const int i = g.ix2;
-      if (g.ibx == 0) {
-        const T a = T(1.0 * g.dx_inv * dtl);
-        ey[fofs(g, i + 1, j, k)] =
-            (T(2) * bz[fofs(g, i, j, k)] - (T(1) - a) * ey[fofs(g, i, j, k)]) / (T(1) + a);
-        if (nf > 3)
-          ez[fofs(g, i + 1, j, k)] =
-              -(T(2) * by[fofs(g, i, j, k)] + (T(1) - a) * ez[fofs(g, i, j, k)]) / (T(1) + a);
-      } else {
-        ey[fofs(g, i + 1, j, k)] = ey[fofs(g, i, j, k)];
-        if (nf > 3) ez[fofs(g, i + 1, j, k)] = ez[fofs(g, i, j, k)];
-      }
+      const int64_t o = fofs(g, g.ix2, g.jy1 + (int)(t % nj), g.kz1 + (int)(t / nj));
+      const T a = T(1.0 * g.dx_inv * dtl);
+      ey[o + 1] = mur_p(bz[o], ey[o], a);
+      if (nf > 3) ez[o + 1] = mur_m(by[o], ez[o], a);
     } else if (t < nxf + nyf) {
-      int64_t u = t - nxf;
-      int i = g.ix1 + (int)(u % ni), k = g.kz1 + (int)(u / ni);
-      const int j = g.jy2;
+      const int64_t u = t - nxf;
+      const int64_t o = fofs(g, g.ix1 + (int)(u % ni), g.jy2, g.kz1 + (int)(u / ni));
       const T a = T(1.0 * g.dy_inv * dtl);
-      ex[fofs(g, i, j + 1, k)] =
-          -(T(2) * bz[fofs(g, i, j, k)] + (T(1) - a) * ex[fofs(g, i, j, k)]) / (T(1) + a);
-      if (nf > 3)
-        ez[fofs(g, i, j + 1, k)] =
-            (T(2) * bx[fofs(g, i, j, k)] - (T(1) - a) * ez[fofs(g, i, j, k)]) / (T(1) + a);
+      ex[o + sy] = mur_m(bz[o], ex[o], a);
+      if (nf > 3) ez[o + sy] = mur_p(bx[o], ez[o], a);
     } else {
-      int64_t u = t - nxf - nyf;
-      int i = g.ix1 + (int)(u % ni), j = g.jy1 + (int)(u / ni);
-      const int k = g.kz2;
+      const int64_t u = t - nxf - nyf;
+      const int64_t o = fofs(g, g.ix1 + (int)(u % ni), g.jy1 + (int)(u / ni), g.kz2);
       const T a = T(1.0 * g.dz_inv * dtl);
-      ex[fofs(g, i, j, k + 1)] =
-          (T(2) * by[fofs(g, i, j, k)] - (T(1) - a) * ex[fofs(g, i, j, k)]) / (T(1) + a);
-      ey[fofs(g, i, j, k + 1)] =
-          -(T(2) * bx[fofs(g, i, j, k)] + (T(1) - a) * ey[fofs(g, i, j, k)]) / (T(1) + a);
+      ex[o + sz] = mur_p(by[o], ex[o], a);
+      ey[o + sz] = mur_m(bx[o], ey[o], a);
     }
   }
 }
 template <typename T>
 __global__ void k_ef_bds(DevGrid g, T* ebf, double dtl) { b_ef_bds<T>(g, ebf, dtl); }
+// ef_bds(0) of the runs with the wide x stencil
 template <typename T>
-__global__ void kb_ef_bds(const FieldRun<T>* runs, int mode) {
+__global__ void kb_ef_bds(const FieldRun<T>* runs) {
   const FieldRun<T>& r = runs[blockIdx.y];
-  b_ef_bds<T>(r.g, r.ebf, mode ? r.g.dt : 0.0);
-}
-
-template <typename T>
-__device__ inline void b_bf_bds(DevGrid g, T* ebf, double dtl) {
-  const int nf = g.nfield;
-  const int ni = g.ix2 - g.ix1 + 1, nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1;
-  const int64_t nxf = (g.ibx < 2) ? (int64_t)nj * nk : 0;
-  const int64_t nyf = (g.iby < 2) ? (int64_t)ni * nk : 0;
-  const int64_t nzf = (g.ndim > 2 && g.ibz < 2) ? (int64_t)ni * nj : 0;
-  T* ex = comp(ebf, g, 1);
-  T* ey = comp(ebf, g, 2);
-  T* ez = comp(ebf, g, 3);
-  T* bx = comp(ebf, g, 4);
-  T* by = comp(ebf, g, 5);
-  T* bz = comp(ebf, g, nf);
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nxf + nyf + nzf;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    if (t < nxf) {
-      int j = g.jy1 + (int)(t % nj), k = g.kz1 + (int)(t / nj);
-      const int i = g.ix1;
-      const T a = T(1.0 * g.dx_inv * dtl);
-      bz[fofs(g, i - 1, j, k)] =
-          -(T(2) * ey[fofs(g, i, j, k)] + (T(1) - a) * bz[fofs(g, i, j, k)]) / (T(1) + a);
-      if (nf > 3)
-        by[fofs(g, i - 1, j, k)] =
-            (T(2) * ez[fofs(g, i, j, k)] - (T(1) - a) * by[fofs(g, i, j, k)]) / (T(1) + a);
-    } else if (t < nxf + nyf) {
-      int64_t u = t - nxf;
-      int i = g.ix1 + (int)(u % ni), k = g.kz1 + (int)(u / ni);
-      const int j = g.jy1;
-      const T a = T(1.0 * g.dy_inv * dtl);
-      T v = T(2) * ex[fofs(g, i, j, k)] - (T(1) - a) * bz[fofs(g, i, j, k)];
-      bz[fofs(g, i, j - 1, k)] = v / (T(1) + a);
-      if (nf > 3) {
-        v = T(-2) * ez[fofs(g, i, j, k)] - (T(1) - a) * bx[fofs(g, i, j, k)];
-        bx[fofs(g, i, j - 1, k)] = v / (T(1) + a);
-      }
-    } else {
-      int64_t u = t - nxf - nyf;
-      int i = g.ix1 + (int)(u % ni), j = g.jy1 + (int)(u / ni);
-      const int k = g.kz1;
-      const T a = T(1.0 * g.dz_inv * dtl);
-      T v = T(2) * ey[fofs(g, i, j, k)] - (T(1) - a) * bx[fofs(g, i, j, k)];
-      bx[fofs(g, i, j, k - 1)] = v / (T(1) + a);
-      v = T(-2) * ex[fofs(g, i, j, k)] - (T(1) - a) * by[fofs(g, i, j, k)];
-      by[fofs(g, i, j, k - 1)] = v / (T(1) + a);
-    }
-  }
-}
-template <typename T>
-__global__ void k_bf_bds(DevGrid g, T* ebf, double dtl) { b_bf_bds<T>(g, ebf, dtl); }
-template <typename T>
-__global__ void kb_bf_bds(const FieldRun<T>* runs) {
-  const FieldRun<T>& r = runs[blockIdx.y];
-  b_bf_bds<T>(r.g, r.ebf, r.g.dt);
+  if (r.g.se[1] != 0.0) b_ef_bds<T>(r.g, r.ebf, 0.0);
 }
 
 // --------------------------------------------------------------- rote / rotb
-// B -= dtf*curl E on the interior (grid_fields.f90:1405-1478).
-template <typename T>
-__device__ inline void b_rote(DevGrid g, T* ebf, double dtf) {
+// advance_lpf_fields (curr_and_fields_util.f90:176-260) with imbd = 0 is
+//   ef_bds(0); rote(dt/2); bf_bds(dt); rotb(dt) + source; ef_bds(dt); rote(dt/2)
+// Each boundary call only reads nodes of the interior layer next to its
+// face and writes the ghost beyond it, so it joins the curl sweep whose
+// thread owns that interior node:
+//   * ef_bds(0) (S0): the thread of a node on a right face forms the ghost
+//     E its forward difference reads from its own E and B before it updates
+//     B.  ef_bds(dt) overwrites every such ghost in the same step, so S0's
+//     values are not stored.  With the wide x stencil a node also reads the
+//     ghost two nodes to its right, so S0 then runs alone first.
+//   * bf_bds(dt) (S2): the thread of a node on a left face writes the ghost
+//     B from its updated B and its E (the B sweep reads no B ghost).
+//   * ef_bds(dt) (S4): the thread of a node on a right face writes the
+//     ghost E from its updated E and B (the E sweep reads E at its own node
+//     only).
+
+// B -= dtf*curl E on the interior (grid_fields.f90:1405-1478).  FIRST: the
+// first half-step, with S0 (when s0) and S2 (a = dt/dh).
+template <typename T, bool FIRST>
+__device__ inline void b_rote(DevGrid g, T* ebf, double dtf, bool s0) {
   const int nf = g.nfield;
   const int ni = g.ix2 - g.ix1 + 1, nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1;
   const int64_t n = (int64_t)ni * nj * nk;
-  const T aphx = T(dtf * g.dx_inv), aphy = T(dtf * g.dy_inv), aphz = T(dtf * g.dz_inv);
+  const T aphy = T(dtf * g.dy_inv), aphz = T(dtf * g.dz_inv);
   const T aph1 = T(double(dtf * g.dx_inv) * g.se[0]), aph2 = T(double(dtf * g.dx_inv) * g.se[1]);
-  (void)aphx;
+  const T zero = T(1.0 * g.dx_inv * 0.0);  // a of ef_bds(0)
+  const T ax = T(1.0 * g.dx_inv * g.dt), ay = T(1.0 * g.dy_inv * g.dt),
+          az = T(1.0 * g.dz_inv * g.dt);
   const T* ex = comp(ebf, g, 1);
   const T* ey = comp(ebf, g, 2);
   const T* ez = comp(ebf, g, 3);
   T* bx = comp(ebf, g, 4);
   T* by = comp(ebf, g, 5);
   T* bz = comp(ebf, g, nf);
+  const int64_t sy = g.n1p, sz = (int64_t)g.n1p * g.n2p;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int i = g.ix1 + (int)(t % ni);
     const int j = g.jy1 + (int)((t / ni) % nj);
     const int k = g.kz1 + (int)(t / ((int64_t)ni * nj));
     const int64_t o = fofs(g, i, j, k);
-    const int64_t sy = g.n1p, sz = (int64_t)g.n1p * g.n2p;
-    const T sdhy = aphy;
-    bz[o] = bz[o] - aph1 * (ey[o + 1] - ey[o]) + sdhy * (ex[o + sy] - ex[o]) -
-            aph2 * (ey[o + 2] - ey[o - 1]);
+    const bool rx = FIRST && s0 && i == g.ix2, ry = FIRST && s0 && j == g.jy2,
+               rz = FIRST && s0 && k == g.kz2;
+    const T ex0 = ex[o], ey0 = ey[o], bz0 = bz[o];
+    const T eyx = rx ? mur_p(bz0, ey0, zero) : ey[o + 1];
+    const T exy = ry ? mur_m(bz0, ex0, zero) : ex[o + sy];
+    const T bz1 = bz0 - aph1 * (eyx - ey0) + aphy * (exy - ex0) - aph2 * (ey[o + 2] - ey[o - 1]);
+    bz[o] = bz1;
     if (nf >= 6) {
-      const T sdhz = aphz;
-      bx[o] = bx[o] - sdhy * (ez[o + sy] - ez[o]) + sdhz * (ey[o + sz] - ey[o]);
-      by[o] = by[o] + aph1 * (ez[o + 1] - ez[o]) - sdhz * (ex[o + sz] - ex[o]) +
-              aph2 * (ez[o + 2] - ez[o - 1]);
+      const T ez0 = ez[o], bx0 = bx[o], by0 = by[o];
+      const T ezy = ry ? mur_p(bx0, ez0, zero) : ez[o + sy];
+      const T eyz = rz ? mur_m(bx0, ey0, zero) : ey[o + sz];
+      const T ezx = rx ? mur_m(by0, ez0, zero) : ez[o + 1];
+      const T exz = rz ? mur_p(by0, ex0, zero) : ex[o + sz];
+      const T bx1 = bx0 - aphy * (ezy - ez0) + aphz * (eyz - ey0);
+      const T by1 = by0 + aph1 * (ezx - ez0) - aphz * (exz - ex0) + aph2 * (ez[o + 2] - ez[o - 1]);
+      bx[o] = bx1;
+      by[o] = by1;
+      if (FIRST) {  // S2
+        if (i == g.ix1) by[o - 1] = mur_p(ez0, by1, ax);
+        if (j == g.jy1) bx[o - sy] = mur_m(ez0, bx1, ay);
+        if (k == g.kz1) {
+          bx[o - sz] = mur_p(ey0, bx1, az);
+          by[o - sz] = mur_m(ex0, by1, az);
+        }
+      }
+    }
+    if (FIRST) {
+      if (i == g.ix1) bz[o - 1] = mur_m(ey0, bz1, ax);
+      if (j == g.jy1) bz[o - sy] = mur_p(ex0, bz1, ay);
     }
   }
 }
-template <typename T>
-__global__ void k_rote(DevGrid g, T* ebf, double dtf) { b_rote<T>(g, ebf, dtf); }
-template <typename T>
+template <typename T, bool FIRST>
+__global__ void k_rote(DevGrid g, T* ebf, double dtf, int s0) {
+  b_rote<T, FIRST>(g, ebf, dtf, s0 != 0);
+}
+template <typename T, bool FIRST>
 __global__ void kb_rote(const FieldRun<T>* runs) {
   const FieldRun<T>& r = runs[blockIdx.y];
-  b_rote<T>(r.g, r.ebf, 0.5 * r.g.dt);
+  b_rote<T, FIRST>(r.g, r.ebf, 0.5 * r.g.dt, r.g.se[1] == 0.0);
 }
 
-// E += dtf*curl B on the interior (grid_fields.f90:1480-1552) followed by the
-// source term E(1:curr_ndim) -= ompe*jc (curr_and_fields_util.f90:233-236).
+// E += dt*curl B on the interior (grid_fields.f90:1480-1552) followed by the
+// source term E(1:curr_ndim) -= ompe*jc (curr_and_fields_util.f90:233-236)
+// and S4, ef_bds(dt) (a = dt/dh).
 template <typename T>
 __device__ inline void b_rotb_source(DevGrid g, T* ebf, const T* jc, double dtf) {
   const int nf = g.nfield;
@@ -409,6 +392,8 @@ __device__ inline void b_rotb_source(DevGrid g, T* ebf, const T* jc, double dtf)
   const int64_t n = (int64_t)ni * nj * nk;
   const T aphy = T(dtf * g.dy_inv), aphz = T(dtf * g.dz_inv);
   const T aph1 = T(double(dtf * g.dx_inv) * g.se[0]), aph2 = T(double(dtf * g.dx_inv) * g.se[1]);
+  const T ax = T(1.0 * g.dx_inv * g.dt), ay = T(1.0 * g.dy_inv * g.dt),
+          az = T(1.0 * g.dz_inv * g.dt);
   const T ompe = T(g.ompe);
   T* ex = comp(ebf, g, 1);
   T* ey = comp(ebf, g, 2);
@@ -419,27 +404,40 @@ __device__ inline void b_rotb_source(DevGrid g, T* ebf, const T* jc, double dtf)
   const T* jx = jc;
   const T* jy = jc + g.ncell;
   const T* jz = jc + 2 * g.ncell;
+  const int64_t sy = g.n1p, sz = (int64_t)g.n1p * g.n2p;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int i = g.ix1 + (int)(t % ni);
     const int j = g.jy1 + (int)((t / ni) % nj);
     const int k = g.kz1 + (int)(t / ((int64_t)ni * nj));
     const int64_t o = fofs(g, i, j, k);
-    const int64_t sy = g.n1p, sz = (int64_t)g.n1p * g.n2p;
-    const T sdy = aphy;
-    T vx = ex[o] + sdy * (bz[o] - bz[o - sy]);
-    T vy = ey[o] - aph1 * (bz[o] - bz[o - 1]) - aph2 * (bz[o + 1] - bz[o - 2]);
+    const T bz0 = bz[o];
+    T vx = ex[o] + aphy * (bz0 - bz[o - sy]);
+    T vy = ey[o] - aph1 * (bz0 - bz[o - 1]) - aph2 * (bz[o + 1] - bz[o - 2]);
     if (nf >= 6) {
-      const T sdz = aphz;
-      vx = vx - sdz * (by[o] - by[o - sz]);
-      vy = vy + sdz * (bx[o] - bx[o - sz]);
-      T vz = ez[o] + aph1 * (by[o] - by[o - 1]) - sdy * (bx[o] - bx[o - sy]) +
+      const T bx0 = bx[o], by0 = by[o];
+      vx = vx - aphz * (by0 - by[o - sz]);
+      vy = vy + aphz * (bx0 - bx[o - sz]);
+      T vz = ez[o] + aph1 * (by0 - by[o - 1]) - aphy * (bx0 - bx[o - sy]) +
              aph2 * (by[o + 1] - by[o - 2]);
       if (g.curr_ndim >= 3) vz = vz - ompe * jz[o];
       ez[o] = vz;
+      vx = vx - ompe * jx[o];
+      vy = vy - ompe * jy[o];
+      if (i == g.ix2) ez[o + 1] = mur_m(by0, vz, ax);
+      if (j == g.jy2) ez[o + sy] = mur_p(bx0, vz, ay);
+      if (k == g.kz2) {
+        ex[o + sz] = mur_p(by0, vx, az);
+        ey[o + sz] = mur_m(bx0, vy, az);
+      }
+    } else {
+      vx = vx - ompe * jx[o];
+      vy = vy - ompe * jy[o];
     }
-    ex[o] = vx - ompe * jx[o];
-    ey[o] = vy - ompe * jy[o];
+    ex[o] = vx;
+    ey[o] = vy;
+    if (i == g.ix2) ey[o + 1] = mur_p(bz0, vy, ax);
+    if (j == g.jy2) ex[o + sy] = mur_m(bz0, vx, ay);
   }
 }
 template <typename T>
@@ -458,12 +456,11 @@ void advance_fields(const DevGrid& g, T* ebf, const T* jc, hipStream_t s) {
   const int ni = g.ix2 - g.ix1 + 1, nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1;
   const int64_t nint = (int64_t)ni * nj * nk;
   const int64_t nface = (int64_t)nj * nk + (int64_t)ni * nk + (g.ndim > 2 ? (int64_t)ni * nj : 0);
-  k_ef_bds<T><<<nblocks(nface), kFieldBlock, 0, s>>>(g, ebf, 0.0);
-  k_rote<T><<<nblocks(nint), kFieldBlock, 0, s>>>(g, ebf, dth);
-  k_bf_bds<T><<<nblocks(nface), kFieldBlock, 0, s>>>(g, ebf, dt_lp);
+  const bool s0 = g.se[1] == 0.0;  // compact x stencil: S0 joins the first sweep
+  if (!s0) k_ef_bds<T><<<nblocks(nface), kFieldBlock, 0, s>>>(g, ebf, 0.0);
+  k_rote<T, true><<<nblocks(nint), kFieldBlock, 0, s>>>(g, ebf, dth, s0 ? 1 : 0);
   k_rotb_source<T><<<nblocks(nint), kFieldBlock, 0, s>>>(g, ebf, jc, dt_lp);
-  k_ef_bds<T><<<nblocks(nface), kFieldBlock, 0, s>>>(g, ebf, dt_lp);
-  k_rote<T><<<nblocks(nint), kFieldBlock, 0, s>>>(g, ebf, dth);
+  k_rote<T, false><<<nblocks(nint), kFieldBlock, 0, s>>>(g, ebf, dth, 0);
 }
 
 // ------------------------------------------------------- batched launches
@@ -482,6 +479,7 @@ BatchDims batch_dims(const DevGrid* gs, int n) {
     d.ncell = std::max(d.ncell, g.ncell);
     d.nface = std::max(d.nface, nj * nk + ni * nk + (g.ndim > 2 ? ni * nj : 0));
     d.nint = std::max(d.nint, ni * nj * nk);
+    d.any_wide = d.any_wide || g.se[1] != 0.0;
   }
   return d;
 }
@@ -509,17 +507,14 @@ void curr_collect_batch(const FieldRun<T>* runs, int nruns, int ndim, const Batc
   kb_jc_xfold<T><<<bgrid(d.jx, nruns), kFieldBlock, 0, s>>>(runs);
   kb_jc_yfold<T><<<bgrid(d.jy, nruns), kFieldBlock, 0, s>>>(runs);
   if (ndim > 2) kb_jc_zfold<T><<<bgrid(d.jz, nruns), kFieldBlock, 0, s>>>(runs);
-  kb_jc_scale<T><<<bgrid(d.ncell, nruns), kFieldBlock, 0, s>>>(runs);
 }
 
 template <typename T>
 void advance_fields_batch(const FieldRun<T>* runs, int nruns, const BatchDims& d, hipStream_t s) {
-  kb_ef_bds<T><<<bgrid(d.nface, nruns), kFieldBlock, 0, s>>>(runs, 0);
-  kb_rote<T><<<bgrid(d.nint, nruns), kFieldBlock, 0, s>>>(runs);
-  kb_bf_bds<T><<<bgrid(d.nface, nruns), kFieldBlock, 0, s>>>(runs);
+  if (d.any_wide) kb_ef_bds<T><<<bgrid(d.nface, nruns), kFieldBlock, 0, s>>>(runs);
+  kb_rote<T, true><<<bgrid(d.nint, nruns), kFieldBlock, 0, s>>>(runs);
   kb_rotb_source<T><<<bgrid(d.nint, nruns), kFieldBlock, 0, s>>>(runs);
-  kb_ef_bds<T><<<bgrid(d.nface, nruns), kFieldBlock, 0, s>>>(runs, 1);
-  kb_rote<T><<<bgrid(d.nint, nruns), kFieldBlock, 0, s>>>(runs);
+  kb_rote<T, false><<<bgrid(d.nint, nruns), kFieldBlock, 0, s>>>(runs);
 }
 
 template <typename T>
@@ -568,7 +563,7 @@ void fields_left_xshift(const DevGrid& g, T* ebf, int shx, hipStream_t s, int nc
 #define ALADYN_INST_FIELDS(T)                                              \
   template void fields_left_xshift<T>(const DevGrid&, T*, int, hipStream_t, int); \
   template void field_xyzbd<T>(const DevGrid&, T*, int, hipStream_t);      \
-  template void curr_collect<T>(const DevGrid&, T*, hipStream_t);          \
+  template void curr_collect<T>(const DevGrid&, T*, hipStream_t, bool);    \
   template void advance_fields<T>(const DevGrid&, T*, const T*, hipStream_t); \
   template void fill<T>(T*, int64_t, T, hipStream_t);                                        \
   template void field_xyzbd_batch<T>(const FieldRun<T>*, int, int, const BatchDims&,          \

@@ -8,7 +8,9 @@ namespace aladyn {
 
 // ---------------------------------------------------------------- fields.hip
 template <typename T> void field_xyzbd(const DevGrid& g, T* ebf, int nc, hipStream_t s);
-template <typename T> void curr_collect(const DevGrid& g, T* jc, hipStream_t s);
+// jc_xyzbd, then jc *= djc unless the fold that wrote jc applied it already
+template <typename T>
+void curr_collect(const DevGrid& g, T* jc, hipStream_t s, bool normalised = false);
 template <typename T> void advance_fields(const DevGrid& g, T* ebf, const T* jc, hipStream_t s);
 template <typename T> void fill(T* a, int64_t n, T v, hipStream_t s);
 // ncomp components of the grid's layout (0: the grid's nfield)
@@ -28,11 +30,13 @@ struct BatchDims {
   int64_t fx, fy, fz;   // field ghost faces (times nfield)
   int64_t jx, jy, jz;   // current ghost folds (times curr_ndim)
   int64_t ncell, nface, nint;
+  bool any_wide;        // some run has the wide (der_ord > 2) x stencil
 };
 BatchDims batch_dims(const DevGrid* g, int n);
 template <typename T>
 void field_xyzbd_batch(const FieldRun<T>* runs, int nruns, int ndim, const BatchDims& d,
                        hipStream_t s);
+// jc_xyzbd only: the batched fold applies djc as it writes jc
 template <typename T>
 void curr_collect_batch(const FieldRun<T>* runs, int nruns, int ndim, const BatchDims& d,
                         hipStream_t s);

@@ -160,16 +160,19 @@ __global__ __launch_bounds__(256) void kb_ovf_particles(const TileRun<R>* __rest
 }
 
 // ------------------------------------------------------------ fold kernel
-// jc(node) [+]= the sum of the tile patches covering node: at most two tiles
+// jc(node) [+]= the sum of the tile patches covering node, for the first nc
+// of the D components each tile's patch slot holds: at most two tiles
 // per axis (T >= 4), the one whose interior holds the node and the neighbour
 // whose halo reaches it.  One thread per node of a 64 x 4 x 1 block of the
 // grid (no index division); the up to 8 FP64 patch values of a component are
 // all loaded before they are summed in FP64, in the order of increasing tile
-// index, and the sum is rounded to the grid's type once.
+// index, multiplied by the component's factor, and rounded to the grid's
+// type once.
 template <typename R>
 __device__ inline void fold_node(const TileGeom& tg, const DevGrid& g,
                                  const double* __restrict__ patch, R* __restrict__ jc,
-                                 int accumulate, const int idx[3]) {
+                                 int accumulate, const int idx[3], const JScale& js,
+                                 int nc) {
   const int D = g.ndim;
   // per axis: candidate tiles t0 = t1 - 1 and t1, local patch indices, validity
   int tl[3][2], loc[3][2];
@@ -195,7 +198,7 @@ __device__ inline void fold_node(const TileGeom& tg, const DevGrid& g,
     }
   }
   const int64_t o = (idx[0] - 1) + (int64_t)g.n1p * ((idx[1] - 1) + (int64_t)g.n2p * (idx[2] - 1));
-  for (int c = 0; c < D; c++) {
+  for (int c = 0; c < nc; c++) {
     double v[8];
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -208,9 +211,12 @@ __device__ inline void fold_node(const TileGeom& tg, const DevGrid& g,
           const int l = loc[0][e] + tg.J[0] * (loc[1][b] + tg.J[1] * loc[2][a]);
           v[a * 4 + b * 2 + e] = use ? patch[((int64_t)tile * D + c) * tg.pvol + l] : 0.0;
         }
-    double sum = accumulate ? (double)jc[(int64_t)c * g.ncell + o] : 0.0;
+    double sum = 0.0;
 #pragma unroll
     for (int q = 0; q < 8; q++) sum += v[q];
+    // the factor by selects, not a dynamic index into the argument
+    sum *= c == 0 ? js.s[0] : (c == 1 ? js.s[1] : js.s[2]);
+    if (accumulate) sum += (double)jc[(int64_t)c * g.ncell + o];
     jc[(int64_t)c * g.ncell + o] = (R)sum;
   }
 }
@@ -218,24 +224,26 @@ __device__ inline void fold_node(const TileGeom& tg, const DevGrid& g,
 template <typename R>
 __global__ __launch_bounds__(256) void k_fold(TileGeom tg, DevGrid g,
                                               const double* __restrict__ patch,
-                                              R* __restrict__ jc, int accumulate) {
+                                              R* __restrict__ jc, int accumulate, JScale js,
+                                              int nc) {
   const int idx[3] = {(int)(blockIdx.x * 64 + (threadIdx.x & 63)) + 1,
                       (int)(blockIdx.y * 4 + (threadIdx.x >> 6)) + 1, (int)blockIdx.z + 1};
   if (idx[0] > g.n1p || idx[1] > g.n2p) return;
-  fold_node<R>(tg, g, patch, jc, accumulate, idx);
+  fold_node<R>(tg, g, patch, jc, accumulate, idx, js, nc);
 }
 
-// jc = the folded patches of many runs in one launch (blockIdx.y = run), one
-// node per thread in storage order
+// jc = djc x the folded patches of many runs in one launch (blockIdx.y =
+// run), one node per thread in storage order
 template <typename R>
 __global__ __launch_bounds__(256) void kb_fold(const TileRun<R>* __restrict__ runs) {
   const TileRun<R>& r = runs[blockIdx.y];
   const DevGrid& g = r.g;
+  const JScale js = jscale_of(g, true);
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < g.ncell;
        q += (int64_t)gridDim.x * blockDim.x) {
     const int64_t jk = q / g.n1p;
     const int idx[3] = {(int)(q - jk * g.n1p) + 1, (int)(jk % g.n2p) + 1, (int)(jk / g.n2p) + 1};
-    fold_node<R>(r.tg, g, r.patch, r.jc, 0, idx);
+    fold_node<R>(r.tg, g, r.patch, r.jc, 0, idx, js, g.ndim);
   }
 }
 
@@ -257,14 +265,20 @@ __global__ void kb_ovf_out(const TileRun<R>* __restrict__ runs, int nruns, int* 
 
 template <typename R>
 void tile_store_fold(const TileGeom& tg, const DevGrid& g, const double* patch, R* jc,
-                     int accumulate, hipStream_t s) {
+                     int accumulate, hipStream_t s, bool normalise) {
   const dim3 fb((g.n1p + 63) / 64, (g.n2p + 3) / 4, g.n3p);
-  k_fold<R><<<fb, 256, 0, s>>>(tg, g, patch, jc, accumulate);
+  k_fold<R><<<fb, 256, 0, s>>>(tg, g, patch, jc, accumulate, jscale_of(g, normalise), g.ndim);
+}
+
+void tile_store_fold_density(const TileGeom& tg, const DevGrid& g, const double* patch,
+                             double* den, hipStream_t s) {
+  const dim3 fb((g.n1p + 63) / 64, (g.n2p + 3) / 4, g.n3p);
+  k_fold<double><<<fb, 256, 0, s>>>(tg, g, patch, den, 0, jscale_of(g, false), 1);
 }
 template void tile_store_fold<float>(const TileGeom&, const DevGrid&, const double*, float*, int,
-                                     hipStream_t);
+                                     hipStream_t, bool);
 template void tile_store_fold<double>(const TileGeom&, const DevGrid&, const double*, double*,
-                                      int, hipStream_t);
+                                      int, hipStream_t, bool);
 
 // ------------------------------------------------- relayout / transfers
 static constexpr int kB = 256;
@@ -1054,7 +1068,7 @@ template <typename R>
 int tile_store_step_enqueue(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf,
                             R* jc, double* patch, R alp, int initial, bool j_is_zero,
                             double* jraw, hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end,
-                            std::string* msg) {
+                            std::string* msg, bool normalise) {
   if (!ts->loaded) return 0;
   if ((int)sizeof(R) != tg.rbytes) {
     *msg = "tile store: step called with the other precision's arrays";
@@ -1092,8 +1106,8 @@ int tile_store_step_enqueue(TileStore* ts, const TileGeom& tg, const DevGrid& g,
                                              ts->count[nxt], ts->ovf_count + nxt, patch, ts->grow);
   {
     const dim3 fb((g.n1p + 63) / 64, (g.n2p + 3) / 4, g.n3p);
-    k_fold<R><<<fb, kB, 0, s>>>(tg, g, patch, jc, j_is_zero ? 0 : 1);
-    if (jraw) k_fold<double><<<fb, kB, 0, s>>>(tg, g, patch, jraw, 0);
+    k_fold<R><<<fb, kB, 0, s>>>(tg, g, patch, jc, j_is_zero ? 0 : 1, jscale_of(g, normalise), D);
+    if (jraw) k_fold<double><<<fb, kB, 0, s>>>(tg, g, patch, jraw, 0, jscale_of(g, false), D);
   }
   TS_CK(hipGetLastError());
   // the overflow count of this step goes to the pinned slot of this step
@@ -1133,19 +1147,21 @@ int tile_store_step_finish(TileStore* ts, const TileGeom& tg, const DevGrid& g, 
 template <typename R>
 int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf, R* jc,
                     double* patch, R alp, int initial, bool j_is_zero, double* jraw,
-                    hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg) {
+                    hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg,
+                    bool normalise) {
   if (int r = tile_store_step_enqueue<R>(ts, tg, g, ebf, jc, patch, alp, initial, j_is_zero,
-                                         jraw, s, ev_begin, ev_end, msg))
+                                         jraw, s, ev_begin, ev_end, msg, normalise))
     return r;
   return tile_store_step_finish(ts, tg, g, s, msg);
 }
 #define ALADYN_STEP_INST(R)                                                                    \
   template int tile_store_step<R>(TileStore*, const TileGeom&, const DevGrid&, const R*, R*,   \
                                   double*, R, int, bool, double*, hipStream_t, hipEvent_t,     \
-                                  hipEvent_t, std::string*);                                   \
+                                  hipEvent_t, std::string*, bool);                             \
   template int tile_store_step_enqueue<R>(TileStore*, const TileGeom&, const DevGrid&,         \
                                           const R*, R*, double*, R, int, bool, double*,        \
-                                          hipStream_t, hipEvent_t, hipEvent_t, std::string*);
+                                          hipStream_t, hipEvent_t, hipEvent_t, std::string*,   \
+                                          bool);
 ALADYN_STEP_INST(float)
 ALADYN_STEP_INST(double)
 #undef ALADYN_STEP_INST

@@ -123,6 +123,16 @@ struct TileRun {
   int* grow;
 };
 
+// Factors the fold applies to each current component as it writes jc: all
+// ones for the raw current, djc when the fold stands in for the
+// normalisation of curr_mpi_collect (curr_and_fields_util.f90:115-119).
+struct JScale {
+  double s[3];
+};
+__host__ __device__ inline JScale jscale_of(const DevGrid& g, bool normalise) {
+  return normalise ? JScale{{g.djc[0], g.djc[1], g.djc[2]}} : JScale{{1.0, 1.0, 1.0}};
+}
+
 int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg);
 const char* tile_geom_error(int code);
 
@@ -156,11 +166,13 @@ int tile_store_get(TileStore* ts, const TileGeom& tg, const DevGrid& g, double* 
 // float or double and must match the precision the geometry was made for.
 // The per-tile current patches are FP64 in both paths.  jraw, when given,
 // receives this species' raw current alone, summed in FP64 (the
-// continuity diagnostic).
+// continuity diagnostic).  With normalise, this species' current goes onto
+// jc multiplied by djc (the normalisation a collect without it then skips).
 template <typename R>
 int tile_store_step(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf, R* jc,
                     double* patch, R alp, int initial, bool j_is_zero, double* jraw,
-                    hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg);
+                    hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end, std::string* msg,
+                    bool normalise = false);
 // The two halves of tile_store_step: the device work (kernels and the copy
 // of the overflow count, nothing that waits, so it may be captured into a
 // graph) and the host side (buffer swap, overflow event, re-layout
@@ -169,20 +181,23 @@ template <typename R>
 int tile_store_step_enqueue(TileStore* ts, const TileGeom& tg, const DevGrid& g, const R* ebf,
                             R* jc, double* patch, R alp, int initial, bool j_is_zero,
                             double* jraw, hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end,
-                            std::string* msg);
+                            std::string* msg, bool normalise = false);
 int tile_store_step_finish(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
                            std::string* msg);
 int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
                         std::string* msg);
 // jc = (accumulate != 0 ? jc : 0) + the sum of the per-tile FP64 current
-// patches, rounded to the grid's type once per node
+// patches (times djc with normalise), rounded to the grid's type once per node
 template <typename R>
 void tile_store_fold(const TileGeom& tg, const DevGrid& g, const double* patch, R* jc,
-                     int accumulate, hipStream_t s);
+                     int accumulate, hipStream_t s, bool normalise = false);
+// den = the sum of component 0 of the per-tile patches (a density)
+void tile_store_fold_density(const TileGeom& tg, const DevGrid& g, const double* patch,
+                             double* den, hipStream_t s);
 // Batched runs: the step's descriptor of one store (the buffers it reads and
 // writes this step), the batched particle-side launches of a step (counter
 // clear, tile kernel over the work list of (run, tile) pairs, overflow
-// kernel, fold of the patches onto jc, the overflow counts into ovf_out),
+// kernel, fold of the patches onto jc times djc, the overflow counts into ovf_out),
 // and the host side of one store after such a step (buffer swap, re-layout
 // on cadence or when ovf_prev, the previous step's overflow count, is large).
 template <typename R>

@@ -122,6 +122,40 @@ module aladyn_gpu
       integer(c_int32_t), value :: iter, initial_time
     end function
 
+    ! Lp_cycle's dt_loc for the next steps (ALaDyn.F90:89-107)
+    integer(c_int) function aladyn_set_dt(ctx, dt) bind(C, name="aladyn_set_dt")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt
+    end function
+
+    ! many contexts stepped as one batch (one launch per kernel for all runs)
+    integer(c_int) function aladyn_batch_create(ctxs, n, use_graphs, batch) &
+        bind(C, name="aladyn_batch_create")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), intent(in) :: ctxs(*)
+      integer(c_int32_t), value :: n, use_graphs
+      type(c_ptr), intent(out) :: batch
+    end function
+
+    integer(c_int) function aladyn_batch_lpf2_evolve(batch, iter, initial_time) &
+        bind(C, name="aladyn_batch_lpf2_evolve")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: iter, initial_time
+    end function
+
+    integer(c_int) function aladyn_batch_synchronize(batch) &
+        bind(C, name="aladyn_batch_synchronize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+
+    integer(c_int) function aladyn_batch_destroy(batch) bind(C, name="aladyn_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+
     integer(c_int) function aladyn_set_graphs(ctx, enable) bind(C, name="aladyn_set_graphs")
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx
@@ -253,6 +287,38 @@ module aladyn_gpu
     integer(c_int) function aladyn_synchronize(ctx) bind(C, name="aladyn_synchronize")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+    end function
+
+    ! the envelope model: env(n1p, n2p, n3p, 4), oml the laser frequency
+    integer(c_int) function aladyn_put_envelope(ctx, oml, env) &
+        bind(C, name="aladyn_put_envelope")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: oml
+      real(c_double), intent(in) :: env(*)
+    end function
+
+    integer(c_int) function aladyn_get_envelope(ctx, env) bind(C, name="aladyn_get_envelope")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: env(*)
+    end function
+
+    ! replaces env_lpf2_evolve(iter) (env_evolve_in_time.f90:196-361)
+    integer(c_int) function aladyn_env_lpf2_evolve(ctx, iter) &
+        bind(C, name="aladyn_env_lpf2_evolve")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: iter
+    end function
+
+    integer(c_int) function aladyn_env_kernel_timing(ctx, reset, push_ms, move_ms, launches) &
+        bind(C, name="aladyn_env_kernel_timing")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: reset
+      real(c_double), intent(out) :: push_ms, move_ms
+      integer(c_int64_t), intent(out) :: launches
     end function
 
     integer(c_int) function aladyn_kernel_timing(ctx, reset, particle_ms, field_ms, launches) &

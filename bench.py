@@ -67,9 +67,16 @@ def measured_traffic(nx, ny, nz):
 
 
 def calibrate_copy():
-    """One device-to-device copy of 1 GiB (the reference kernel for the HBM
-    byte counters): 1 GiB read, 1 GiB written."""
+    """The reference kernels for the HBM byte counters: one device-to-device
+    copy of 1 GiB (torch: 1 GiB read, 1 GiB written), then two of the
+    library's own kernels on the 256x128x128 grid in FP32: k_field_energy
+    (reads one field component's interior, 256*128*128*4 bytes, once per
+    component) and k_fill (writes jc whole, ncell*3*4 bytes)."""
+    import numpy as np
     import torch
+
+    from aladyn_amd import PREC_F32, LPStep
+    from aladyn_amd.config import LPConfig
 
     n = (1 << 30) // 4
     a = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -77,7 +84,17 @@ def calibrate_copy():
     torch.cuda.synchronize()
     b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"calibration_copy_bytes": n * 4}), flush=True)
+    del a, b
+    cfg = LPConfig(ndim=3, nx=256, ny=128, nz=128, k0=20.0, yx_rat=4.0, zx_rat=4.0, cfl=0.8,
+                   n_over_nc=0.005)
+    with LPStep(cfg, PREC_F32) as gpu:
+        gpu.put_fields(np.ones(cfg.field_shape, order="F"))
+        gpu.field_energy()
+        gpu.curr_clean()
+        gpu.synchronize()
+    print(json.dumps({"calibration_copy_bytes": n * 4,
+                      "k_field_energy_read_bytes": cfg.nx * cfg.ny * cfg.nz * 4,
+                      "k_fill_write_bytes": cfg.n1p * cfg.n2p * cfg.n3p * 3 * 4}), flush=True)
 
 
 def log(*a):
