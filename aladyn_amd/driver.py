@@ -72,15 +72,25 @@ def run_interleaved(steppers, steps: int, *, start_iter: int = 0, initial_time: 
 
 
 def run_batched(steppers, steps: int, *, start_iter: int = 0, initial_time: bool = True,
-                graphs: bool = True):
+                graphs: bool = True, groups: int = 1):
     """Advances independent runs (one LPStep each, one species, same precision
     and tile configuration) by `steps` lpf2_evolve steps with every kernel of
-    a step launched once for all runs (aladyn_batch_lpf2_evolve: the particle
-    kernel over the work list of every run's tiles), the step's launches
-    replayed as one HIP graph.  The result of each run is that of running it
-    alone."""
+    a step launched once for all runs of a batch (aladyn_batch_lpf2_evolve:
+    the particle kernel over the work list of every run's tiles), the step's
+    launches replayed as one HIP graph.  groups > 1 splits the runs into that
+    many batches, each on its own stream, stepped in turn, so that one
+    batch's kernels fill the GPU while another's drain.  The result of each
+    run is that of running it alone."""
+    from contextlib import ExitStack
+
     from .lp_step import LPBatch
 
-    with LPBatch(steppers, graphs=graphs) as b:
+    groups = max(1, min(groups, len(steppers)))
+    parts = [steppers[g::groups] for g in range(groups)]
+    with ExitStack() as stack:
+        batches = [stack.enter_context(LPBatch(p, graphs=graphs)) for p in parts]
         for k in range(steps):
-            b.lpf2_evolve(start_iter + k, initial_time=initial_time and k == 0)
+            for b in batches:
+                b.lpf2_evolve(start_iter + k, initial_time=initial_time and k == 0)
+        for b in batches:
+            b.synchronize()

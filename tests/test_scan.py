@@ -184,22 +184,22 @@ def _make(cases, prec, **kw):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("graphs", [False, True])
-def test_batched_runs_match_runs_alone(graphs):
+@pytest.mark.parametrize("graphs,groups", [(False, 1), (True, 1), (True, 2)])
+def test_batched_runs_match_runs_alone(graphs, groups):
     from aladyn_amd import PREC_F64
     from aladyn_amd.driver import run_batched
     from conftest import rel_maxerr
 
     runs = [scan_2d_params()[i] for i in (13, 24, 3, 40)]
     cases = [scan_2d_case(r) for r in runs]
-    print(f"batch of {len(cases)} runs, graphs={graphs}", flush=True)
+    print(f"batch of {len(cases)} runs, graphs={graphs}, {groups} batches", flush=True)
     alone = _make(cases, PREC_F64, sort_every=4, track_ids=True)
     batch = _make(cases, PREC_F64, sort_every=4, track_ids=True)
     try:
         for g in alone:
             for it in range(20):
                 g.lpf2_evolve(it, initial_time=(it == 0))
-        run_batched(batch, 20, graphs=graphs)
+        run_batched(batch, 20, graphs=graphs, groups=groups)
         for r, a, b in zip(runs, alone, batch):
             pa, pb = a.get_species(0), b.get_species(0)
             assert pa.shape == pb.shape, (r["run"], pa.shape, pb.shape)
@@ -220,14 +220,16 @@ def test_scan_fp32_against_fp64_batched():
     cases = [scan_2d_case(r) for r in runs]
     npart = sum(c[1].shape[0] for c in cases)
     m, wall = {}, {}
-    for name, prec in (("f64", PREC_F64), ("f32", PREC_F32)):
-        print(f"scan batch {name}: {len(cases)} runs, {npart} particles", flush=True)
+    # FP64 as one batch, FP32 as four batches of 16 stepped in turn
+    for name, prec, groups in (("f64", PREC_F64, 1), ("f32", PREC_F32, 4)):
+        print(f"scan batch {name}: {len(cases)} runs, {npart} particles, {groups} batches",
+              flush=True)
         ctx = _make(cases, prec)
         try:
             for g in ctx:
                 g.synchronize()
             t = time.perf_counter()
-            run_batched(ctx, NSTEPS, graphs=True)
+            run_batched(ctx, NSTEPS, graphs=True, groups=groups)
             wall[name] = time.perf_counter() - t
             m[name] = [_moments(g) for g in ctx]
         finally:
