@@ -31,20 +31,6 @@ static unsigned blocks_for(int64_t n) {
   return (unsigned)b;
 }
 
-// ------------------------------------------------------------- shapes
-// qqh_*_spline for one axis (grid_part_lib.f90:145-210, 261-330) from the
-// nearest node N and offset s (xx = N + s, s in [-1/2, 1/2)): quadratic
-// weights at integer nodes from node N-1, and at half-integer nodes from
-// node ih = int(xx) - 1, i.e. N-2 for s < 0 and N-1 otherwise, with offset
-// xx - int(xx) - 1/2 = s + 1/2 or s - 1/2.  hb = ih - (N-1).
-template <typename R>
-__device__ inline void qqh(R s, R a[3], R ah[3], int& hb) {
-  qweights<R>(s, a);
-  const bool neg = s < R(0);
-  qweights<R>(neg ? s + R(0.5) : s - R(0.5), ah);
-  hb = neg ? -1 : 0;
-}
-
 // sum_{k1,j1,i1} wx[i1] wy[j1] wz[k1] f(i0+i1, j0+j1, k0+k1), in the
 // reference's loop order (x innermost, dvol = wy*wz first)
 template <int D, typename R>
@@ -292,24 +278,6 @@ __device__ inline void env_stage(const DevGrid& g, const EnvTile& e, const R* sr
     const int i = e.T0[0] - 2 + li, j = e.T0[1] - 2 + lj, k = D == 3 ? e.T0[2] - 2 + lk : 1;
     dst[l] = (i <= g.n1p && j <= g.n2p && k <= g.n3p) ? src[fofs(g, i, j, k)] : R(0);
   }
-}
-
-// contract3 over an LDS box (x stride 1, y stride GX, z stride GXY), the
-// same loop order
-template <int D, typename R>
-__device__ inline R contract3l(const R* f, int GX, int GXY, const R wx[3], const R wy[3],
-                               const R wz[3], int i0, int j0, int k0) {
-  R r = R(0);
-  const int nk = D == 3 ? 3 : 1;
-  for (int k1 = 0; k1 < nk; k1++)
-#pragma unroll
-    for (int j1 = 0; j1 < 3; j1++) {
-      const R dvol = D == 3 ? wy[j1] * wz[k1] : wy[j1];
-      const R* row = f + i0 + GX * (j0 + j1) + GXY * (k0 + k1);
-#pragma unroll
-      for (int i1 = 0; i1 < 3; i1++) r = r + (dvol * wx[i1]) * row[i1];
-    }
-  return r;
 }
 
 // Adds to an FP64 LDS patch over nodes o .. o+J-1 (component stride pvol).
@@ -874,18 +842,31 @@ int env_particles_move(TileStore* ts, const TileGeom& tg, const DevGrid& g, cons
     fill<int>(ts->ovf_count + nxt, 1, 0, s);
     const size_t lds = env_move_lds<R>(tg, g);
     if (ev_begin) (void)hipEventRecord(ev_begin, s);
-    // every tile writes its whole patch; the overflow list adds to them
+    // every tile writes its whole patch; the overflow list adds to them.
+    // The tile kernel of the LP step with the envelope's advance (cell-sorted
+    // chunks, register sums of the current) where it is compiled for the
+    // workgroup size, else the simpler tile workgroups above.
+    const hipError_t tk = launch_env_move_kernel<R>(
+        tg, g, sc, je, ts->buf[cur], ts->buf[nxt], ts->seg_start, ts->seg_cap, ts->count[cur],
+        ts->count[nxt], ts->ovf_count + cur, ts->ovf_count + nxt, patch, ts->grow, s);
+    if (tk != hipSuccess && tk != hipErrorInvalidConfiguration) {
+      *msg = "envelope tile kernel launch failed";
+      return ALADYN_EHIP;
+    }
+    const bool tiled = tk == hipSuccess;
     if (g.ndim == 2) {
-      k_env_move_tile<2, R><<<tg.ntiles, kEB, lds, s>>>(
-          tg, g, sc, ts->buf[cur], ts->buf[nxt], ts->seg_start, ts->seg_cap, ts->count[cur], je,
-          ts->count[nxt], ts->ovf_count + nxt, patch, ts->grow);
+      if (!tiled)
+        k_env_move_tile<2, R><<<tg.ntiles, kEB, lds, s>>>(
+            tg, g, sc, ts->buf[cur], ts->buf[nxt], ts->seg_start, ts->seg_cap, ts->count[cur],
+            je, ts->count[nxt], ts->ovf_count + nxt, patch, ts->grow);
       k_env_move<2, R><<<64, kEB, 0, s>>>(tg, g, sc, ts->buf[cur], ts->buf[nxt], ts->seg_start,
                                           ts->seg_cap, ts->ovf_count + cur, je, ts->count[nxt],
                                           ts->ovf_count + nxt, patch, ts->grow);
     } else {
-      k_env_move_tile<3, R><<<tg.ntiles, kEB, lds, s>>>(
-          tg, g, sc, ts->buf[cur], ts->buf[nxt], ts->seg_start, ts->seg_cap, ts->count[cur], je,
-          ts->count[nxt], ts->ovf_count + nxt, patch, ts->grow);
+      if (!tiled)
+        k_env_move_tile<3, R><<<tg.ntiles, kEB, lds, s>>>(
+            tg, g, sc, ts->buf[cur], ts->buf[nxt], ts->seg_start, ts->seg_cap, ts->count[cur],
+            je, ts->count[nxt], ts->ovf_count + nxt, patch, ts->grow);
       k_env_move<3, R><<<64, kEB, 0, s>>>(tg, g, sc, ts->buf[cur], ts->buf[nxt], ts->seg_start,
                                           ts->seg_cap, ts->ovf_count + cur, je, ts->count[nxt],
                                           ts->ovf_count + nxt, patch, ts->grow);

@@ -352,6 +352,38 @@ __device__ inline int wave_max(int v) {
   return v;
 }
 
+// qqh_*_spline for one axis (grid_part_lib.f90:145-210, 261-330) from the
+// nearest node N and offset s (xx = N + s, s in [-1/2, 1/2)): quadratic
+// weights at integer nodes from node N-1, and at half-integer nodes from
+// node ih = int(xx) - 1, i.e. N-2 for s < 0 and N-1 otherwise, with offset
+// xx - int(xx) - 1/2 = s + 1/2 or s - 1/2.  hb = ih - (N-1).
+template <typename R>
+__device__ inline void qqh(R s, R a[3], R ah[3], int& hb) {
+  qweights<R>(s, a);
+  const bool neg = s < R(0);
+  qweights<R>(neg ? s + R(0.5) : s - R(0.5), ah);
+  hb = neg ? -1 : 0;
+}
+
+// sum_{k1,j1,i1} wx[i1] wy[j1] wz[k1] f(i0+i1, j0+j1, k0+k1) over an LDS box
+// (x stride 1, y stride GX, z stride GXY), in the reference's loop order
+// (x innermost, dvol = wy*wz first)
+template <int D, typename R>
+__device__ inline R contract3l(const R* f, int GX, int GXY, const R wx[3], const R wy[3],
+                               const R wz[3], int i0, int j0, int k0) {
+  R r = R(0);
+  const int nk = D == 3 ? 3 : 1;
+  for (int k1 = 0; k1 < nk; k1++)
+#pragma unroll
+    for (int j1 = 0; j1 < 3; j1++) {
+      const R dvol = D == 3 ? wy[j1] * wz[k1] : wy[j1];
+      const R* row = f + i0 + GX * (j0 + j1) + GXY * (k0 + k1);
+#pragma unroll
+      for (int i1 = 0; i1 < 3; i1++) r = r + (dvol * wx[i1]) * row[i1];
+    }
+  return r;
+}
+
 __device__ inline int tile_of(const TileGeom& tg, int cell, int D) {
   const int i = cell & ((1 << tg.bx) - 1);
   const int j = (cell >> tg.bx) & ((1 << tg.by) - 1);

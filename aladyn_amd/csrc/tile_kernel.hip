@@ -25,6 +25,8 @@
 //   4. write the LDS current patch to HBM for the fold.
 // LDS accumulation is FP64 because LDS FP32 atomic adds are far slower than
 // FP64 ones on gfx950; there are no global float atomics.
+#include <type_traits>
+
 #include "tile_device.hpp"
 
 
@@ -309,10 +311,29 @@ struct CellAcc {
 //   M   per wave: the record words of MCAP movers (word-major), their
 //       destination tiles and ids
 //   offs, fillc int [NCL+1], [NCL]; misc int [4]; wsum int [NW]; perm u16 [kChunk]
-template <int D, int TX, int TY, int TZ, int NT, typename R>
+//
+// What the kernel does to a particle before it moves, and so what its field
+// patch holds:
+//   LpPush   the LP step's gather + exact-gamma push: alp*q*F of the nfield
+//            components over nodes T0-1 .. T0+T (the reach of the qlh shapes)
+//   EnvMove  the envelope step's second half (set_env_grad_interp and the
+//            gamma of lpf_env_positions): je(1:D+1) = |A|^2/2 and its
+//            gradient over nodes T0-2 .. T0+T (the reach of the qqh shapes)
+// Everything after (removal, deposit, re-binning) is the same.
+struct LpPush {
+  template <int D> static constexpr int NF = D == 3 ? 6 : 3;
+  static constexpr int LO = 1, EXT = 2;  // patch nodes T0-LO .. T0-LO+T+EXT-1
+};
+struct EnvMove {
+  template <int D> static constexpr int NF = D + 1;
+  static constexpr int LO = 2, EXT = 3;
+};
+
+template <int D, int TX, int TY, int TZ, int NT, typename R, class P = LpPush>
 struct TileShape {
-  static constexpr int NCL = TX * TY * TZ, NF = D == 3 ? 6 : 3, NW = NT / 64;
-  static constexpr int GX = TX + 2, GY = TY + 2, GZ = D == 3 ? TZ + 2 : 1, GVOL = GX * GY * GZ;
+  static constexpr int NCL = TX * TY * TZ, NF = P::template NF<D>, NW = NT / 64;
+  static constexpr int GX = TX + P::EXT, GY = TY + P::EXT, GZ = D == 3 ? TZ + P::EXT : 1;
+  static constexpr int GVOL = GX * GY * GZ;
   static constexpr int JX = TX + 4, JY = TY + 4, JZ = D == 3 ? TZ + 4 : 1, PVOL = JX * JY * JZ;
   static constexpr int QCAP = 64;  // one entry per lane: a drain takes all
   static constexpr int F_BYTES = (NF * GVOL * (int)sizeof(R) + 15) & ~15;
@@ -495,16 +516,51 @@ __device__ inline void flush_movers(const MoverStage<R, MCAP>& ms, int n, const 
   }
 }
 
+// The envelope step's advance of one particle (set_env_grad_interp,
+// grid_part_connect.f90:760-882, and the gamma of lpf_env_positions,
+// boris_push.f90:225-295): F and grad F at x^n with qqh shapes from the
+// patch at node N-2 (fb), gamp = sqrt(1 + p^2 + F) corrected by the
+// half-step change of F along p; returns dt/gamp.  a0 receives the quadratic
+// weights at x^n that the deposit reuses.
+template <int D, int GX, int GXY, int GVOL, typename R>
+__device__ inline R env_advance(const Rec<R>& q, const R* fb, R a0[3][3], R dt) {
+  R axh[3], ayh[3], azh[3] = {R(1), R(0), R(0)};
+  int hx, hy, hz = 0;
+  qqh<R>(q.s[0], a0[0], axh, hx);
+  qqh<R>(q.s[1], a0[1], ayh, hy);
+  if (D == 3) qqh<R>(q.s[2], a0[2], azh, hz);
+  const int k1 = D == 3 ? 1 : 0;  // integer-shape base N-1 is local 1
+  R ap[4] = {R(0), R(0), R(0), R(0)};
+  ap[0] = contract3l<D, R>(fb + GVOL, GX, GXY, axh, a0[1], a0[2], 1 + hx, 1, k1);
+  ap[D] = contract3l<D, R>(fb, GX, GXY, a0[0], a0[1], a0[2], 1, 1, k1);
+  ap[1] = contract3l<D, R>(fb + 2 * GVOL, GX, GXY, a0[0], ayh, a0[2], 1, 1 + hy, k1);
+  if (D == 3) ap[2] = contract3l<D, R>(fb + 3 * GVOL, GX, GXY, a0[0], a0[1], azh, 1, 1, 1 + hz);
+  const R one = R(1);
+  R pp2 = R(0), pv = R(0);
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    pp2 = pp2 + q.p[d] * q.p[d];
+    pv = pv + q.p[d] * ap[d];
+  }
+  const R gam2 = one + pp2 + ap[D];
+  const R gam = sqrt(gam2);
+  const R gam3 = gam2 * gam;
+  const R b2 = R(0.25) * pv;
+  const R gam_inv = (one / gam) * (one - dt * b2 / gam3);
+  return dt * gam_inv;
+}
+
 // The workgroup of tile t.
-template <int D, int TX, int TY, int TZ, int NT, typename R>
+template <int D, int TX, int TY, int TZ, int NT, typename R, class P>
 __device__ inline void tile_body(const TileArgs& ta, const StepConst<R>& sc,
                                  const R* __restrict__ ebf, const RecPtr& A, const RecPtr& B,
                                  const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
                                  const int* __restrict__ cnt_cur, int* __restrict__ cnt_next,
                                  int* __restrict__ ovf_cur, int* __restrict__ ovf_next,
                                  double* __restrict__ patch, int* __restrict__ grow, const int t) {
-  using S = TileShape<D, TX, TY, TZ, NT, R>;
+  using S = TileShape<D, TX, TY, TZ, NT, R, P>;
   constexpr int NCL = S::NCL, NF = S::NF, GX = S::GX, GY = S::GY, GVOL = S::GVOL;
+  constexpr int LO = P::LO;
   constexpr int JX = S::JX, JY = S::JY, PVOL = S::PVOL, QCAP = S::QCAP;
   extern __shared__ double lds_d[];
   char* lds = (char*)lds_d;
@@ -539,9 +595,9 @@ __device__ inline void tile_body(const TileArgs& ta, const StepConst<R>& sc,
       lcv[u] = q < m ? rec_cell<R>(A, base + c0 + q) : kHole;
     }
   };
-  // field patch: nodes T0-1 .. T0+T; every load of the thread, and those of
-  // the first chunk's cells, is issued before the first LDS store, so the
-  // HBM latency is paid once per tile
+  // field patch: nodes T0-LO .. T0-LO+G-1; every load of the thread, and
+  // those of the first chunk's cells, is issued before the first LDS store,
+  // so the HBM latency is paid once per tile
   {
     // rows of GX consecutive x nodes: row = (c*GZ + lk)*GY + lj holds LDS
     // entries row*GX .. row*GX+GX-1; a pass covers RPP rows, one node per
@@ -549,27 +605,28 @@ __device__ inline void tile_body(const TileArgs& ta, const StepConst<R>& sc,
     constexpr int GZ = S::GZ, ROWS = NF * GZ * GY, RPP = NT / GX;
     constexpr int PASSES = (ROWS + RPP - 1) / RPP;
     const int li = me % GX, r0 = me / GX;
-    const int i = T0x - 1 + li;
+    const int i = T0x - LO + li;
     const bool xin = r0 < RPP && i <= ta.n1p;
-    // node (T0x-1+li, T0y-1, T0z-1) of component 0
+    // node (T0x-LO+li, T0y-LO, T0z-LO) of component 0
     const R* eb = ebf + (i - 1) +
-                  (int64_t)ta.n1p * ((T0y - 2) + (int64_t)ta.n2p * (D == 3 ? T0z - 2 : 0));
+                  (int64_t)ta.n1p *
+                      ((T0y - LO - 1) + (int64_t)ta.n2p * (D == 3 ? T0z - LO - 1 : 0));
     R fv[PASSES];
 #pragma unroll
     for (int u = 0; u < PASSES; u++) {
       const int row = r0 + u * RPP;
       const int c = row / (GZ * GY), rem = row - c * (GZ * GY);
       const int lk = rem / GY, lj = rem - lk * GY;
-      const int j = T0y - 1 + lj, k = D == 3 ? T0z - 1 + lk : 1;
+      const int j = T0y - LO + lj, k = D == 3 ? T0z - LO + lk : 1;
       fv[u] = R(0);
       if (xin && row < ROWS && j <= ta.n2p && k <= ta.n3p)
         fv[u] = eb[(int64_t)c * ta.ncell + ta.n1p * (lj + ta.n2p * lk)];
     }
     load_cells(0);
-    // the patch holds alp*charge*F: the push's charge multiply and alp
+    // the LP patch holds alp*charge*F: the push's charge multiply and alp
     // factor (field_charge_multiply, boris_push.f90:75-168) are done here
     // once per node instead of once per particle
-    const R fs = sc.alp * sc.qch;
+    const R fs = std::is_same<P, LpPush>::value ? sc.alp * sc.qch : R(1);
 #pragma unroll
     for (int u = 0; u < PASSES; u++) {
       const int row = r0 + u * RPP;
@@ -685,20 +742,25 @@ __device__ inline void tile_body(const TileArgs& ta, const StepConst<R>& sc,
         decode_cell<D, R>(q, ta.bx, ta.by);
         const int li = q.N[0] - T0x, lj = q.N[1] - T0y, lk = D == 3 ? q.N[2] - T0z : 0;
         lc = li + TX * (lj + TY * lk);
-        // LDS field patch base at node N-1 of the particle's cell
+        // LDS field patch base at node N-LO of the particle's cell
         const R* fb = Fl + li + GX * (lj + GY * lk);
-        const auto F = [&](int c, int i1, int j1, int k1) -> R {
-          return fb[c * GVOL + i1 + GX * (j1 + GY * k1)];
-        };
-        R ef[6];
-        gather<D, R>(q.s, F, ef, a0);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           so[d] = q.s[d];
           N0[d] = q.N[d];
         }
-        // the fields are already alp*charge*F
-        const R dtg = push<D, R>(q.p, ef, R(1), R(1), sc.dt, sc.initial);
+        R dtg;
+        if constexpr (std::is_same<P, LpPush>::value) {
+          const auto F = [&](int c, int i1, int j1, int k1) -> R {
+            return fb[c * GVOL + i1 + GX * (j1 + GY * k1)];
+          };
+          R ef[6];
+          gather<D, R>(q.s, F, ef, a0);
+          // the fields are already alp*charge*F
+          dtg = push<D, R>(q.p, ef, R(1), R(1), sc.dt, sc.initial);
+        } else {
+          dtg = env_advance<D, GX, GX * GY, GVOL, R>(q, fb, a0, sc.dt);
+        }
         keep = move_and_test<D, R>(q, dtg, sc.inv, sc.nn);
         // real(sp) :: wght = charge*wgh (grid_part_connect.f90:983, 1205):
         // the product is rounded to FP32 in both paths
@@ -846,17 +908,17 @@ __device__ inline void tile_body(const TileArgs& ta, const StepConst<R>& sc,
   }
   if (acc_lc >= 0) acc.template flush<JX, JY, PVOL>(patch_node(acc_lc));
   __syncthreads();
-  double* P = patch + (int64_t)t * D * PVOL;
-  for (int q = threadIdx.x; q < D * PVOL; q += NT) P[q] = Jl[q];
+  double* Pt = patch + (int64_t)t * D * PVOL;
+  for (int q = threadIdx.x; q < D * PVOL; q += NT) Pt[q] = Jl[q];
 }
 
-template <int D, int TX, int TY, int TZ, int NT, typename R>
+template <int D, int TX, int TY, int TZ, int NT, typename R, class P = LpPush>
 __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile(
     TileArgs ta, StepConst<R> sc, const R* __restrict__ ebf, RecPtr A, RecPtr B,
     const int* __restrict__ seg_start, const int* __restrict__ seg_cap,
     const int* __restrict__ cnt_cur, int* __restrict__ cnt_next, int* __restrict__ ovf_cur,
     int* __restrict__ ovf_next, double* __restrict__ patch, int* __restrict__ grow) {
-  tile_body<D, TX, TY, TZ, NT, R>(ta, sc, ebf, A, B, seg_start, seg_cap, cnt_cur, cnt_next,
+  tile_body<D, TX, TY, TZ, NT, R, P>(ta, sc, ebf, A, B, seg_start, seg_cap, cnt_cur, cnt_next,
                                   ovf_cur, ovf_next, patch, grow, (int)blockIdx.x);
 }
 
@@ -867,7 +929,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile_batc
     const TileRun<R>* __restrict__ runs, const int2* __restrict__ work) {
   const int2 w = work[blockIdx.x];
   const TileRun<R>& r = runs[w.x];
-  tile_body<D, TX, TY, TZ, NT, R>(r.ta, r.sc, r.ebf, r.A, r.B, r.seg_start, r.seg_cap,
+  tile_body<D, TX, TY, TZ, NT, R, LpPush>(r.ta, r.sc, r.ebf, r.A, r.B, r.seg_start, r.seg_cap,
                                   r.cnt_cur, r.cnt_next, r.ovf_cur, r.ovf_next, r.patch, r.grow,
                                   w.y);
 }
@@ -879,18 +941,19 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 512 / NT : 1) void k_tile_batc
 // workgroup holds twice the per-wave queues in LDS.  The register sums of
 // the current need more than the 128 VGPRs a 1024-thread workgroup would
 // allow, so NT stays at most 512.
-template <int D, int TX, int TY, int TZ, int NT, typename R>
+template <int D, int TX, int TY, int TZ, int NT, typename R, class P = LpPush>
 static int lds_bytes_for() {
-  return TileShape<D, TX, TY, TZ, NT, R>::BYTES;
+  return TileShape<D, TX, TY, TZ, NT, R, P>::BYTES;
 }
 
-template <int D, int TX, int TY, int TZ, int NT, typename R>
+template <int D, int TX, int TY, int TZ, int NT, typename R, class P = LpPush>
 static hipError_t launch(const TileGeom& tg, const TileArgs& ta, const StepConst<R>& sc,
                          const R* ebf, const RecPtr& A, const RecPtr& B, const int* ss,
                          const int* sc_, const int* cc, int* cn, int* oc, int* on, double* patch,
                          int* grow, hipStream_t s) {
-  k_tile<D, TX, TY, TZ, NT, R><<<tg.ntiles, NT, lds_bytes_for<D, TX, TY, TZ, NT, R>(), s>>>(
-      ta, sc, ebf, A, B, ss, sc_, cc, cn, oc, on, patch, grow);
+  k_tile<D, TX, TY, TZ, NT, R, P>
+      <<<tg.ntiles, NT, lds_bytes_for<D, TX, TY, TZ, NT, R, P>(), s>>>(
+          ta, sc, ebf, A, B, ss, sc_, cc, cn, oc, on, patch, grow);
   return hipGetLastError();
 }
 
@@ -949,6 +1012,41 @@ hipError_t launch_tile_kernel(const TileGeom& tg, const DevGrid& g, const StepCo
 #undef ALADYN_TILE_CASE
   return hipErrorInvalidConfiguration;
 }
+
+// The envelope step's second half on the tile kernel (P = EnvMove): je
+// holds je(1:D+1); compiled for one thread per node-cell of each tile shape
+// (hipErrorInvalidConfiguration for two).
+template <typename R>
+hipError_t launch_env_move_kernel(const TileGeom& tg, const DevGrid& g, const StepConst<R>& sc,
+                                  const R* je, const RecPtr& A, const RecPtr& B,
+                                  const int* seg_start, const int* seg_cap, const int* cnt_cur,
+                                  int* cnt_next, int* ovf_cur, int* ovf_next, double* patch,
+                                  int* grow, hipStream_t s) {
+  TileArgs ta{tg.nt[0], tg.nt[1], tg.bx, tg.by, g.n1p, g.n2p, g.n3p, g.ncell};
+  const int* T = tg.T;
+#define ALADYN_TILE_CASE(DD, X, Y, Z, NT)                                                    \
+  if (g.ndim == DD && T[0] == X && T[1] == Y && (DD == 2 || T[2] == Z) && tg.block == NT)    \
+    return launch<DD, X, Y, Z, NT, R, EnvMove>(tg, ta, sc, je, A, B, seg_start, seg_cap,     \
+                                               cnt_cur, cnt_next, ovf_cur, ovf_next, patch,  \
+                                               grow, s);
+  ALADYN_TILE_CASE(3, 8, 8, 8, 512)
+  ALADYN_TILE_CASE(3, 8, 8, 4, 256)
+  if constexpr (sizeof(R) == 4) ALADYN_TILE_CASE(3, 16, 8, 4, 512)
+  ALADYN_TILE_CASE(2, 16, 16, 1, 256)
+  ALADYN_TILE_CASE(2, 32, 16, 1, 512)
+#undef ALADYN_TILE_CASE
+  return hipErrorInvalidConfiguration;
+}
+template hipError_t launch_env_move_kernel<float>(const TileGeom&, const DevGrid&,
+                                                  const StepConst<float>&, const float*,
+                                                  const RecPtr&, const RecPtr&, const int*,
+                                                  const int*, const int*, int*, int*, int*,
+                                                  double*, int*, hipStream_t);
+template hipError_t launch_env_move_kernel<double>(const TileGeom&, const DevGrid&,
+                                                   const StepConst<double>&, const double*,
+                                                   const RecPtr&, const RecPtr&, const int*,
+                                                   const int*, const int*, int*, int*, int*,
+                                                   double*, int*, hipStream_t);
 
 template <int D, int TX, int TY, int TZ, int NT, typename R>
 static hipError_t launch_b(const TileRun<R>* runs, const int2* work, int nwork, hipStream_t s) {

@@ -134,6 +134,15 @@ __host__ __device__ inline JScale jscale_of(const DevGrid& g, bool normalise) {
 }
 
 int tile_geom_init(const DevGrid& g, const aladyn_config& cfg, TileGeom* tg);
+// The tile kernel with the envelope step's advance in place of the LP push
+// (je = je(1:D+1)); hipErrorInvalidConfiguration for a workgroup size it is
+// not compiled for.
+template <typename R>
+hipError_t launch_env_move_kernel(const TileGeom& tg, const DevGrid& g, const StepConst<R>& sc,
+                                  const R* je, const RecPtr& A, const RecPtr& B,
+                                  const int* seg_start, const int* seg_cap, const int* cnt_cur,
+                                  int* cnt_next, int* ovf_cur, int* ovf_next, double* patch,
+                                  int* grow, hipStream_t s);
 const char* tile_geom_error(int code);
 
 int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const double* part,
