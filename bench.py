@@ -317,9 +317,10 @@ def bench_env(args):
                    "particles": int(r32["n0"]), "cells": cells},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_env_push<3, float>" if k_b == push_b else "k_env_move<3, float>",
+                     "kernel": ("k_env_push_tile<3, float> with the density fold"
+                                if k_b == push_b else "k_tile<3, 8, 8, 4, 256, float, EnvMove>"),
                      "kernel_ms": k_ms, "algorithmic_bytes_per_launch": k_b},
-        "kernels_ms": {"k_env_push": r32["push_ms"], "k_env_move": r32["move_ms"]},
+        "kernels_ms": {"push phase": r32["push_ms"], "move phase": r32["move_ms"]},
         "cpu_baseline": None,
     }
     if r64:
