@@ -55,14 +55,14 @@ def _case(ndim, der_ord=2):
     return cfg, part, ebf, env
 
 
-def _run(cfg, part, ebf, env, precision, nsteps):
+def _run(cfg, part, ebf, env, precision, nsteps, tile=None, block=0):
     orc = Oracle(cfg)
     orc.set_om0(OML)
     p = part.copy(order="F")
     e = ebf.copy(order="F")
     a = env.copy(order="F")
     jc = np.zeros((cfg.n1p, cfg.n2p, cfg.n3p, cfg.curr_ndim + 1), order="F")
-    with LPStep(cfg, precision, track_ids=True) as gpu:
+    with LPStep(cfg, precision, track_ids=True, tile=tile, block=block) as gpu:
         gpu.put_species(0, part)
         gpu.put_fields(ebf)
         gpu.put_envelope(env, OML)
@@ -103,6 +103,18 @@ def test_env_step_high_order_matches_oracle(der_ord):
     print(f"envelope 2-D der_ord={der_ord}", flush=True)
     cfg, part, ebf, env = _case(2, der_ord)
     _check(cfg, _run(cfg, part, ebf, env, PREC_F64, 3), 1e-10)
+
+
+@pytest.mark.parametrize("ndim,tile,block", [(3, (8, 8, 8), 0), (3, (16, 8, 4), 0),
+                                             (3, (8, 8, 4), 512), (2, (32, 16), 0),
+                                             (2, (16, 16), 512)])
+def test_env_step_other_tiles_match_oracle(ndim, tile, block):
+    """The other tile shapes the move phase's tile kernel is compiled for,
+    and two threads per node-cell, for which it is not (the move phase then
+    runs on the envelope's own tile workgroups): FP32, three steps."""
+    print(f"envelope {ndim}-D tile={tile} block={block}", flush=True)
+    cfg, part, ebf, env = _case(ndim)
+    _check(cfg, _run(cfg, part, ebf, env, PREC_F32, 3, tile=tile, block=block), 1e-4)
 
 
 def test_env_steps_with_moving_window_match_oracle():
