@@ -77,6 +77,7 @@ SIGNATURES = {
     "aladyn_field_energy": [_P, _D],
     "aladyn_grid_charge": [_P, C.c_int32, _D],
     "aladyn_grid_charge_noalloc": [_P, C.c_int32, _D],
+    "aladyn_beam_fields": [_P, C.c_int32, C.c_double, C.c_double],
     "aladyn_continuity_residual": [_P, _D],
     "aladyn_set_diagnostics": [_P, C.c_int32],
     "aladyn_synchronize": [_P],

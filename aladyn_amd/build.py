@@ -17,7 +17,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libaladyn_gpu.so"
 SOURCES = ["capi.hip", "fields.hip", "tile_kernel.hip", "tile_store.hip", "reductions.hip",
-           "envelope.hip"]
+           "envelope.hip", "beam.hip"]
 ARCH = os.environ.get("ALADYN_OFFLOAD_ARCH", "gfx950")
 
 

@@ -1172,6 +1172,27 @@ extern "C" int aladyn_grid_charge(aladyn_ctx* c, int32_t ic, double* rho) {
   return ALADYN_OK;
 }
 
+extern "C" int aladyn_beam_fields(aladyn_ctx* c, int32_t ic, double gam2, double bet0) {
+  if (int r = check_species(c, ic)) return r;
+  if (!c->have_fields) return fail(c, ALADYN_ESTATE, "beam_fields before put_fields");
+  if (!(gam2 >= 1.0) || !std::isfinite(gam2) || !(bet0 >= 0.0 && bet0 < 1.0))
+    return fail(c, ALADYN_EINVAL, "beam_fields: need gam2 >= 1 and 0 <= bet0 < 1");
+  const DevGrid& g = c->g;
+  double* rho = nullptr;
+  HIPCK(c, hipMalloc(&rho, (size_t)g.ncell * sizeof(double)));
+  fill<double>(rho, g.ncell, 0.0, c->stream);
+  int r = grid_charge_dev(c, ic, rho);
+  hipError_t e = hipSuccess;
+  if (r == 0)
+    e = f64(c) ? beam_fields<double>(g, rho, gam2, bet0, c->ebf64, c->stream)
+               : beam_fields<float>(g, rho, gam2, bet0, c->ebf32, c->stream);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(rho);
+  if (r) return r;
+  HIPCK(c, e);
+  return ALADYN_OK;
+}
+
 // ------------------------------------------- allocation-free grid charge
 // Device staging of aladyn_grid_charge_noalloc: one array per GPU, in the
 // code object's data (no allocation call ever makes it).  Its host-side name

@@ -43,6 +43,14 @@ void curr_collect_batch(const FieldRun<T>* runs, int nruns, int ndim, const Batc
 template <typename T>
 void advance_fields_batch(const FieldRun<T>* runs, int nruns, const BatchDims& d, hipStream_t s);
 
+// ----------------------------------------------------------------- beam.hip
+// The bunch fields of charge density rho (FP64, ncell; its interior becomes
+// the potential) added to ebf (init_beam_part_distrib.f90:538-602); waits
+// for the stream.
+template <typename R>
+hipError_t beam_fields(const DevGrid& g, double* rho, double gam2, double bet0, R* ebf,
+                       hipStream_t s);
+
 // ---------------------------------------------------------- reductions.hip
 template <typename T>
 void field_energy(const DevGrid& g, const T* ebf, double* out, hipStream_t s);

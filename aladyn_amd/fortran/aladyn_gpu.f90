@@ -270,6 +270,16 @@ module aladyn_gpu
       real(c_double), intent(inout) :: rho(*)
     end function
 
+    ! replaces the bunch-field part of init_beam_part_distrib
+    ! (set_grid_charge, fft_3d_psolv, initial_beam_fields, ebf += ebf_bunch)
+    integer(c_int) function aladyn_beam_fields(ctx, ic, gam2, bet0) &
+        bind(C, name="aladyn_beam_fields")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ic
+      real(c_double), value :: gam2, bet0
+    end function
+
     integer(c_int) function aladyn_continuity_residual(ctx, out) &
         bind(C, name="aladyn_continuity_residual")
       import :: c_int, c_ptr, c_double

@@ -198,6 +198,13 @@ class LPStep:
         self._ck(self.lib.aladyn_field_energy(self._h, _dptr(out)))
         return out[: self.cfg.nfield]
 
+    def beam_fields(self, ic: int, gam2: float, bet0: float):
+        """Adds the bunch fields of species ic to the fields
+        (init_beam_part_distrib.f90:538-602: set_grid_charge, the
+        sine-transform Poisson solve of fft_3d_psolv / fft_2d_psolv,
+        initial_beam_fields); gam2 = gamma^2, bet0 = beta of the bunch."""
+        self._ck(self.lib.aladyn_beam_fields(self._h, int(ic), float(gam2), float(bet0)))
+
     def grid_charge(self, ic: int = 0) -> np.ndarray:
         rho = np.zeros((self.cfg.n1p, self.cfg.n2p, self.cfg.n3p), dtype=np.float64, order="F")
         self._ck(self.lib.aladyn_grid_charge(self._h, ic, _dptr(rho)))

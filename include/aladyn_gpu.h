@@ -247,6 +247,15 @@ int aladyn_grid_charge(aladyn_ctx *ctx, int32_t ic, double *rho);
  * nodes (n1p*n2p*n3p) give ALADYN_EINVAL.  Calls that overlap on one GPU
  * share that array and race. */
 int aladyn_grid_charge_noalloc(aladyn_ctx *ctx, int32_t ic, double *rho);
+/* The bunch fields at the start of a run, added to the context's fields
+ * (init_beam_part_distrib.f90:538-602, uniform grid, one rank; replaces its
+ * set_grid_charge + fft_3d_psolv / fft_2d_psolv + initial_beam_fields +
+ * ebf += ebf_bunch): the charge density of species ic (the bunch), its
+ * potential by the sine-transform Poisson solve (psolve.f90:117-247, ft_mod
+ * = 2, sym = 1), then (Ex, Ey, Ez, By, Bz) of initial_beam_fields
+ * (init_grid_fields.f90:32-131), in FP64.  gam2 = gamma^2 and bet0 = beta of
+ * the bunch.  Allocates and frees its work arrays, waits for the stream. */
+int aladyn_beam_fields(aladyn_ctx *ctx, int32_t ic, double gam2, double bet0);
 /* Discrete continuity residual of the last particle_stage on the device in
  * FP64: r = (rho^{n+1} - rho^n) + (1/curr_ndim) sum_d D_d J_d(raw) on the
  * interior, with rho from the particle positions before and after the stage

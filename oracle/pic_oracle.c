@@ -2002,3 +2002,198 @@ int64_t orc_env_lpf2_evolve(const orc_grid *g, const orc_env *e, int64_t np, dou
   orc_advance_fields(g, ebf, jc);
   return np;
 }
+
+/* ---------------------------------------------------------------------------
+ * Bunch fields at the start of a run (init_beam_part_distrib.f90:538-602 on a
+ * uniform grid, one rank): the bunch charge density in jc(1) becomes its
+ * potential by fft_3d_psolv / fft_2d_psolv with sine transforms (ft_mod = 2,
+ * sym = 1, psolve.f90:117-247), jc(2) = bet0*jc(1), and initial_beam_fields
+ * (init_grid_fields.f90:32-131) makes the bunch fields, which are added to
+ * ebf with Bx = 0.
+ * ------------------------------------------------------------------------- */
+
+/* FFTW_RODFT00 of length n along a strided line (the r2r plans of
+ * modern_fft_lib.F90:196-224): y_k = 2 sum_j x_j sin(pi (j+1)(k+1)/(n+1)),
+ * times sc. */
+static void orc_dst1(double *v, int n, int64_t st, double sc, double *tmp) {
+  for (int k = 0; k < n; k++) {
+    double s = 0.0;
+    for (int j = 0; j < n; j++)
+      s += v[j * st] * sin(acos(-1.0) * (double)(j + 1) * (double)(k + 1) / (double)(n + 1));
+    tmp[k] = 2.0 * s;
+  }
+  for (int k = 0; k < n; k++) v[k * st] = sc * tmp[k];
+}
+
+/* ftw1d_sc along axis dir (1, 2, 3) of wb(n1, n2, n3); is < 0 scales by
+ * 1/logical_dimension: 2((n1+1)+1) along x (the x call passes n1+1,
+ * modern_fft_lib.F90:689), 2(n+1) along y and z (:732, :748). */
+static void orc_ftw1d_sc(double *wb, int n1, int n2, int n3, int is, int dir, double *tmp) {
+  const int n = dir == 1 ? n1 : (dir == 2 ? n2 : n3);
+  const int64_t st = dir == 1 ? 1 : (dir == 2 ? n1 : (int64_t)n1 * n2);
+  const double sc = is < 0 ? 1.0 / (double)(2 * ((dir == 1 ? n1 + 1 : n) + 1)) : 1.0;
+  for (int k = 0; k < (dir == 3 ? 1 : n3); k++)
+    for (int j = 0; j < (dir == 2 ? 1 : n2); j++)
+      for (int i = 0; i < (dir == 1 ? 1 : n1); i++)
+        orc_dst1(wb + i + (int64_t)n1 * (j + (int64_t)n2 * k), n, st, sc, tmp);
+}
+
+/* sine-transform wavenumbers sk(i, 2) = 2 sin(dh*ak/2)/dh, ak = i*pi/l_box
+ * (set_grid_param.f90:959-978), i = 1 .. n; l_box = x(nx) - x(1) = (nx-1)dx
+ * for an open x (set_grid_param.f90:104-108), ny*dy and nz*dz (:170, :223). */
+static double orc_sk(int i, double dh, double l_box) {
+  const double wk = acos(-1.0) / l_box;
+  const double ak = wk * (double)i;
+  return 2. * sin(0.5 * dh * ak) / dh;
+}
+
+void orc_sine_transform(double *v, int n) {
+  if (n <= 0) return;
+  double *tmp = (double *)malloc((size_t)n * sizeof(double));
+  orc_dst1(v, n, 1, 1.0, tmp);
+  free(tmp);
+}
+
+void orc_beam_potential(const orc_grid *g, double gam2, double *jc) {
+  const int n1 = g->nx, n2 = g->ny, n3 = g->ndim == 3 ? g->nz : 1;
+  const int64_t nw = (int64_t)n1 * n2 * n3;
+  double *wb = (double *)calloc((size_t)nw, sizeof(double));
+  double *tmp = (double *)malloc((size_t)imax(n1, imax(n2, n3)) * sizeof(double));
+  /* x(nx) - x(1), y(ny+1) - y(1), z(nz+1) - z(1) with x(i) = dx*(i-1),
+   * y(i) = dy*(i-1-ny/2), z(i) = dz*(i-1-nz/2) (set_grid_param.f90:76-78,
+   * 128-130, 187-189) */
+  const double lx = g->dx * (double)(g->nx - 1) - g->dx * 0.0;
+  const double ly = g->dy * (double)(g->ny - g->ny / 2) - g->dy * (double)(-(g->ny / 2));
+  const double lz = g->dz * (double)(g->nz - g->nz / 2) - g->dz * (double)(-(g->nz / 2));
+#define WB(i, j, k) wb[(i)-1 + (int64_t)n1 * ((j)-1 + (int64_t)n2 * ((k)-1))]
+  /* wb(1:n1, 1:n2, 1:n3) = rho(i1:i2, j1:j2, k1:k2, 1) */
+  for (int k = 1; k <= n3; k++)
+    for (int j = 1; j <= n2; j++)
+      for (int i = 1; i <= n1; i++)
+        WB(i, j, k) = F4(jc, g->ix1 + i - 1, g->jy1 + j - 1, g->kz1 + k - 1, 1);
+  if (g->ndim == 3) {
+    /* fft_3d_psolv: pftw3d_sc(-1) = x, then pftw2d_sc(-1) = y, z */
+    orc_ftw1d_sc(wb, n1, n2, n3, -1, 1, tmp);
+    if (n2 > 2) {
+      orc_ftw1d_sc(wb, n1, n2, n3, -1, 2, tmp);
+      if (n3 > 2) orc_ftw1d_sc(wb, n1, n2, n3, -1, 3, tmp);
+    }
+    for (int64_t q = 0; q < nw; q++) wb[q] = g->ompe * wb[q];
+    /* beam_potential (psolve.f90:66-115), n3_loc > 1 */
+    for (int iz = 1; iz <= n3; iz++)
+      for (int iy = 1; iy <= n2; iy++) {
+        const double sz = orc_sk(iz, g->dz, lz), sy = orc_sk(iy, g->dy, ly);
+        const double ak2p = sz * sz + sy * sy;
+        if (ak2p > 0.0) {
+          for (int ix = 1; ix <= n1; ix++) {
+            const double sx = orc_sk(ix, g->dx, lx);
+            const double ak2 = ak2p + sx * sx / gam2;
+            WB(ix, iy, iz) = WB(ix, iy, iz) / ak2;
+          }
+        } else {
+          for (int ix = 2; ix <= n1; ix++) {
+            const double sx = orc_sk(ix, g->dx, lx);
+            const double ak2 = sx * sx / gam2;
+            WB(ix, iy, iz) = WB(ix, iy, iz) / ak2;
+          }
+        }
+      }
+    /* pftw3d_sc(+1) = pftw2d_sc(+1) (z, then y), then x */
+    if (n3 > 1) orc_ftw1d_sc(wb, n1, n2, n3, 1, 3, tmp);
+    if (n2 > 1) orc_ftw1d_sc(wb, n1, n2, n3, 1, 2, tmp);
+    orc_ftw1d_sc(wb, n1, n2, n3, 1, 1, tmp);
+  } else {
+    /* fft_2d_psolv: pftw2d_sc(-1) = y (n3 = 1: no z) */
+    if (n2 > 2) orc_ftw1d_sc(wb, n1, n2, n3, -1, 2, tmp);
+    for (int64_t q = 0; q < nw; q++) wb[q] = g->ompe * wb[q];
+    /* beam_2d_potential (psolve.f90:38-64); skz = 0 for nz = 1 */
+    for (int iy = 1; iy <= n2; iy++) {
+      const double sy = orc_sk(iy, g->dy, ly);
+      const double ak2p = 0.0 * 0.0 + sy * sy;
+      for (int ix = 1; ix <= n1; ix++)
+        WB(ix, iy, 1) = ak2p > 0.0 ? WB(ix, iy, 1) / ak2p : 0.0;
+    }
+    if (n2 > 1) orc_ftw1d_sc(wb, n1, n2, n3, 1, 2, tmp);
+  }
+  for (int k = 1; k <= n3; k++)
+    for (int j = 1; j <= n2; j++)
+      for (int i = 1; i <= n1; i++)
+        F4(jc, g->ix1 + i - 1, g->jy1 + j - 1, g->kz1 + k - 1, 1) = WB(i, j, k);
+#undef WB
+  free(tmp);
+  free(wb);
+}
+
+void orc_initial_beam_fields(const orc_grid *g, double *jc, double *efb, double gam2,
+                             double bet0) {
+  const int64_t nc = (int64_t)g->n1p * g->n2p * g->n3p;
+  const int ix1 = g->ix1, ix2 = g->ix2, jy1 = g->jy1, jy2 = g->jy2, kz1 = g->kz1,
+            kz2 = g->kz2;
+#define EF(i, j, k, c) efb[((int64_t)(c)-1) * nc + ((i)-1) + (int64_t)g->n1p * (((j)-1) + (int64_t)g->n2p * ((k)-1))]
+  const double dx_inv = g->dx_inv, dy_inv = g->dy_inv, dz_inv = g->dz_inv;
+  for (int k = kz1; k <= kz2; k++) /* pe1y */
+    for (int i = ix1; i <= ix2; i++)
+      F4(jc, i, jy2 + 1, k, 1) = 3. * (F4(jc, i, jy2, k, 1) - F4(jc, i, jy2 - 1, k, 1)) +
+                                 F4(jc, i, jy2 - 2, k, 1);
+  for (int k = kz1; k <= kz2; k++) /* pe1x */
+    for (int j = jy1; j <= jy2; j++) {
+      F4(jc, ix2 + 1, j, k, 1) = F4(jc, ix2, j, k, 1);
+      F4(jc, ix2 + 1, j, k, 2) = 2. * F4(jc, ix2, j, k, 2) - F4(jc, ix2 - 1, j, k, 2);
+    }
+  for (int k = kz1; k <= kz2; k++) /* jx(i+1/2): component 4, zeroed by the caller */
+    for (int j = jy1; j <= jy2; j++)
+      for (int i = ix1; i <= ix2; i++)
+        EF(i, j, k, 4) = 0.5 * (F4(jc, i, j, k, 2) + F4(jc, i + 1, j, k, 2));
+  for (int k = kz1; k <= kz2; k++)
+    for (int j = jy1; j <= jy2; j++) {
+      const double sdhy = 1.0 * dy_inv; /* loc_yg(jj, 3, imody) = 1, unstretched */
+      for (int i = ix1; i <= ix2 + 1; i++)
+        EF(i, j, k, 2) = -sdhy * (F4(jc, i, j + 1, k, 1) - F4(jc, i, j, k, 1));
+      for (int i = ix1; i <= ix2; i++)
+        EF(i, j, k, 1) = -dx_inv * (F4(jc, i + 1, j, k, 1) - F4(jc, i, j, k, 1)) / gam2;
+    }
+  if (g->ndim == 2) {
+    for (int k = kz1; k <= kz2; k++)
+      for (int j = jy1; j <= jy2; j++)
+        for (int i = ix1; i <= ix2; i++)
+          EF(i, j, k, 3) = 0.5 * bet0 * (EF(i, j, k, 2) + EF(i + 1, j, k, 2));
+  } else {
+    for (int j = jy1; j <= jy2; j++) /* pe1z */
+      for (int i = ix1; i <= ix2; i++)
+        F4(jc, i, j, kz2 + 1, 1) = 2. * F4(jc, i, j, kz2, 1) - F4(jc, i, j, kz2 - 1, 1);
+    for (int k = kz1; k <= kz2; k++)
+      for (int j = jy1; j <= jy2; j++)
+        for (int i = ix1; i <= ix2; i++)
+          EF(i, j, k, 6) = 0.5 * bet0 * (EF(i, j, k, 2) + EF(i + 1, j, k, 2));
+    for (int k = kz1; k <= kz2; k++) {
+      const double sdhz = 1.0 * dz_inv;
+      for (int j = jy1; j <= jy2; j++)
+        for (int i = ix1; i <= ix2; i++)
+          EF(i, j, k, 3) = -sdhz * (F4(jc, i, j, k + 1, 1) - F4(jc, i, j, k, 1));
+    }
+    for (int k = kz1; k <= kz2; k++)
+      for (int j = jy1; j <= jy2; j++)
+        for (int i = ix1; i <= ix2; i++) {
+          F4(jc, i, j, k, 2) = 0.5 * (EF(i, j, k, 3) + EF(i + 1, j, k, 3));
+          EF(i, j, k, 5) = -bet0 * F4(jc, i, j, k, 2);
+        }
+  }
+#undef EF
+}
+
+void orc_beam_fields(const orc_grid *g, double gam2, double bet0, double *jc, double *ebf) {
+  orc_beam_potential(g, gam2, jc);
+  /* jc(:, :, :, 2) = bet0*jc(:, :, :, 1) */
+  const int64_t nc = (int64_t)g->n1p * g->n2p * g->n3p;
+  for (int64_t q = 0; q < nc; q++) jc[nc + q] = bet0 * jc[q];
+  /* initial_beam_fields(jc, efb, gam2, bet0) into efb = 0 (ebf_bunch is
+   * zeroed when allocated, init_beam_part_distrib.f90:488-491) */
+  if (nc <= 0) return;
+  double *efb = (double *)calloc((size_t)nc * 6, sizeof(double));
+  orc_initial_beam_fields(g, jc, efb, gam2, bet0);
+  /* ebf_bunch(:,:,:,4) = 0; ebf(1:nfield) += ebf_bunch(1:nfield) */
+  for (int64_t q = 0; q < nc; q++) efb[3 * nc + q] = 0.0;
+  for (int c = 0; c < g->nfield; c++)
+    for (int64_t q = 0; q < nc; q++) ebf[(int64_t)c * nc + q] += efb[(int64_t)c * nc + q];
+  free(efb);
+}

@@ -219,6 +219,23 @@ void orc_den_zyxbd(const orc_grid *g, double *rho, int ik);
 int64_t orc_env_lpf2_evolve(const orc_grid *g, const orc_env *e, int64_t np, double *part,
                             int64_t ld, double *ebfp, double *ebf, double *env, double *jc);
 
+/* The bunch fields at the start of a run (init_beam_part_distrib.f90:538-602,
+ * uniform grid, one rank): jc(n1p, n2p, n3p, 2) enters with the bunch charge
+ * density in component 1 (set_grid_charge); the potential of fft_3d_psolv /
+ * fft_2d_psolv (sine transforms, psolve.f90:117-247) replaces its interior,
+ * jc(2) = bet0*jc(1), and initial_beam_fields (init_grid_fields.f90:32-131)
+ * adds (Ex, Ey, [Ez], [0, By], Bz) to ebf.  gam2 = gamma^2 of the bunch. */
+void orc_beam_fields(const orc_grid *g, double gam2, double bet0, double *jc, double *ebf);
+/* Its parts: FFTW_RODFT00 of one line of n values (y_k = 2 sum_j x_j
+ * sin(pi (j+1)(k+1)/(n+1)), in place); the potential of jc(1) in place
+ * (fft_3d_psolv / fft_2d_psolv with ft_mod = 2, sym = 1, s_ind = 0); and
+ * initial_beam_fields from jc(1) = potential, jc(2) = bet0*potential into
+ * efb(n1p, n2p, n3p, max(nfield, 6)) (entries it does not set are left). */
+void orc_sine_transform(double *v, int n);
+void orc_beam_potential(const orc_grid *g, double gam2, double *jc);
+void orc_initial_beam_fields(const orc_grid *g, double *jc, double *efb, double gam2,
+                             double bet0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -101,6 +101,10 @@ class Oracle:
             "orc_window_xshift": [_G, C.POINTER(OrcWindow), C.c_int, C.c_int, C.c_double, _D,
                                   C.POINTER(C.c_int64), _D, _I64],
             "orc_fields_left_xshift": [_G, _D, C.c_int, C.c_int],
+            "orc_sine_transform": [_D, C.c_int],
+            "orc_beam_potential": [_G, C.c_double, _D],
+            "orc_initial_beam_fields": [_G, _D, _D, C.c_double, C.c_double],
+            "orc_beam_fields": [_G, C.c_double, C.c_double, _D, _D],
             "orc_inject_select": [_D, _I64, C.POINTER(C.c_int64), C.c_double, C.c_double,
                                   C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
             "orc_add_particles": [_G, _D, _I64, _I64, _D, C.c_int, _D, C.c_int, _D, _D, C.c_int,
@@ -257,6 +261,39 @@ class Oracle:
         nshx = self.L.orc_window_xshift(C.byref(self.g), C.byref(win), witr, init_iter, w_speed,
                                         _p(_f(ebf)), C.byref(n), _p(_f(part)), self._ld(part))
         return int(nshx), np.asfortranarray(part[:n.value])
+
+    def sine_transform(self, v):
+        """FFTW_RODFT00 of the 1-D array v (a new array)."""
+        out = np.array(v, dtype=np.float64)
+        self.L.orc_sine_transform(_p(out), out.size)
+        return out
+
+    def beam_potential(self, rho, gam2):
+        """fft_3d_psolv / fft_2d_psolv (sine transforms) of rho(n1p, n2p,
+        n3p): a copy with the potential in the interior, ghosts kept."""
+        jc = np.array(rho, dtype=np.float64, order="F")
+        self.L.orc_beam_potential(C.byref(self.g), gam2, _p(jc))
+        return jc
+
+    def initial_beam_fields(self, pot, gam2, bet0):
+        """initial_beam_fields from jc(1) = pot, jc(2) = bet0*pot on every
+        node: returns efb(n1p, n2p, n3p, 6), zero where it sets nothing."""
+        cfg = self.cfg
+        jc = np.zeros((cfg.n1p, cfg.n2p, cfg.n3p, 2), order="F")
+        jc[..., 0] = pot
+        jc[..., 1] = bet0 * jc[..., 0]
+        efb = np.zeros((cfg.n1p, cfg.n2p, cfg.n3p, 6), order="F")
+        self.L.orc_initial_beam_fields(C.byref(self.g), _p(jc), _p(efb), gam2, bet0)
+        return efb
+
+    def beam_fields(self, rho, ebf, gam2, bet0):
+        """The bunch fields of charge density rho added to ebf in place
+        (init_beam_part_distrib.f90:538-602)."""
+        cfg = self.cfg
+        jc = np.zeros((cfg.n1p, cfg.n2p, cfg.n3p, 2), order="F")
+        jc[..., 0] = rho
+        assert ebf.flags.f_contiguous and ebf.dtype == np.float64
+        self.L.orc_beam_fields(C.byref(self.g), gam2, bet0, _p(jc), _p(ebf))
 
     def fields_left_xshift(self, fld, nshx):
         """fields_left_xshift of every component of fld (n1p, n2p, n3p, nc),
