@@ -17,8 +17,8 @@ the envelope/fluid model), so parity is the LP step's, species by species:
   bunch momentum, field energy) agree with FP64 within 1e-3.
 
 Round-off bars of the full-size run: FP64 continuity <= 1e-12 of max|drho|
-and Gauss-law change <= 1e-9 of max ompe*rho; FP32 continuity <= 1e-5 (the
-north-star bar) and Gauss-law change <= 1e-3 after 400 steps.  The stage's
+and Gauss-law change <= 1e-9 of max ompe*rho; FP32 continuity <= 1e-5 and
+Gauss-law change <= 1e-3 after 400 steps.  The stage's
 raw current is summed in FP64 (register and LDS sums per tile, FP64 tile
 patches, the overflowed particles' current added to those patches with FP64
 atomics) and rounded to FP32 once per node; the diagnostic takes it before
