@@ -220,8 +220,8 @@ def test_scan_fp32_against_fp64_batched():
     cases = [scan_2d_case(r) for r in runs]
     npart = sum(c[1].shape[0] for c in cases)
     m, wall = {}, {}
-    # FP64 as one batch, FP32 as four batches of 16 stepped in turn
-    for name, prec, groups in (("f64", PREC_F64, 1), ("f32", PREC_F32, 4)):
+    # FP64 as one batch, FP32 as two batches of 32 stepped in turn
+    for name, prec, groups in (("f64", PREC_F64, 1), ("f32", PREC_F32, 2)):
         print(f"scan batch {name}: {len(cases)} runs, {npart} particles, {groups} batches",
               flush=True)
         ctx = _make(cases, prec)
