@@ -487,17 +487,25 @@ extern "C" int aladyn_lp_window_xshift(aladyn_ctx* c, int32_t witr, int32_t init
         fields_left_xshift<float>(g, c->env32, (int)nshx, c->stream, 4);
       HIPCK(c, hipGetLastError());
     }
-    // particles behind the new left face leave (cell_part_dist(.true.))
+    // particles behind the new left face leave (cell_part_dist(.true.)); the
+    // shift waits for the re-layout of the injection below, so that one pass
+    // over the store does both
     for (int ic = 0; ic < c->cfg.nsp; ic++) {
       std::string msg;
-      if (int r = tile_store_window(&c->ts[ic], c->tg, g, (int)nshx, c->stream, &msg))
+      if (int r = tile_store_window(&c->ts[ic], c->tg, g, (int)nshx, c->stream, &msg, true))
         return fail(c, r, "%s", msg.c_str());
     }
   }
   // plasma entering on the right from the registered target tables
-  for (int ic = 0; ic < c->cfg.nsp; ic++)
-    if (int r = inject_from_table(c, ic)) return r;
-  return ALADYN_OK;
+  int rc = ALADYN_OK;
+  for (int ic = 0; ic < c->cfg.nsp && rc == ALADYN_OK; ic++) rc = inject_from_table(c, ic);
+  // species with nothing injected take the shift on their own
+  for (int ic = 0; ic < c->cfg.nsp; ic++) {
+    std::string msg;
+    if (int r = tile_store_window_apply(&c->ts[ic], c->tg, g, c->stream, &msg))
+      return rc != ALADYN_OK ? rc : fail(c, r, "%s", msg.c_str());
+  }
+  return rc;
 }
 
 extern "C" int aladyn_inject_species(aladyn_ctx* c, int32_t ic, const double* part, int64_t ld,

@@ -352,6 +352,13 @@ __device__ inline int wave_max(int v) {
   return v;
 }
 
+// Sum of v over the wavefront (every lane active).
+__device__ inline int wave_sum(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 // qqh_*_spline for one axis (grid_part_lib.f90:145-210, 261-330) from the
 // nearest node N and offset s (xx = N + s, s in [-1/2, 1/2)): quadratic
 // weights at integer nodes from node N-1, and at half-integer nodes from

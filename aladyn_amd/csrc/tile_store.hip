@@ -3,6 +3,7 @@
 // patches, the global-memory kernel for overflowed particles, the re-layout
 // (counting sort with fresh capacities), transfers and FP64 diagnostics.
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <numeric>
 #include <type_traits>
@@ -346,54 +347,91 @@ __global__ __launch_bounds__(kB) void k_nbmax(TileGeom tg, const int* __restrict
 // flag is consumed).  With nb (k_nbmax of v), the base capacity is that of
 // the neighbourhood's largest count when all of those fit the budget, else
 // the tile's own.
-__global__ __launch_bounds__(kB) void k_scan_tiles(const int* v_in, int n, int* out, int* cap_out,
-                                                   int* grow = nullptr, int64_t budget = 0,
-                                                   const int* nb = nullptr) {
-  __shared__ int sh[kB];
-  __shared__ long long red[2][kB];
+// One workgroup of kScanB threads: the sums are wave shuffles with one
+// barrier per pass of kScanB tiles, and the next pass's loads are issued
+// before that barrier, so the pass time is not the HBM latency.
+static constexpr int kScanB = 1024, kScanW = kScanB / 64;
+__device__ inline long long wave_sum_ll(long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__global__ __launch_bounds__(kScanB) void k_scan_tiles(const int* v_in, int n, int* out,
+                                                       int* cap_out, int* grow = nullptr,
+                                                       int64_t budget = 0,
+                                                       const int* nb = nullptr) {
+  __shared__ long long red[2][kScanW];
+  __shared__ int wtot[2][kScanW];
+  const int me = threadIdx.x, lane = me & 63, wv = me >> 6;
   bool use_nb = false;
   long long base_sum = 0;
   if (cap_out) {
     long long own = 0, wide = 0;
-    for (int t = threadIdx.x; t < n; t += kB) {
+#pragma unroll 4
+    for (int t = me; t < n; t += kScanB) {
       const bool gr = grow && grow[t];
       own += tile_base_capacity(v_in[t], gr);
       if (nb) wide += tile_base_capacity(nb[t], gr);
     }
-    red[0][threadIdx.x] = own;  // block sums of the base capacities
-    red[1][threadIdx.x] = wide;
-    __syncthreads();
-    for (int o = kB / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) {
-        red[0][threadIdx.x] += red[0][threadIdx.x + o];
-        red[1][threadIdx.x] += red[1][threadIdx.x + o];
-      }
-      __syncthreads();
+    own = wave_sum_ll(own);  // block sums of the base capacities
+    wide = wave_sum_ll(wide);
+    if (lane == 0) {
+      red[0][wv] = own;
+      red[1][wv] = wide;
     }
-    use_nb = nb && red[1][0] <= budget;
-    base_sum = use_nb ? red[1][0] : red[0][0];
+    __syncthreads();
+    long long so = 0, sw = 0;
+    for (int w = 0; w < kScanW; w++) {
+      so += red[0][w];
+      sw += red[1][w];
+    }
+    use_nb = nb && sw <= budget;
+    base_sum = use_nb ? sw : so;
   }
   const long long left = cap_out ? budget - base_sum : 0;
   const long long extra = left > 0 ? left / n : 0;
-  __syncthreads();
-  long long carry = 0;
-  for (int b0 = 0; b0 < n; b0 += kB) {
-    const int t = b0 + threadIdx.x;
-    int v = 0;
+  // the tile's value: its count, or with cap_out its segment capacity
+  const auto value = [&](int t, int raw, int nbv, bool gr) -> int {
+    if (t >= n) return 0;
+    return cap_out ? (int)(tile_base_capacity(use_nb ? nbv : raw, gr) + extra) : raw;
+  };
+  int raw = 0, nbv = 0;
+  bool gr = false;
+  const auto fetch = [&](int t) {
     if (t < n) {
-      v = v_in[t];
-      if (cap_out) {
-        v = (int)(tile_base_capacity(use_nb ? nb[t] : v, grow && grow[t]) + extra);
-        if (grow) grow[t] = 0;
-        cap_out[t] = v;
-      }
+      raw = v_in[t];
+      if (cap_out && use_nb) nbv = nb[t];
+      gr = cap_out && grow && grow[t];
     }
-    int total;
-    const int ex = block_scan_excl(v, sh, total);
-    if (t < n) out[t] = (int)(carry + ex);
-    carry += total;
+  };
+  fetch(me);
+  long long carry = 0;
+  for (int b0 = 0, p = 0; b0 < n; b0 += kScanB, p ^= 1) {
+    const int t = b0 + me;
+    const int v = value(t, raw, nbv, gr);
+    if (t < n && cap_out) {
+      if (grow) grow[t] = 0;
+      cap_out[t] = v;
+    }
+    fetch(t + kScanB);  // the next pass's loads, in flight across the barrier
+    int x = v;  // inclusive scan within the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wtot[p][wv] = x;
+    __syncthreads();  // wtot[p] complete; wtot[p^1] is free for the next pass
+    int pre = 0, tot = 0;
+    for (int w = 0; w < kScanW; w++) {
+      const int s = wtot[p][w];
+      pre += w < wv ? s : 0;
+      tot += s;
+    }
+    if (t < n) out[t] = (int)(carry + pre + x - v);
+    carry += tot;
   }
-  if (threadIdx.x == 0) out[n] = (int)carry;
+  if (me == 0) out[n] = (int)carry;
 }
 
 // live (non-hole) records per tile: live[t], and the overflow count at [ntiles]
@@ -408,11 +446,18 @@ __global__ __launch_bounds__(kB) void k_count_live(RecPtr A, const int* seg_star
     return;
   }
   const int n = min(cnt[t], seg_cap[t]);
+  const int64_t s0 = seg_start[t];
   int c = 0;
-  for (int q = threadIdx.x; q < n; q += kB) c += rec_cell<R>(A, (int64_t)seg_start[t] + q) != kHole;
-  int total;
-  block_scan_excl(c, sh, total);
-  if (threadIdx.x == 0) live[t] = total;
+#pragma unroll 4
+  for (int q = threadIdx.x; q < n; q += kB) c += rec_cell<R>(A, s0 + q) != kHole;
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int total = 0;
+    for (int w = 0; w < kB / 64; w++) total += sh[w];
+    live[t] = total;
+  }
 }
 
 // live records of A (tile segments in order, then the overflow list) -> U
@@ -466,6 +511,16 @@ __global__ void k_scatter(TileGeom tg, RecPtr U, int64_t n, RecPtr A, const int*
 // (an exact change of the packed node index) and drops the records now at or
 // behind the box's left face, x <= xmin: node < 3, or node 3 with s <= 0
 // (the test of reset_all_part_dist, mpi_part_interface.f90:494-498).
+// The overflow list's records from index ovf_keep on were appended after the
+// shift was decided (the plasma injected at the same move) and are already in
+// the new frame: they are not shifted.
+template <typename R>
+__device__ inline int shift_cell(const TileGeom& tg, int cell, R s0, int shx) {
+  if (shx == 0 || cell == kHole) return cell;
+  const int i = (cell & ((1 << tg.bx) - 1)) - shx;
+  if (i < 3 || (i == 3 && !(s0 > R(0)))) return kHole;
+  return cell - shx;
+}
 template <typename R>
 __device__ inline int shifted_cell(const TileGeom& tg, const RecPtr& A, int64_t src, int cell,
                                    int shx) {
@@ -475,25 +530,73 @@ __device__ inline int shifted_cell(const TileGeom& tg, const RecPtr& A, int64_t 
   return cell - shx;
 }
 
+// Records per thread and pass of the re-layout kernels: their loads are
+// issued together, so a pass pays the HBM latency once.
+static constexpr int kRU = 4;
+
 template <int D, typename R>
 __global__ __launch_bounds__(kB) void k_hist_seg(TileGeom tg, RecPtr A, const int* seg_start,
                                                  const int* seg_cap, const int* cnt,
                                                  int64_t ovf_base, const int* ovf_cnt,
-                                                 int* hcount, int shx) {
+                                                 int* hcount, int shx, int ovf_keep) {
   const int t = blockIdx.x;
   if (t < tg.ntiles) {
+    // records that stay in tile t are counted per thread and added with one
+    // atomic per wave; the others go through the wave-aggregated counters
     const int n = min(cnt[t], seg_cap[t]);
-    for (int q = threadIdx.x; q < n; q += kB) {
-      const int64_t src = (int64_t)seg_start[t] + q;
-      const int cell = shifted_cell<R>(tg, A, src, rec_cell<R>(A, src), shx);
-      if (cell != kHole) (void)reserve_slot(tile_of(tg, cell, D), hcount);
+    const int64_t s0 = seg_start[t];
+    int own = 0;
+    for (int q0 = threadIdx.x; q0 < n; q0 += kRU * kB) {
+      int cell[kRU];
+#pragma unroll
+      for (int u = 0; u < kRU; u++) {
+        const int q = q0 + u * kB;
+        cell[u] = q < n ? rec_cell<R>(A, s0 + q) : kHole;
+      }
+#pragma unroll
+      for (int u = 0; u < kRU; u++) {
+        const int c = shifted_cell<R>(tg, A, s0 + q0 + u * kB, cell[u], shx);
+        if (c == kHole) continue;
+        const int d = tile_of(tg, c, D);
+        if (d == t) own++;
+        else (void)reserve_slot(d, hcount);
+      }
     }
+    own = wave_sum(own);
+    if ((threadIdx.x & 63) == 0 && own > 0) atomicAdd(&hcount[t], own);
   } else {
     const int n = *ovf_cnt;
-    for (int q = (t - tg.ntiles) * kB + threadIdx.x; q < n; q += (gridDim.x - tg.ntiles) * kB) {
-      const int cell = shifted_cell<R>(tg, A, ovf_base + q, rec_cell<R>(A, ovf_base + q), shx);
-      if (cell != kHole) (void)reserve_slot(tile_of(tg, cell, D), hcount);
+    const int stride = (gridDim.x - tg.ntiles) * kB;
+    for (int q0 = (t - tg.ntiles) * kB + threadIdx.x; q0 < n; q0 += kRU * stride) {
+      int cell[kRU];
+#pragma unroll
+      for (int u = 0; u < kRU; u++) {
+        const int q = q0 + u * stride;
+        cell[u] = q < n ? rec_cell<R>(A, ovf_base + q) : kHole;
+      }
+#pragma unroll
+      for (int u = 0; u < kRU; u++) {
+        const int q = q0 + u * stride;
+        const int c = shifted_cell<R>(tg, A, ovf_base + q, cell[u], q < ovf_keep ? shx : 0);
+        if (c != kHole) (void)reserve_slot(tile_of(tg, c, D), hcount);
+      }
     }
+  }
+}
+
+// the packed cell and the x offset of a record's first word
+template <typename R>
+__device__ inline int word_cell(const typename RecWord<R>::V& w) {
+  if constexpr (sizeof(R) == 4) return __float_as_int(w.x);
+  else return (int)(__double_as_longlong(w.x) & 0xffffffffll);
+}
+template <typename R>
+__device__ inline void set_word_cell(typename RecWord<R>::V& w, int cell) {
+  if constexpr (sizeof(R) == 4) {
+    w.x = __int_as_float(cell);
+  } else {
+    const long long cw = __double_as_longlong(w.x);
+    w.x = __longlong_as_double((cw & ~0xffffffffll) | (long long)(unsigned)cell);
   }
 }
 
@@ -502,29 +605,71 @@ template <int D, typename R>
 __global__ __launch_bounds__(kB) void k_scatter_seg(TileGeom tg, RecPtr A, const int* seg_start,
                                                     const int* seg_cap, const int* cnt,
                                                     int64_t ovf_base, const int* ovf_cnt, RecPtr U,
-                                                    const int* new_start, int* fillc, int shx) {
+                                                    const int* new_start, int* fillc, int shx,
+                                                    int ovf_keep) {
+  using V = typename RecWord<R>::V;
+  constexpr int NW = RecWord<R>::N;
   const int t = blockIdx.x;
-  if (t < tg.ntiles) {
-    const int n = min(cnt[t], seg_cap[t]);
-    for (int q = threadIdx.x; q < n; q += kB) {
-      const int64_t src = (int64_t)seg_start[t] + q;
-      const int cell = shifted_cell<R>(tg, A, src, rec_cell<R>(A, src), shx);
-      if (cell == kHole) continue;
-      const int d = tile_of(tg, cell, D);
-      const int64_t dst = (int64_t)new_start[d] + reserve_slot(d, fillc);
-      copy_rec<R>(A, src, U, dst);
-      set_rec_cell<R>(U, dst, cell);
+  const int lane = (int)__lane_id();
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  const bool seg = t < tg.ntiles;
+  const int n = seg ? min(cnt[t], seg_cap[t]) : *ovf_cnt;
+  const int64_t s0 = seg ? (int64_t)seg_start[t] : ovf_base;
+  const int first = seg ? threadIdx.x : (t - tg.ntiles) * kB + threadIdx.x;
+  const int stride = seg ? kB : (gridDim.x - tg.ntiles) * kB;
+  for (int q0 = first; q0 < n; q0 += kRU * stride) {
+    V w[kRU][NW];
+    int id[kRU], d[kRU];
+#pragma unroll
+    for (int u = 0; u < kRU; u++) {
+      const int q = q0 + u * stride;
+      d[u] = -1;
+      id[u] = 0;
+      if (q < n) {
+        const V* src = rec_at<R>(A, s0 + q);
+#pragma unroll
+        for (int k = 0; k < NW; k++) w[u][k] = src[k];
+        if (A.id && U.id) id[u] = A.id[s0 + q];
+      }
     }
-  } else {
-    const int n = *ovf_cnt;
-    for (int q = (t - tg.ntiles) * kB + threadIdx.x; q < n; q += (gridDim.x - tg.ntiles) * kB) {
-      const int64_t src = ovf_base + q;
-      const int cell = shifted_cell<R>(tg, A, src, rec_cell<R>(A, src), shx);
-      if (cell == kHole) continue;
-      const int d = tile_of(tg, cell, D);
-      const int64_t dst = (int64_t)new_start[d] + reserve_slot(d, fillc);
-      copy_rec<R>(A, src, U, dst);
-      set_rec_cell<R>(U, dst, cell);
+#pragma unroll
+    for (int u = 0; u < kRU; u++) {
+      const int q = q0 + u * stride;
+      if (q >= n) continue;
+      const int c = shift_cell<R>(tg, word_cell<R>(w[u][0]), (R)w[u][0].y,
+                                  seg || q < ovf_keep ? shx : 0);
+      if (c == kHole) continue;
+      set_word_cell<R>(w[u][0], c);
+      d[u] = tile_of(tg, c, D);
+    }
+    // records that stay in tile t: one atomic per wave for all of them
+    int base = 0, nown = 0;
+    unsigned long long ownm[kRU];
+#pragma unroll
+    for (int u = 0; u < kRU; u++) {
+      ownm[u] = __ballot(seg && d[u] == t);
+      nown += __popcll(ownm[u]);
+    }
+    if (nown > 0) {
+      const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+      if (lane == leader) base = atomicAdd(&fillc[t], nown);
+      base = __shfl(base, leader);
+    }
+#pragma unroll
+    for (int u = 0; u < kRU; u++) {
+      int64_t dst = -1;
+      if (seg && d[u] == t) {
+        dst = (int64_t)new_start[t] + base + __popcll(ownm[u] & lt_mask);
+      } else if (d[u] >= 0) {
+        dst = (int64_t)new_start[d[u]] + reserve_slot(d[u], fillc);
+      }
+      base += __popcll(ownm[u]);
+      if (dst >= 0) {
+        V* o = rec_at<R>(U, dst);
+#pragma unroll
+        for (int k = 0; k < NW; k++) o[k] = w[u][k];
+        if (A.id && U.id) U.id[dst] = id[u];
+      }
     }
   }
 }
@@ -643,7 +788,7 @@ static int sort_into_cur(TileStore* ts, const TileGeom& tg, int64_t n, hipStream
   fill<int>(h, tg.ntiles, 0, s);
   if (n > 0) k_hist<D, R><<<grid_for(n), kB, 0, s>>>(tg, ts->buf[oth], n, h);
   k_nbmax<<<(tg.ntiles + kB - 1) / kB, kB, 0, s>>>(tg, h, ts->scratch);
-  k_scan_tiles<<<1, kB, 0, s>>>(h, tg.ntiles, ts->seg_start, ts->seg_cap, ts->grow,
+  k_scan_tiles<<<1, kScanB, 0, s>>>(h, tg.ntiles, ts->seg_start, ts->seg_cap, ts->grow,
                                 seg_budget(ts->ovf_cap, tg.ntiles), ts->scratch);
   int total = 0;
   TS_CK(hipMemcpyAsync(&total, ts->seg_start + tg.ntiles, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -658,7 +803,7 @@ static int sort_into_cur(TileStore* ts, const TileGeom& tg, int64_t n, hipStream
                                             ts->scratch);
   fill<int>(ts->ovf_count, 2, 0, s);
   TS_CK(hipGetLastError());
-  ts->np = n;
+  ts->live_bound = n;
   ts->ovf_valid[0] = ts->ovf_valid[1] = false;
   ts->steps_since_sort = 0;
   return 0;
@@ -673,6 +818,7 @@ int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const do
                    int64_t ld, int64_t np, hipStream_t s, bool track_ids, std::string* msg) {
   const int D = g.ndim;
   ts->layout_version++;
+  ts->pending_shx = 0;  // the particles given are in the current box
   // the record keeps the FP32 weight; charge and part_ind are per species
   int charge = 0, pind = 0;
   for (int64_t n = 0; n < np; n++) {
@@ -764,7 +910,7 @@ static int gather_live(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t
   k_count_live<R><<<tg.ntiles + 1, kB, 0, s>>>(ts->buf[cur], ts->seg_start, ts->seg_cap,
                                                ts->count[cur], tg.ntiles, ts->ovf_count + cur,
                                                live);
-  k_scan_tiles<<<1, kB, 0, s>>>(live, tg.ntiles + 1, ts->scratch, nullptr);
+  k_scan_tiles<<<1, kScanB, 0, s>>>(live, tg.ntiles + 1, ts->scratch, nullptr);
   k_gather_live<D, R><<<tg.ntiles + 64, kB, 0, s>>>(ts->buf[cur], ts->buf[oth], ts->seg_start,
                                                     ts->seg_cap, ts->count[cur], ts->scratch,
                                                     tg.ntiles, ts->ovf_base, ts->ovf_count + cur);
@@ -782,21 +928,21 @@ static int gather_live(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t
 // (live particles never exceed the count given to put).
 template <int D, typename R>
 static int relayout_seg(TileStore* ts, const TileGeom& tg, hipStream_t s, std::string* msg,
-                        int shx = 0) {
+                        int shx = 0, int ovf_keep = INT_MAX) {
   const int cur = ts->cur, oth = 1 - cur;
   int* h = ts->count[oth];  // per-tile counts of the new layout
   fill<int>(h, tg.ntiles, 0, s);
   k_hist_seg<D, R><<<tg.ntiles + 64, kB, 0, s>>>(tg, ts->buf[cur], ts->seg_start, ts->seg_cap,
                                                  ts->count[cur], ts->ovf_base, ts->ovf_count + cur,
-                                                 h, shx);
+                                                 h, shx, ovf_keep);
   k_nbmax<<<(tg.ntiles + kB - 1) / kB, kB, 0, s>>>(tg, h, ts->scratch);
-  k_scan_tiles<<<1, kB, 0, s>>>(h, tg.ntiles, ts->seg_start2, ts->seg_cap2, ts->grow,
+  k_scan_tiles<<<1, kScanB, 0, s>>>(h, tg.ntiles, ts->seg_start2, ts->seg_cap2, ts->grow,
                                 seg_budget(ts->ovf_cap, tg.ntiles), ts->scratch);
   fill<int>(ts->scratch, tg.ntiles, 0, s);
   k_scatter_seg<D, R><<<tg.ntiles + 64, kB, 0, s>>>(tg, ts->buf[cur], ts->seg_start, ts->seg_cap,
                                                     ts->count[cur], ts->ovf_base,
                                                     ts->ovf_count + cur, ts->buf[oth],
-                                                    ts->seg_start2, ts->scratch, shx);
+                                                    ts->seg_start2, ts->scratch, shx, ovf_keep);
   fill<int>(ts->ovf_count, 2, 0, s);
   TS_CK(hipGetLastError());
   std::swap(ts->seg_start, ts->seg_start2);
@@ -818,13 +964,23 @@ int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hip
 
 // Moves the box window_cells cells to the right (window.f90:276-347): every
 // record's node index drops by window_cells, records left behind the new
-// left face are dropped, and the store is re-laid out in the same pass.
+// left face are dropped, and the store is re-laid out in the same pass (or,
+// deferred, in the pass of the injection that follows).
 int tile_store_window(TileStore* ts, const TileGeom& tg, const DevGrid& g, int window_cells,
-                      hipStream_t s, std::string* msg) {
+                      hipStream_t s, std::string* msg, bool defer) {
   if (!ts->loaded || window_cells == 0) return 0;
   ts->layout_version++;
+  ts->pending_shx += window_cells;
+  return defer ? 0 : tile_store_window_apply(ts, tg, g, s, msg);
+}
+
+int tile_store_window_apply(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
+                            std::string* msg) {
+  const int shx = ts->pending_shx;
+  ts->pending_shx = 0;
+  if (!ts->loaded || shx == 0) return 0;
   return with_dr(tg, g.ndim, [&](auto dc, auto rv) {
-    return relayout_seg<decltype(dc)::value, decltype(rv)>(ts, tg, s, msg, window_cells);
+    return relayout_seg<decltype(dc)::value, decltype(rv)>(ts, tg, s, msg, shx);
   });
 }
 
@@ -876,10 +1032,14 @@ static int grow_store(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t 
 template <int D, typename R, typename Fill>
 static int append_impl(TileStore* ts, const TileGeom& tg, int64_t n, hipStream_t s,
                        std::string* msg, Fill&& fill) {
-  int64_t live = 0;
-  if (int r = tile_store_count(ts, tg, s, &live, msg)) return r;
-  if (live + n > ts->ovf_cap)
-    if (int r = grow_store<D, R>(ts, tg, s, std::max(2 * ts->ovf_cap, live + n), msg)) return r;
+  // the exact count (a pass over the store) only when the bound does not fit
+  int64_t live = ts->live_bound;
+  if (live + n > ts->ovf_cap) {
+    if (int r = tile_store_count(ts, tg, s, &live, msg)) return r;
+    ts->live_bound = live;
+    if (live + n > ts->ovf_cap)
+      if (int r = grow_store<D, R>(ts, tg, s, std::max(2 * ts->ovf_cap, live + n), msg)) return r;
+  }
   const int cur = ts->cur;
   int ovf = 0;
   TS_CK(hipMemcpyAsync(&ovf, ts->ovf_count + cur, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -892,10 +1052,15 @@ static int append_impl(TileStore* ts, const TileGeom& tg, int64_t n, hipStream_t
   fill(dst, ts->next_id);
   TS_CK(hipGetLastError());
   ts->next_id += n;
+  ts->live_bound += n;
   const int total = ovf + (int)n;
   TS_CK(hipMemcpyAsync(ts->ovf_count + cur, &total, sizeof(int), hipMemcpyHostToDevice, s));
   TS_CK(hipStreamSynchronize(s));
-  return relayout_seg<D, R>(ts, tg, s, msg);
+  // a deferred window shift moves every record but the ones just written
+  // (they were placed in the shifted box)
+  const int shx = ts->pending_shx;
+  ts->pending_shx = 0;
+  return relayout_seg<D, R>(ts, tg, s, msg, shx, ovf);
 }
 
 // Appends n particles (reference layout, inside the box) to the species
@@ -1005,7 +1170,7 @@ int tile_store_count(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t* 
     k_count_live<float><<<tg.ntiles + 1, kB, 0, s>>>(ts->buf[cur], ts->seg_start, ts->seg_cap,
                                                      ts->count[cur], tg.ntiles,
                                                      ts->ovf_count + cur, live);
-  k_scan_tiles<<<1, kB, 0, s>>>(live, tg.ntiles + 1, ts->scratch, nullptr);
+  k_scan_tiles<<<1, kScanB, 0, s>>>(live, tg.ntiles + 1, ts->scratch, nullptr);
   TS_CK(hipGetLastError());
   int tot = 0;
   TS_CK(hipMemcpyAsync(&tot, ts->scratch + tg.ntiles + 1, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -69,9 +69,15 @@ struct TileStore {
   int charge = 0, part_ind = 0;
   bool track_ids = false;
   bool loaded = false;
-  int64_t np = 0;       // particles at the last put / re-layout
+  // bound on the live records: exact when the store is binned from a dense
+  // list (put, growth), plus every append since (between them particles only
+  // leave)
+  int64_t live_bound = 0;
   int64_t next_id = 0;  // id of the next injected particle (ids tracked)
   int steps_since_sort = 0;
+  // a window shift (cells) deferred to the re-layout of the injection that
+  // follows it at the same move, so that one pass does both
+  int pending_shx = 0;
   // bumped by every change of the layout or of the buffers (put, injection,
   // window, re-layout): work captured into a graph is valid for one version
   int64_t layout_version = 0;
@@ -149,8 +155,12 @@ int tile_store_put(TileStore* ts, const TileGeom& tg, const DevGrid& g, const do
                    int64_t ld, int64_t np, hipStream_t s, bool track_ids, std::string* msg);
 int tile_store_count(TileStore* ts, const TileGeom& tg, hipStream_t s, int64_t* np,
                      std::string* msg);
+// With defer, the shift is only recorded: the next append (injection) or
+// tile_store_window_apply re-lays the store out with it.
 int tile_store_window(TileStore* ts, const TileGeom& tg, const DevGrid& g, int window_cells,
-                      hipStream_t s, std::string* msg);
+                      hipStream_t s, std::string* msg, bool defer = false);
+int tile_store_window_apply(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
+                            std::string* msg);
 int tile_store_inject(TileStore* ts, const TileGeom& tg, const DevGrid& g, const double* part,
                       int64_t ld, int64_t n, hipStream_t s, bool track_ids, std::string* msg);
 // Device copy of a target table for injection at window moves (xpt,
