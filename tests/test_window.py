@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from aladyn_amd import PREC_F32, PREC_F64
-from aladyn_amd.inputs import plasma_table, uniform_plasma, unpack_weight
+from aladyn_amd.inputs import pack_weight, plasma_table, uniform_plasma, unpack_weight
 from cases import small_2d, small_3d
 from conftest import rel_maxerr
 from oracle.pyoracle import Oracle, OrcWindow
@@ -202,6 +202,92 @@ def test_window_table_injection_matches_oracle(dim, precision, tol):
     D = cfg.curr_ndim
     for d in range(D):
         span = (cfg.xmx, cfg.ymx, cfg.zmx)[d] - (cfg.xmn, cfg.ymn, cfg.zmn)[d]
+        assert np.max(np.abs(pg[:, d] - p[:, d])) / span <= tol
+    assert rel_maxerr(pg[:, D:2 * D], p[:, D:2 * D]) <= tol
+    assert np.array_equal(pg[:, 2 * D], p[:, 2 * D])
+    assert rel_maxerr(eg, e) <= tol
+    assert rel_maxerr(jg, jc) <= tol
+
+
+def _sheet_case():
+    """The table plasma of the right half of small_2d's box plus a dense sheet
+    of fast electrons (24000, p_x = 20) just left of the first tile boundary
+    in x (16 cells).  In the first step the sheet crosses into the next tile
+    (0.64 cells a step), whose segment was sized at the put for its own few
+    particles: most of the sheet goes to the overflow list."""
+    cfg, _, ebf = small_2d()
+    table = plasma_table(cfg, (2, 2), x_start_cell=cfg.nx // 2, x_end_cell=cfg.nx + 40)
+    table["charge"] = -1
+    orc = Oracle(cfg)
+    nin = int(np.sum(table["xpt"] <= cfg.xmx))
+    plasma = orc.add_particles(np.zeros((0, cfg.ncol), order="F"), table, 0, nin - 1)
+    rng = np.random.default_rng(11)
+    ns = 24000
+    sheet = np.zeros((ns, cfg.ncol))
+    sheet[:, 0] = cfg.xmn + 15.2 * cfg.dx
+    sheet[:, 1] = cfg.ymn + rng.uniform(0.5, cfg.ny - 0.5, ns) * cfg.dy
+    sheet[:, 2] = 20.0
+    sheet[:, 4] = pack_weight(-1, 0, np.full(ns, table["wghpt"][0]))
+    part = np.asfortranarray(np.concatenate([sheet, plasma]))
+    return cfg, orc, table, nin, part, ebf
+
+
+@pytest.mark.gpu
+def test_sheet_case_overflows_its_tile():
+    """The case below does fill the overflow list: stored without ids, the
+    species comes back as the tile segments in order and then the overflow
+    list, so after the first step its last records are sheet electrons (near
+    the first tile boundary), not the plasma of the last tile."""
+    from aladyn_amd import LPStep
+
+    cfg, _, _, _, part, ebf = _sheet_case()
+    with LPStep(cfg, PREC_F64) as gpu:
+        gpu.put_species(0, part)
+        gpu.put_fields(ebf)
+        gpu.lpf2_evolve(0, initial_time=True)
+        pg = gpu.get_species(0)
+    assert pg.shape[0] == part.shape[0]
+    assert np.all(pg[-2000:, 0] < cfg.xmn + 17.0 * cfg.dx)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [(PREC_F64, 1e-10), (PREC_F32, 1e-4)])
+def test_window_move_with_a_full_overflow_list(precision, tol):
+    """A window move right after the step in which the sheet overflowed: the
+    shift and the table injection share one re-layout, which must shift the
+    overflowed records (placed in the old box) and not the injected ones
+    (placed in the new box).  Three more steps, then against the oracle."""
+    from aladyn_amd import LPStep
+
+    cfg, orc, table, nin, part, ebf = _sheet_case()
+    targ = (table["xpt"][0], table["xpt"][-1] + cfg.dx)
+    print(f"window with overflow precision={precision} np={part.shape[0]}", flush=True)
+    win = OrcWindow()
+    p, e, jc = part.copy(order="F"), ebf.copy(order="F"), orc.zeros_current()
+    nptx = nin
+    witr, w_speed = 4, 1.0
+    with LPStep(cfg, precision, track_ids=True) as gpu:
+        gpu.put_species(0, part)
+        gpu.put_fields(ebf)
+        gpu.set_injection(0, table, nptx=nin, targ_in=targ[0], targ_end=targ[1])
+        assert gpu.lp_window_xshift(witr, 0, w_speed) == 0
+        orc.window_xshift(win, witr, 0, w_speed, e, p)
+        p = orc.lpf2_evolve(p, e, jc, initial_time=True)
+        gpu.lpf2_evolve(0, initial_time=True)
+        nsh, p = orc.window_xshift(win, witr, 1, w_speed, e, p)
+        k, c0, c1, nptx = orc.inject_select(table["xpt"], nptx, orc.g.xmx, *targ)
+        assert k > 0
+        p = orc.add_particles(p, table, c0, c1)
+        assert gpu.lp_window_xshift(witr, 1, w_speed) == nsh > 0
+        assert gpu.injection_state(0) == nptx
+        for it in range(1, 4):
+            p = orc.lpf2_evolve(p, e, jc)
+            gpu.lpf2_evolve(it)
+        pg, eg, jg = gpu.get_species(0), gpu.get_fields(), gpu.get_current()
+    assert pg.shape == p.shape, (pg.shape, p.shape)
+    D = cfg.curr_ndim
+    for d in range(D):
+        span = (cfg.xmx, cfg.ymx)[d] - (cfg.xmn, cfg.ymn)[d]
         assert np.max(np.abs(pg[:, d] - p[:, d])) / span <= tol
     assert rel_maxerr(pg[:, D:2 * D], p[:, D:2 * D]) <= tol
     assert np.array_equal(pg[:, 2 * D], p[:, 2 * D])
