@@ -270,13 +270,42 @@ __device__ inline EnvTile env_tile(const TileGeom& tg, int t) {
   return e;
 }
 
-// Stages component c of src (ncell stride) over the tile's box into dst.
+// Stages n0 components of a, then n1 components of b (ncell stride each)
+// over the tile's box into dst (component stride gvol).  The threads take
+// whole x rows of the box, kStageRows rows each per pass, and every load of
+// a pass is issued before its LDS stores, so a pass pays the HBM latency
+// once instead of once per node.
+static constexpr int kStageRows = 8;
 template <int D, typename R>
-__device__ inline void env_stage(const DevGrid& g, const EnvTile& e, const R* src, R* dst) {
-  for (int l = threadIdx.x; l < e.gvol; l += blockDim.x) {
-    const int li = l % e.G[0], lj = (l / e.G[0]) % e.G[1], lk = l / (e.G[0] * e.G[1]);
-    const int i = e.T0[0] - 2 + li, j = e.T0[1] - 2 + lj, k = D == 3 ? e.T0[2] - 2 + lk : 1;
-    dst[l] = (i <= g.n1p && j <= g.n2p && k <= g.n3p) ? src[fofs(g, i, j, k)] : R(0);
+__device__ inline void env_stage_rows(const DevGrid& g, const EnvTile& e, const R* a, int n0,
+                                      const R* b, int n1, R* dst) {
+  const int G0 = e.G[0], G1 = e.G[1], G12 = e.G[1] * e.G[2];
+  // rows per pass, one node per thread (the compiled tile shapes have
+  // G0 <= 35 < blockDim)
+  const int rpp = max(1, (int)blockDim.x / G0);
+  const int li = threadIdx.x % G0, r0 = threadIdx.x / G0;
+  const int rows = (n0 + n1) * G12;
+  const int i = e.T0[0] - 2 + li;
+  const bool xin = r0 < rpp && i <= g.n1p;
+  for (int rb = r0; rb < rows; rb += kStageRows * rpp) {
+    R v[kStageRows];
+#pragma unroll
+    for (int u = 0; u < kStageRows; u++) {
+      const int row = rb + u * rpp;
+      v[u] = R(0);
+      if (xin && row < rows) {
+        const int c = row / G12, rem = row - c * G12;
+        const int lk = rem / G1, lj = rem - lk * G1;
+        const int j = e.T0[1] - 2 + lj, k = D == 3 ? e.T0[2] - 2 + lk : 1;
+        const R* src = c < n0 ? comp_c(a, g, c) : comp_c(b, g, c - n0);
+        if (j <= g.n2p && k <= g.n3p) v[u] = src[fofs(g, i, j, k)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kStageRows; u++) {
+      const int row = rb + u * rpp;
+      if (r0 < rpp && row < rows) dst[row * G0 + li] = v[u];
+    }
   }
 }
 
@@ -312,24 +341,29 @@ __global__ __launch_bounds__(kEB) void k_env_push_tile(TileGeom tg, DevGrid g, R
   extern __shared__ double lds_d[];
   const int t = blockIdx.x;
   const EnvTile e = env_tile<D>(tg, t);
+  const int gvol = e.gvol, pvol = tg.pvol;
   const int nf = g.nfield, nc = nf + D + 1;
-  double* Pd = lds_d;                 // density patch, pvol
-  R* Fl = (R*)(lds_d + tg.pvol);      // nf ebf components, then je(1:D+1)
-  for (int l = threadIdx.x; l < tg.pvol; l += blockDim.x) Pd[l] = 0.0;
-  for (int c = 0; c < nf; c++) env_stage<D, R>(g, e, comp_c(ebf, g, c), Fl + c * e.gvol);
-  for (int c = 0; c <= D; c++) env_stage<D, R>(g, e, comp_c(je, g, c), Fl + (nf + c) * e.gvol);
+  double* Pd = lds_d;              // density patch, pvol
+  R* Fl = (R*)(lds_d + pvol);      // nf ebf components, then je(1:D+1)
+  const int n = min(cnt[t], seg_cap[t]);
+  const int64_t base = seg_start[t];
+  // the thread's first record is loaded with the staging loads
+  Rec<R> nxt = {};
+  if ((int)threadIdx.x < n) load_raw<D, R>(A, base + threadIdx.x, nxt);
+  for (int l = threadIdx.x; l < pvol; l += blockDim.x) Pd[l] = 0.0;
+  env_stage_rows<D, R>(g, e, ebf, nf, je, D + 1, Fl);
   __syncthreads();
   (void)nc;
   const int GX = e.G[0], GXY = e.G[0] * e.G[1];
   const R* EB = Fl;
-  const R* JE = Fl + nf * e.gvol;
-  const int n = min(cnt[t], seg_cap[t]);
-  const int64_t base = seg_start[t];
+  const R* JE = Fl + nf * gvol;
   for (int q0 = threadIdx.x; q0 < n; q0 += blockDim.x) {
     const int64_t src = base + q0;
-    if (rec_cell<R>(A, src) == kHole) continue;
-    Rec<R> q;
-    load_rec<D, R>(A, src, q, tg.bx, tg.by);
+    Rec<R> q = nxt;
+    // the next record's load is in flight during this one's push
+    if (q0 + (int)blockDim.x < n) load_raw<D, R>(A, src + blockDim.x, nxt);
+    if (q.N[0] == kHole) continue;
+    decode_cell<D, R>(q, tg.bx, tg.by);
     R ax[3], axh[3], ay[3], ayh[3], az[3] = {R(1), R(0), R(0)}, azh[3] = {R(1), R(0), R(0)};
     int hx, hy, hz = 0;
     qqh<R>(q.s[0], ax, axh, hx);
@@ -340,15 +374,15 @@ __global__ __launch_bounds__(kEB) void k_env_push_tile(TileGeom tg, DevGrid g, R
     const int ih = i + hx, jh = j + hy, kh = D == 3 ? k + hz : 0;
     const R one = R(1), half = R(0.5);
     const EnvLdsAcc<D> acc{Pd, {e.T0[0] - 2, e.T0[1] - 2, e.T0[2] - (D == 3 ? 2 : 0)},
-                           tg.J[0], tg.J[1], tg.pvol};
+                           tg.J[0], tg.J[1], pvol};
     if constexpr (D == 2) {
       R ap[6];
       ap[5] = contract3l<2, R>(JE, GX, GXY, ax, ay, az, i, j, 0);
       ap[0] = contract3l<2, R>(EB, GX, GXY, axh, ay, az, ih, j, 0);
-      ap[3] = contract3l<2, R>(JE + e.gvol, GX, GXY, axh, ay, az, ih, j, 0);
-      ap[1] = contract3l<2, R>(EB + e.gvol, GX, GXY, ax, ayh, az, i, jh, 0);
-      ap[4] = contract3l<2, R>(JE + 2 * e.gvol, GX, GXY, ax, ayh, az, i, jh, 0);
-      ap[2] = contract3l<2, R>(EB + 2 * e.gvol, GX, GXY, axh, ayh, az, ih, jh, 0);
+      ap[3] = contract3l<2, R>(JE + gvol, GX, GXY, axh, ay, az, ih, j, 0);
+      ap[1] = contract3l<2, R>(EB + gvol, GX, GXY, ax, ayh, az, i, jh, 0);
+      ap[4] = contract3l<2, R>(JE + 2 * gvol, GX, GXY, ax, ayh, az, i, jh, 0);
+      ap[2] = contract3l<2, R>(EB + 2 * gvol, GX, GXY, axh, ayh, az, ih, jh, 0);
       const R u1 = q.p[0], u2 = q.p[1];
       const R gam2 = one + u1 * u1 + u2 * u2 + ap[5];
       for (int c = 0; c < 3; c++) ap[c] = qch * ap[c];
@@ -375,14 +409,14 @@ __global__ __launch_bounds__(kEB) void k_env_push_tile(TileGeom tg, DevGrid g, R
       R ap[10];
       ap[9] = contract3l<3, R>(JE, GX, GXY, ax, ay, az, i, j, k);
       ap[0] = contract3l<3, R>(EB, GX, GXY, axh, ay, az, ih, j, k);
-      ap[6] = contract3l<3, R>(JE + e.gvol, GX, GXY, axh, ay, az, ih, j, k);
-      ap[1] = contract3l<3, R>(EB + e.gvol, GX, GXY, ax, ayh, az, i, jh, k);
-      ap[7] = contract3l<3, R>(JE + 2 * e.gvol, GX, GXY, ax, ayh, az, i, jh, k);
-      ap[5] = contract3l<3, R>(EB + 5 * e.gvol, GX, GXY, axh, ayh, az, ih, jh, k);
-      ap[3] = contract3l<3, R>(EB + 3 * e.gvol, GX, GXY, ax, ayh, azh, i, jh, kh);
-      ap[4] = contract3l<3, R>(EB + 4 * e.gvol, GX, GXY, axh, ay, azh, ih, j, kh);
-      ap[2] = contract3l<3, R>(EB + 2 * e.gvol, GX, GXY, ax, ay, azh, i, j, kh);
-      ap[8] = contract3l<3, R>(JE + 3 * e.gvol, GX, GXY, ax, ay, azh, i, j, kh);
+      ap[6] = contract3l<3, R>(JE + gvol, GX, GXY, axh, ay, az, ih, j, k);
+      ap[1] = contract3l<3, R>(EB + gvol, GX, GXY, ax, ayh, az, i, jh, k);
+      ap[7] = contract3l<3, R>(JE + 2 * gvol, GX, GXY, ax, ayh, az, i, jh, k);
+      ap[5] = contract3l<3, R>(EB + 5 * gvol, GX, GXY, axh, ayh, az, ih, jh, k);
+      ap[3] = contract3l<3, R>(EB + 3 * gvol, GX, GXY, ax, ayh, azh, i, jh, kh);
+      ap[4] = contract3l<3, R>(EB + 4 * gvol, GX, GXY, axh, ay, azh, ih, j, kh);
+      ap[2] = contract3l<3, R>(EB + 2 * gvol, GX, GXY, ax, ay, azh, i, j, kh);
+      ap[8] = contract3l<3, R>(JE + 3 * gvol, GX, GXY, ax, ay, azh, i, j, kh);
       const R u[3] = {q.p[0], q.p[1], q.p[2]};
       const R gam2 = one + u[0] * u[0] + u[1] * u[1] + u[2] * u[2] + ap[9];
       for (int c = 0; c < 6; c++) ap[c] = qch * ap[c];
@@ -430,7 +464,7 @@ __global__ __launch_bounds__(kEB) void k_env_move_tile(TileGeom tg, DevGrid g, S
   double* Jl = lds_d;                      // D current components, pvol each
   R* JE = (R*)(lds_d + D * tg.pvol);       // je(1:D+1)
   for (int l = threadIdx.x; l < D * tg.pvol; l += blockDim.x) Jl[l] = 0.0;
-  for (int c = 0; c <= D; c++) env_stage<D, R>(g, e, comp_c(je, g, c), JE + c * e.gvol);
+  env_stage_rows<D, R>(g, e, je, D + 1, je, 0, JE);
   __syncthreads();
   const int GX = e.G[0], GXY = e.G[0] * e.G[1];
   const EnvLdsAcc<D> acc{Jl, {e.T0[0] - 2, e.T0[1] - 2, e.T0[2] - (D == 3 ? 2 : 0)},
