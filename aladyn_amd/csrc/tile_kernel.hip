@@ -320,13 +320,19 @@ struct CellAcc {
 //            gamma of lpf_env_positions): je(1:D+1) = |A|^2/2 and its
 //            gradient over nodes T0-2 .. T0+T (the reach of the qqh shapes)
 // Everything after (removal, deposit, re-binning) is the same.
+// SPREAD: whether the lanes of a wave take slices of the sorted chunk far
+// enough apart that no two of them sit in one cell (fewer same-address LDS
+// atomics, more bank conflicts in the gather); without it thread me takes
+// slice me.
 struct LpPush {
   template <int D> static constexpr int NF = D == 3 ? 6 : 3;
   static constexpr int LO = 1, EXT = 2;  // patch nodes T0-LO .. T0-LO+T+EXT-1
+  static constexpr bool SPREAD = true;
 };
 struct EnvMove {
   template <int D> static constexpr int NF = D + 1;
   static constexpr int LO = 2, EXT = 3;
+  static constexpr bool SPREAD = false;
 };
 
 template <int D, int TX, int TY, int TZ, int NT, typename R, class P = LpPush>
@@ -653,7 +659,7 @@ __device__ inline void tile_body(const TileArgs& ta, const StepConst<R>& sc,
   for (int c0 = 0; c0 < cnt; c0 += kChunk) {
     __syncthreads();  // patches staged / previous chunk consumed
     if (me < NCL) offs[me] = 0;
-    if (me == 0) misc[2] = 0;
+    if (P::SPREAD && me == 0) misc[2] = 0;
     // local cell index, kHole for a hole, -2 for a record of another tile
 #pragma unroll
     for (int u = 0; u < CPT; u++) {
@@ -678,7 +684,7 @@ __device__ inline void tile_body(const TileArgs& ta, const StepConst<R>& sc,
         const int y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
       }
-      const int vmax = wave_max(v);
+      const int vmax = P::SPREAD ? wave_max(v) : 0;
       if (lane == 63) {
         wsum[me >> 6] = x;
         if (vmax > 0) atomicMax(&misc[2], vmax);
@@ -718,11 +724,13 @@ __device__ inline void tile_body(const TileArgs& ta, const StepConst<R>& sc,
     // on different cells when no cell holds more than sigma*W (no two lanes
     // of an LDS atomic on one address), and on nearby cells when sigma is as
     // small as that allows (stencil addresses spread over the LDS banks).
-    // sigma is the smallest such stride for the fullest cell of the chunk.
+    // sigma is the smallest such stride for the fullest cell of the chunk
+    // (1 for a policy without SPREAD).
     constexpr int NW = S::NW;
     const int W = (nval + NT - 1) / NT;
     int sigma = 1;
-    while (sigma < NW && sigma * W < misc[2]) sigma <<= 1;
+    if constexpr (P::SPREAD)
+      while (sigma < NW && sigma * W < misc[2]) sigma <<= 1;
     const int wv = me >> 6;
     const int sl = (wv / sigma) * (64 * sigma) + (wv % sigma) + sigma * lane;
     const int q0 = sl * W, q1 = min(q0 + W, nval);
