@@ -342,7 +342,7 @@ __global__ __launch_bounds__(kEB) void k_env_push_tile(TileGeom tg, DevGrid g, R
   const int t = blockIdx.x;
   const EnvTile e = env_tile<D>(tg, t);
   const int gvol = e.gvol, pvol = tg.pvol;
-  const int nf = g.nfield, nc = nf + D + 1;
+  const int nf = g.nfield;
   double* Pd = lds_d;              // density patch, pvol
   R* Fl = (R*)(lds_d + pvol);      // nf ebf components, then je(1:D+1)
   const int n = min(cnt[t], seg_cap[t]);
@@ -353,7 +353,6 @@ __global__ __launch_bounds__(kEB) void k_env_push_tile(TileGeom tg, DevGrid g, R
   for (int l = threadIdx.x; l < pvol; l += blockDim.x) Pd[l] = 0.0;
   env_stage_rows<D, R>(g, e, ebf, nf, je, D + 1, Fl);
   __syncthreads();
-  (void)nc;
   const int GX = e.G[0], GXY = e.G[0] * e.G[1];
   const R* EB = Fl;
   const R* JE = Fl + nf * gvol;
