@@ -639,7 +639,11 @@ __device__ inline void tile_body(const TileArgs& ta, const StepConst<R>& sc,
       if (r0 < RPP && row < ROWS) Fl[row * GX + li] = fs * fv[u];
     }
   }
-  for (int q = threadIdx.x; q < D * PVOL; q += NT) Jl[q] = 0.0;
+  // the FP64 patch moves as 16-byte pairs (D*PVOL is even: every patch
+  // extent T+4 is)
+  static_assert((D * PVOL) % 2 == 0, "patch of whole double pairs");
+  for (int q = threadIdx.x; q < D * PVOL / 2; q += NT)
+    reinterpret_cast<double2*>(Jl)[q] = make_double2(0.0, 0.0);
 
   const WaveQueue<R, QCAP> qu(Q + (me >> 6) * S::QW_BYTES);
   int qcount = 0;  // wave-uniform: entries 0 .. qcount-1 are waiting
@@ -917,7 +921,8 @@ __device__ inline void tile_body(const TileArgs& ta, const StepConst<R>& sc,
   if (acc_lc >= 0) acc.template flush<JX, JY, PVOL>(patch_node(acc_lc));
   __syncthreads();
   double* Pt = patch + (int64_t)t * D * PVOL;
-  for (int q = threadIdx.x; q < D * PVOL; q += NT) Pt[q] = Jl[q];
+  for (int q = threadIdx.x; q < D * PVOL / 2; q += NT)
+    reinterpret_cast<double2*>(Pt)[q] = reinterpret_cast<const double2*>(Jl)[q];
 }
 
 template <int D, int TX, int TY, int TZ, int NT, typename R, class P = LpPush>
