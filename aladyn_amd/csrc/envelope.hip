@@ -871,8 +871,7 @@ int env_particles_move(TileStore* ts, const TileGeom& tg, const DevGrid& g, cons
     sc.nn[1] = g.ny;
     sc.nn[2] = g.nz;
     sc.ovf_base = ts->ovf_base;
-    fill<int>(ts->count[nxt], tg.ntiles, 0, s);
-    fill<int>(ts->ovf_count + nxt, 1, 0, s);
+    tile_store_clear_next(ts, tg, s);
     const size_t lds = env_move_lds<R>(tg, g);
     if (ev_begin) (void)hipEventRecord(ev_begin, s);
     // every tile writes its whole patch; the overflow list adds to them.

@@ -953,6 +953,17 @@ static int relayout_seg(TileStore* ts, const TileGeom& tg, hipStream_t s, std::s
   return 0;
 }
 
+__global__ __launch_bounds__(kB) void k_clear_next(int* __restrict__ cnt, int n,
+                                                   int* __restrict__ ovf) {
+  for (int t = blockIdx.x * kB + threadIdx.x; t < n; t += gridDim.x * kB) cnt[t] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ovf = 0;
+}
+
+void tile_store_clear_next(TileStore* ts, const TileGeom& tg, hipStream_t s) {
+  const int nxt = 1 - ts->cur;
+  k_clear_next<<<grid_for(tg.ntiles), kB, 0, s>>>(ts->count[nxt], tg.ntiles, ts->ovf_count + nxt);
+}
+
 int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hipStream_t s,
                         std::string* msg) {
   if (!ts->loaded) return 0;
@@ -1252,8 +1263,7 @@ int tile_store_step_enqueue(TileStore* ts, const TileGeom& tg, const DevGrid& g,
   sc.nn[2] = g.nz;
   sc.initial = initial;
   sc.ovf_base = ts->ovf_base;
-  fill<int>(ts->count[nxt], tg.ntiles, 0, s);
-  fill<int>(ts->ovf_count + nxt, 1, 0, s);
+  tile_store_clear_next(ts, tg, s);
   if (ev_begin) TS_CK(hipEventRecord(ev_begin, s));
   TS_CK(launch_tile_kernel<R>(tg, g, sc, ebf, ts->buf[cur], ts->buf[nxt], ts->seg_start,
                               ts->seg_cap, ts->count[cur], ts->count[nxt], ts->ovf_count + cur,

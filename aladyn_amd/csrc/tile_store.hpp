@@ -210,6 +210,9 @@ int tile_store_relayout(TileStore* ts, const TileGeom& tg, const DevGrid& g, hip
 template <typename R>
 void tile_store_fold(const TileGeom& tg, const DevGrid& g, const double* patch, R* jc,
                      int accumulate, hipStream_t s, bool normalise = false);
+// The counters a step's particle kernels fill (the next buffer's tile counts
+// and its overflow count) set to zero, in one launch.
+void tile_store_clear_next(TileStore* ts, const TileGeom& tg, hipStream_t s);
 // den = the sum of component 0 of the per-tile patches (a density)
 void tile_store_fold_density(const TileGeom& tg, const DevGrid& g, const double* patch,
                              double* den, hipStream_t s);
