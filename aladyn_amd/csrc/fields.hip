@@ -107,8 +107,78 @@ __global__ void kb_field_zbd(const FieldRun<T>* runs) {
   b_field_zbd<T>(r.g, r.ebf, r.g.nfield);
 }
 
+// field_xyzbd with open boundaries on every axis in one launch: one thread
+// per ghost node the three face kernels above write (the x ghosts at interior
+// j, k; the y ghosts over the x-extended i range at interior k; the z ghosts
+// over the x- and y-extended ranges), which forms its value from interior
+// nodes in the order the face kernels do (x extrapolations, then y, then
+// z), so the result is the same to the bit and no thread reads a node
+// another writes.
+template <typename T>
+__device__ inline T ghost_x(const T* f, const DevGrid& g, int i, int j, int k) {
+  if (i == g.ix1 - 1) return T(2) * f[fofs(g, g.ix1, j, k)] - f[fofs(g, g.ix1 + 1, j, k)];
+  if (i == g.ix2 + 1) return T(2) * f[fofs(g, g.ix2, j, k)] - f[fofs(g, g.ix2 - 1, j, k)];
+  return f[fofs(g, i, j, k)];
+}
+template <typename T>
+__device__ inline T ghost_xy(const T* f, const DevGrid& g, int i, int j, int k) {
+  if (j == g.jy1 - 1) return T(2) * ghost_x(f, g, i, g.jy1, k) - ghost_x(f, g, i, g.jy1 + 1, k);
+  if (j == g.jy2 + 1) return T(2) * ghost_x(f, g, i, g.jy2, k) - ghost_x(f, g, i, g.jy2 - 1, k);
+  return ghost_x(f, g, i, j, k);
+}
+template <typename T>
+__global__ __launch_bounds__(kFieldBlock) void k_field_bd_open(DevGrid g, T* ebf, int nc) {
+  const int ni = g.ix2 - g.ix1 + 1, nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1;
+  const int64_t nxs = 2LL * nj * nk, nys = g.ndim > 1 ? 2LL * (ni + 2) * nk : 0;
+  const int64_t nzs = g.ndim > 2 ? 2LL * (ni + 2) * (nj + 2) : 0;
+  const int64_t per = nxs + nys + nzs;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < per * nc;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    T* f = comp(ebf, g, 1 + (int)(t / per));
+    int64_t r = t % per;
+    int i, j, k;
+    T v;
+    if (r < nxs) {  // x ghosts: (ix1-1 | ix2+1, j, k)
+      const int side = (int)(r / ((int64_t)nj * nk));
+      r -= (int64_t)side * nj * nk;
+      j = g.jy1 + (int)(r % nj);
+      k = g.kz1 + (int)(r / nj);
+      i = side == 0 ? g.ix1 - 1 : g.ix2 + 1;
+      v = ghost_x(f, g, i, j, k);
+    } else if ((r -= nxs) < nys) {  // y ghosts over i = ix1-1 .. ix2+1
+      const int side = (int)(r / ((int64_t)(ni + 2) * nk));
+      r -= (int64_t)side * (ni + 2) * nk;
+      i = g.ix1 - 1 + (int)(r % (ni + 2));
+      k = g.kz1 + (int)(r / (ni + 2));
+      j = side == 0 ? g.jy1 - 1 : g.jy2 + 1;
+      v = ghost_xy(f, g, i, j, k);
+    } else {  // z ghosts over i = ix1-1 .. ix2+1, j = jy1-1 .. jy2+1
+      r -= nys;
+      const int side = (int)(r / ((int64_t)(ni + 2) * (nj + 2)));
+      r -= (int64_t)side * (ni + 2) * (nj + 2);
+      i = g.ix1 - 1 + (int)(r % (ni + 2));
+      j = g.jy1 - 1 + (int)(r / (ni + 2));
+      if (side == 0) {
+        k = g.kz1 - 1;
+        v = T(2) * ghost_xy(f, g, i, j, g.kz1) - ghost_xy(f, g, i, j, g.kz1 + 1);
+      } else {
+        k = g.kz2 + 1;
+        v = T(2) * ghost_xy(f, g, i, j, g.kz2) - ghost_xy(f, g, i, j, g.kz2 - 1);
+      }
+    }
+    f[fofs(g, i, j, k)] = v;
+  }
+}
+
 template <typename T>
 void field_xyzbd(const DevGrid& g, T* ebf, int nc, hipStream_t s) {
+  if (g.ibx == 0 && g.iby == 0 && (g.ndim < 3 || g.ibz == 0) && g.ndim > 1) {
+    const int64_t ni = g.ix2 - g.ix1 + 1, nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1;
+    const int64_t n = (2 * nj * nk + 2 * (ni + 2) * nk +
+                       (g.ndim > 2 ? 2 * (ni + 2) * (nj + 2) : 0)) * nc;
+    k_field_bd_open<T><<<nblocks(n), kFieldBlock, 0, s>>>(g, ebf, nc);
+    return;
+  }
   int64_t nx = (int64_t)(g.jy2 - g.jy1 + 1) * (g.kz2 - g.kz1 + 1) * nc;
   k_field_xbd<T><<<nblocks(nx), kFieldBlock, 0, s>>>(g, ebf, nc);
   if (g.ndim < 2) return;
@@ -218,8 +288,100 @@ __device__ inline void b_jc_scale(DevGrid g, T* jc) {
 template <typename T>
 __global__ void k_jc_scale(DevGrid g, T* jc) { b_jc_scale<T>(g, jc); }
 
+// jc_xyzbd's three folds in one launch: one thread per node on the faces of
+// the interior box (x faces whole; y faces without the x-face nodes; z faces
+// without either), which adds the ghosts of every fold that reaches it in
+// the order of the three kernels above (x, then y, then z) and zeroes them.
+// Each ghost is read by one such node only, so no thread reads what another
+// writes, and the sums are the same to the bit.
+template <typename T>
+__global__ __launch_bounds__(kFieldBlock) void k_jc_fold_all(DevGrid g, T* jc, int fx, int fy,
+                                                             int fz) {
+  const int ni = g.ix2 - g.ix1 + 1, nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1;
+  const int xi = ni > 1 ? 2 : 1, yj = nj > 1 ? 2 : 1, zk = g.ndim > 2 && nk > 1 ? 2 : 1;
+  const int mi = max(ni - 2, 0), mj = max(nj - 2, 0);
+  const int64_t nxs = (int64_t)xi * nj * nk, nys = (int64_t)yj * mi * nk;
+  const int64_t nzs = g.ndim > 2 ? (int64_t)zk * mi * mj : 0;
+  const int64_t per = nxs + nys + nzs;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < per * g.curr_ndim;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    T* c = comp(jc, g, 1 + (int)(t / per));
+    int64_t r = t % per;
+    int i, j, k;
+    if (r < nxs) {  // x faces, i = ix1 | ix2
+      const int side = (int)(r / ((int64_t)nj * nk));
+      r -= (int64_t)side * nj * nk;
+      i = side == 0 ? g.ix1 : g.ix2;
+      j = g.jy1 + (int)(r % nj);
+      k = g.kz1 + (int)(r / nj);
+    } else if ((r -= nxs) < nys) {  // y faces, j = jy1 | jy2, ix1 < i < ix2
+      const int side = (int)(r / ((int64_t)mi * nk));
+      r -= (int64_t)side * mi * nk;
+      j = side == 0 ? g.jy1 : g.jy2;
+      i = g.ix1 + 1 + (int)(r % mi);
+      k = g.kz1 + (int)(r / mi);
+    } else {  // z faces, k = kz1 | kz2, interior of the y and x faces
+      r -= nys;
+      const int side = (int)(r / ((int64_t)mi * mj));
+      r -= (int64_t)side * mi * mj;
+      k = side == 0 ? g.kz1 : g.kz2;
+      i = g.ix1 + 1 + (int)(r % mi);
+      j = g.jy1 + 1 + (int)(r / mi);
+    }
+    const int64_t o = fofs(g, i, j, k);
+    T v = c[o];
+    const int64_t sy = g.n1p, sz = (int64_t)g.n1p * g.n2p;
+    if (fx) {
+      if (i == g.ix1) {
+        v = v + c[o - 1];
+        c[o - 1] = T(0);
+      }
+      if (i == g.ix2) {
+        v = v + c[o + 1] + c[o + 2];
+        c[o + 1] = T(0);
+        c[o + 2] = T(0);
+      }
+    }
+    if (fy) {
+      if (j == g.jy1) {
+        v = v + c[o - sy] + c[o - 2 * sy];
+        c[o - 2 * sy] = T(0);
+        c[o - sy] = T(0);
+      }
+      if (j == g.jy2) {
+        v = v + c[o + sy] + c[o + 2 * sy];
+        c[o + sy] = T(0);
+        c[o + 2 * sy] = T(0);
+      }
+    }
+    if (fz) {
+      if (k == g.kz1) {
+        v = v + c[o - sz];
+        c[o - sz] = T(0);
+      }
+      if (k == g.kz2) {
+        v = v + c[o + sz] + c[o + 2 * sz];
+        c[o + sz] = T(0);
+        c[o + 2 * sz] = T(0);
+      }
+    }
+    c[o] = v;
+  }
+}
+
 template <typename T>
 void curr_collect(const DevGrid& g, T* jc, hipStream_t s, bool normalised) {
+  if (g.ndim > 1) {
+    const int fx = g.ibx == 0, fy = g.iby == 0, fz = g.ndim > 2 && g.ibz == 0;
+    const int64_t ni = g.ix2 - g.ix1 + 1, nj = g.jy2 - g.jy1 + 1, nk = g.kz2 - g.kz1 + 1;
+    const int64_t mi = std::max<int64_t>(ni - 2, 0), mj = std::max<int64_t>(nj - 2, 0);
+    const int64_t n = ((ni > 1 ? 2 : 1) * nj * nk + (nj > 1 ? 2 : 1) * mi * nk +
+                       (g.ndim > 2 ? (nk > 1 ? 2 : 1) * mi * mj : 0)) *
+                      g.curr_ndim;
+    if (fx || fy || fz) k_jc_fold_all<T><<<nblocks(n), kFieldBlock, 0, s>>>(g, jc, fx, fy, fz);
+    if (!normalised) k_jc_scale<T><<<nblocks(g.ncell), kFieldBlock, 0, s>>>(g, jc);
+    return;
+  }
   if (g.ibx == 0) {
     int64_t n = (int64_t)(g.jy2 - g.jy1 + 1) * (g.kz2 - g.kz1 + 1) * g.curr_ndim;
     k_jc_xfold<T><<<nblocks(n), kFieldBlock, 0, s>>>(g, jc);
